@@ -1,0 +1,997 @@
+// mc.hip -- multichannel VEGAS on gfx950: kernels, device context and the
+// chili_mc_* / chili_construct_channels entry points of include/chili_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "mc_kernels.hpp"
+
+using namespace chili_dev;
+using chili_host::ChanProg;
+
+namespace {
+
+// Record layout of one workgroup partial: [n, min, max, sum, sum2, nf, nnan,
+// T_0..T_{C-1}, hist d x B of the workgroup's channel].
+constexpr int kStatW = 7;
+
+// ---------------------------------------------------------------------------
+// k_pick: channel of every attempt; per-(channel, block) counts, channel-major.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTile) void k_pick(MCArgs A, int64_t n, uint16_t* __restrict__ pick, int* __restrict__ counts, int nblk) {
+    extern __shared__ int s_cnt[];
+    for (int c = threadIdx.x; c < A.C; c += blockDim.x) s_cnt[c] = 0;
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (i < n) {
+        int c = pick_channel(A, channel_uniform(A, A.a0 + (uint64_t)i));
+        pick[i] = (uint16_t)c;
+        atomicAdd(&s_cnt[c], 1);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < A.C; c += blockDim.x) counts[(int64_t)c * nblk + blockIdx.x] = s_cnt[c];
+}
+
+// ---------------------------------------------------------------------------
+// Single-workgroup exclusive scan of an int array (len up to a few million);
+// out may alias in.  total receives the sum.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int* out, int64_t len, int64_t* total) {
+    __shared__ int64_t s_part[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (len + 1023) / 1024;
+    const int64_t b = t * chunk, e = std::min<int64_t>(len, b + chunk);
+    int64_t sum = 0;
+    for (int64_t i = b; i < e; ++i) sum += in[i];
+    s_part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = t >= off ? s_part[t - off] : 0;
+        __syncthreads();
+        s_part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = s_part[t] - sum;
+    for (int64_t i = b; i < e; ++i) {
+        int v = in[i];
+        out[i] = (int)run;
+        run += v;
+    }
+    if (t == 1023 && total) *total = s_part[1023];
+}
+
+// Per-wave stable ranks of equal keys: rank among earlier lanes with the same
+// key, plus the per-key lane count written by the key's first lane.
+__device__ __forceinline__ int wave_rank(int key, bool valid, int* wave_count_row) {
+    const int lane = threadIdx.x & 63;
+    uint64_t rem = __ballot(valid);
+    int rank = 0;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    while (rem) {
+        int leader = __ffsll((unsigned long long)rem) - 1;
+        int k0 = __shfl(key, leader);
+        uint64_t m = __ballot(valid && key == k0);
+        if (valid && key == k0) rank = __popcll(m & lt);
+        if (lane == leader) wave_count_row[k0] = __popcll(m);
+        rem &= ~m;
+    }
+    return rank;
+}
+
+// ---------------------------------------------------------------------------
+// k_scatter: stable counting sort of attempt indices by channel.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTile) void k_scatter(int C, int64_t n, const uint16_t* __restrict__ pick, const int* __restrict__ offs,
+                                                   int nblk, uint32_t* __restrict__ sorted) {
+    extern __shared__ int s_wc[];   // [kWaves][C]
+    for (int i = threadIdx.x; i < kWaves * C; i += blockDim.x) s_wc[i] = 0;
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    bool valid = i < n;
+    int c = valid ? pick[i] : 0;
+    int rank = wave_rank(c, valid, s_wc + w * C);
+    __syncthreads();
+    if (valid) {
+        int before = 0;
+        for (int ww = 0; ww < w; ++ww) before += s_wc[ww * C + c];
+        sorted[offs[(int64_t)c * nblk + blockIdx.x] + before + rank] = (uint32_t)i;
+    }
+}
+
+// Channel segment table: seg_start[c] (entries), tile_start[c] (tiles of kTile).
+__global__ void k_segments(int C, const int* __restrict__ offs, int nblk, int64_t total, int* seg_start, int* tile_start) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int t = 0;
+    for (int c = 0; c < C; ++c) {
+        int s = offs[(int64_t)c * nblk];
+        int e = (c + 1 < C) ? offs[(int64_t)(c + 1) * nblk] : (int)total;
+        seg_start[c] = s;
+        tile_start[c] = t;
+        t += (e - s + kTile - 1) / kTile;
+    }
+    seg_start[C] = (int)total;
+    tile_start[C] = t;
+}
+
+__device__ __forceinline__ bool locate_tile(int C, const int* __restrict__ seg_start, const int* __restrict__ tile_start, int tile,
+                                            int& ch, int& e0, int& e1) {
+    if (tile >= tile_start[C]) return false;
+    int lo = 0, hi = C - 1;
+    while (lo < hi) {            // last c with tile_start[c] <= tile
+        int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= tile) lo = mid; else hi = mid - 1;
+    }
+    ch = lo;
+    e0 = seg_start[lo] + (tile - tile_start[lo]) * kTile;
+    e1 = std::min(e0 + kTile, seg_start[lo + 1]);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// k_forward: VEGAS forward map, FSMapper::GeneratePoint and the cut for one
+// channel per workgroup.  Momenta of external legs go to mom[(i*4+k)*nmax + e].
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restrict__ seg_start, const int* __restrict__ tile_start,
+                                                   const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,
+                                                   uint8_t* __restrict__ acc, PointRec R) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    int ch, e0, e1;
+    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;
+    const int e = e0 + threadIdx.x;
+    if (e >= e1) return;
+    const uint32_t a = sorted[e];
+    double r[D + 1];
+    load_uniforms<D>(A, A.a0 + a, r);
+    // VEGAS forward map on the picked channel's grid (AdaptiveMap.cc:40-56);
+    // the multichannel path drops its Jacobian (Integrand.hh:107-110).
+    const double* __restrict__ edges = A.edges + (int64_t)ch * D * (A.B + 1);
+    const double Bd = (double)A.B;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double pos = r[i] * Bd;
+        int idx = (int)pos;
+        double loc = pos - (double)idx;
+        const double* ed = edges + i * (A.B + 1);
+        double left = ed[idx];
+        double size = ed[idx + 1] - ed[idx];
+        r[i] = left + loc * size;
+    }
+    const ChanProg& p = A.progs[ch];
+    V4 P[2 * NT];
+    double m2[2 * NT];
+    generate_point<NT>(A, p, r, P, m2);
+    bool ok = cut_pass<NT>(A.f, P, A.sqrts);
+    acc[a] = ok ? 1 : 0;
+    if (R.status) {
+        R.status[a] = ok ? 1 : 0;
+        R.pick[a] = ch;
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            R.mom[(int64_t)(i * 4 + 0) * R.n + a] = P[i].t;
+            R.mom[(int64_t)(i * 4 + 1) * R.n + a] = P[i].x;
+            R.mom[(int64_t)(i * 4 + 2) * R.n + a] = P[i].y;
+            R.mom[(int64_t)(i * 4 + 3) * R.n + a] = P[i].z;
+        }
+    }
+    if (ok) {
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            mom[(int64_t)(i * 4 + 0) * nmax + e] = P[i].t;
+            mom[(int64_t)(i * 4 + 1) * nmax + e] = P[i].x;
+            mom[(int64_t)(i * 4 + 2) * nmax + e] = P[i].y;
+            mom[(int64_t)(i * 4 + 3) * nmax + e] = P[i].z;
+        }
+    }
+}
+
+// Accepted attempts per block of kTile attempts (attempt order).
+__global__ __launch_bounds__(kTile) void k_acc_count(const uint8_t* __restrict__ acc, int64_t n, int* __restrict__ out) {
+    __shared__ int s[kWaves];
+    int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    bool v = i < n && acc[i];
+    uint64_t m = __ballot(v);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kWaves; ++w) t += s[w];
+        out[blockIdx.x] = t;
+    }
+}
+
+// Kept entries (accepted and below the cutoff) per forward tile.
+__global__ __launch_bounds__(kTile) void k_keep_count(int C, const int* __restrict__ seg_start, const int* __restrict__ tile_start,
+                                                      const uint32_t* __restrict__ sorted, const uint8_t* __restrict__ acc, uint32_t cut,
+                                                      int* __restrict__ kept) {
+    __shared__ int s[kWaves];
+    int ch, e0, e1;
+    if (!locate_tile(C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) {
+        if (threadIdx.x == 0) kept[blockIdx.x] = 0;
+        return;
+    }
+    const int e = e0 + threadIdx.x;
+    bool k = false;
+    if (e < e1) { uint32_t a = sorted[e]; k = a < cut && acc[a]; }
+    uint64_t m = __ballot(k);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kWaves; ++w) t += s[w];
+        kept[blockIdx.x] = t;
+    }
+}
+
+// Stable compaction of the kept entries (their sorted positions).
+__global__ __launch_bounds__(kTile) void k_keep_scatter(int C, const int* __restrict__ seg_start, const int* __restrict__ tile_start,
+                                                        const uint32_t* __restrict__ sorted, const uint8_t* __restrict__ acc, uint32_t cut,
+                                                        const int* __restrict__ kept_off, uint32_t* __restrict__ comp) {
+    __shared__ int s[kWaves];
+    int ch, e0, e1;
+    if (!locate_tile(C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;
+    const int e = e0 + threadIdx.x;
+    bool k = false;
+    if (e < e1) { uint32_t a = sorted[e]; k = a < cut && acc[a]; }
+    uint64_t m = __ballot(k);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s[w] = __popcll(m);
+    __syncthreads();
+    if (k) {
+        int before = 0;
+        for (int ww = 0; ww < w; ++ww) before += s[ww];
+        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        comp[kept_off[blockIdx.x] + before + __popcll(m & lt)] = (uint32_t)e;
+    }
+}
+
+// Segments of the compacted list: per channel, kept_off at its first tile.
+__global__ void k_keep_segments(int C, const int* __restrict__ tile_start, const int* __restrict__ kept_off, int64_t kept_total,
+                                int* seg2, int* tile2) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int t = 0;
+    for (int c = 0; c < C; ++c) {
+        // tiles at or past tile_start[C] belong to no channel: the segment runs to the end
+        int s = (tile_start[c] >= tile_start[C]) ? (int)kept_total : kept_off[tile_start[c]];
+        int e = (c + 1 < C && tile_start[c + 1] < tile_start[C]) ? kept_off[tile_start[c + 1]] : (int)kept_total;
+        seg2[c] = s;
+        tile2[c] = t;
+        t += (e - s + kTile - 1) / kTile;
+    }
+    seg2[C] = (int)kept_total;
+    tile2[C] = t;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) { double o = __shfl_xor(v, off); v = (o < v) ? o : v; }
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) { double o = __shfl_xor(v, off); v = (v < o) ? o : v; }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// k_main: for every kept point of the workgroup's channel: f, all C inverse
+// maps and VEGAS densities, W = 1 / sum_j alpha_j / ps_j / vegas_j, the value
+// val = W == 0 ? 0 : f W (NaN -> 0, counted), and the workgroup partials of
+// StatsData, alpha-training T_j += val^2 W / ps_j and the picked channel's
+// f^2 histogram (Integrand.hh:89-123, MultiChannel.hh:167-180, 224-247).
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict__ seg2, const int* __restrict__ tile2,
+                                                const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
+                                                double* __restrict__ psbuf, int64_t psstride, int train,
+                                                double* __restrict__ part, int64_t rec, double* __restrict__ vals_out, int64_t vals_off,
+                                                int64_t vals_cap, const uint32_t* __restrict__ sorted, PointRec R) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    extern __shared__ double s_mem[];
+    // LDS: hist [D][B] doubles, per-wave sums [kWaves][8 + C] doubles, bins [D][kTile] shorts
+    const int B = A.B, C = A.C;
+    double* s_hist = s_mem;
+    double* s_small = s_hist + D * B;
+    short* s_bins = (short*)(s_small + kWaves * (8 + C));
+    int ch, e0, e1;
+    if (!locate_tile(C, seg2, tile2, blockIdx.x, ch, e0, e1)) return;
+    const int e = e0 + threadIdx.x;
+    const bool valid = e < e1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t gt = (int64_t)blockIdx.x * kTile + threadIdx.x;
+
+    V4 P[2 * NT];
+    double val = 0, W = 0, fv = 0;
+    bool isnanv = false;
+    uint32_t att = 0;
+    if (valid) {
+        const uint32_t sp = comp[e];
+        if (R.status) att = sorted[sp];
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            P[i].t = mom[(int64_t)(i * 4 + 0) * nmax + sp];
+            P[i].x = mom[(int64_t)(i * 4 + 1) * nmax + sp];
+            P[i].y = mom[(int64_t)(i * 4 + 2) * nmax + sp];
+            P[i].z = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+        }
+        fv = integrand<NT>(A.f, P);
+    } else {
+#pragma unroll
+        for (int i = 0; i < NT; ++i) P[i] = mk(1., 0., 0., 1.);
+    }
+    // Integrand::GenerateWeight (Integrand.hh:112-123)
+    double wsum = 0;
+    for (int j = 0; j < C; ++j) {
+        double vw;
+        short* bo = (j == ch) ? (s_bins + threadIdx.x) : nullptr;
+        const bool recx = R.status && valid && j == ch;
+        double ps = generate_weight<NT>(A, A.progs[j], j, P, &vw, bo, kTile, recx ? R.x + att : nullptr,
+                                        recx ? R.bins + att : nullptr, R.n);
+        wsum += A.alpha[j] / ps / vw;
+        psbuf[(int64_t)j * psstride + gt] = ps;
+        if (R.status && valid) R.ps[(int64_t)j * R.n + att] = ps;
+    }
+    if (valid) {
+        W = 1.0 / wsum;
+        val = (W == 0) ? 0.0 : fv * W;
+        isnanv = isnan(val);
+        if (isnanv) { val = 0; W = 0; }
+        if (vals_out && vals_off + e < vals_cap) vals_out[vals_off + e] = val;
+        if (R.status) {
+            R.status[att] = isnanv ? 2 : 1;
+            R.w[att] = W;
+            R.f[att] = fv;
+            R.val[att] = val;
+        }
+    }
+    // ---- StatsData partial (Statistics.hh:92-102) ----
+    const double v2 = val * val;
+    {
+        double n = wave_sum(valid ? 1.0 : 0.0);
+        double s1 = wave_sum(val);
+        double s2 = wave_sum(v2);
+        double nf = wave_sum((valid && val != 0) ? 1.0 : 0.0);
+        double nn = wave_sum(isnanv ? 1.0 : 0.0);
+        double mn = wave_min(valid ? val : std::numeric_limits<double>::max());
+        double mx = wave_max(valid ? val : std::numeric_limits<double>::min());
+        if (lane == 0) {
+            double* q = s_small + w * (8 + C);
+            q[0] = n; q[1] = mn; q[2] = mx; q[3] = s1; q[4] = s2; q[5] = nf; q[6] = nn;
+        }
+    }
+    // ---- alpha training T_j (MultiChannel.hh:174-177) ----
+    if (train) {
+        for (int j = 0; j < C; ++j) {
+            double t = valid ? v2 * W / psbuf[(int64_t)j * psstride + gt] : 0.0;
+            t = wave_sum(t);
+            if (lane == 0) s_small[w * (8 + C) + 8 + j] = t;
+        }
+    }
+    double* out = part + (int64_t)blockIdx.x * rec;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc[7] = {0, std::numeric_limits<double>::max(), std::numeric_limits<double>::min(), 0, 0, 0, 0};
+        for (int ww = 0; ww < kWaves; ++ww) {
+            const double* q = s_small + ww * (8 + C);
+            acc[0] += q[0]; acc[1] = (q[1] < acc[1]) ? q[1] : acc[1]; acc[2] = (acc[2] < q[2]) ? q[2] : acc[2];
+            acc[3] += q[3]; acc[4] += q[4]; acc[5] += q[5]; acc[6] += q[6];
+        }
+        for (int k = 0; k < kStatW; ++k) out[k] = acc[k];
+    }
+    if (!train) return;
+    for (int j = threadIdx.x; j < C; j += kTile) {
+        double s = 0;
+        for (int ww = 0; ww < kWaves; ++ww) s += s_small[ww * (8 + C) + 8 + j];
+        out[kStatW + j] = s;
+    }
+    // ---- f^2 histogram of the picked channel (Integrand.hh:89-96) ----
+    // Lane l of a wave owns bins l, l+64, ... and adds the wave's values in
+    // lane order; the waves then add into LDS one after the other.
+    for (int k = 0; k < D; ++k) {
+        const int bk = valid ? s_bins[k * kTile + threadIdx.x] : 0;
+        const double vk = valid ? v2 : 0.0;
+        double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+        for (int src = 0; src < 64; ++src) {
+            const int b = __builtin_amdgcn_readlane(bk, src);
+            const double v = readlane_f64(vk, src);
+            if ((b & 63) == lane) {
+                const int hi = b >> 6;
+                if (hi == 0) h0 += v; else if (hi == 1) h1 += v; else if (hi == 2) h2 += v; else h3 += v;
+            }
+        }
+        for (int ww = 0; ww < kWaves; ++ww) {
+            if (w == ww) {
+                double hq[4] = {h0, h1, h2, h3};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int b = lane + 64 * q;
+                    if (b < B) s_hist[k * B + b] = (ww == 0) ? hq[q] : s_hist[k * B + b] + hq[q];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < D * B; i += kTile) out[kStatW + C + i] = s_hist[i];
+}
+
+// ---------------------------------------------------------------------------
+// k_reduce: fixed-order sums of the workgroup partials into the running
+// totals: tot = [stats(7), T(C), hist(C x d x B)].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reduce(int C, int DB, const int* __restrict__ tile2, const double* __restrict__ part, int64_t rec,
+                                                double* __restrict__ tot, int train) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ntile = tile2[C];
+    if (i == 0) {
+        double n = tot[0], mn = tot[1], mx = tot[2], s1 = tot[3], s2 = tot[4], nf = tot[5], nn = tot[6];
+        for (int t = 0; t < ntile; ++t) {
+            const double* q = part + (int64_t)t * rec;
+            n += q[0]; mn = (q[1] < mn) ? q[1] : mn; mx = (mx < q[2]) ? q[2] : mx; s1 += q[3]; s2 += q[4]; nf += q[5]; nn += q[6];
+        }
+        tot[0] = n; tot[1] = mn; tot[2] = mx; tot[3] = s1; tot[4] = s2; tot[5] = nf; tot[6] = nn;
+        return;
+    }
+    if (!train) return;
+    const int64_t j = i - 1;
+    if (j < C) {
+        double s = tot[kStatW + j];
+        for (int t = 0; t < ntile; ++t) s += part[(int64_t)t * rec + kStatW + j];
+        tot[kStatW + j] = s;
+        return;
+    }
+    const int64_t h = j - C;
+    if (h >= (int64_t)C * DB) return;
+    const int c = (int)(h / DB), k = (int)(h % DB);
+    double s = tot[kStatW + C + h];
+    for (int t = tile2[c]; t < tile2[c + 1]; ++t) s += part[(int64_t)t * rec + kStatW + C + k];
+    tot[kStatW + C + h] = s;
+}
+
+}  // namespace
+
+// ===========================================================================
+// host context
+// ===========================================================================
+struct chili_mc {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    chili_host::ChannelSet set;
+    chili_integrand f{};
+    int C = 0, d = 0, NT = 0, B = 0, P2 = 0;
+    std::vector<double> grids;   // [C][d][B+1]
+    std::vector<double> alpha;   // [C]
+    bool have_grids = false;
+    // device buffers
+    DevBuf<ChanProg> d_progs;
+    DevBuf<double> d_edges, d_search, d_alpha, d_cum, d_sext, d_tot, d_part, d_mom, d_ps, d_u;
+    DevBuf<chili_integrand> d_f;
+    DevBuf<uint16_t> d_pick;
+    DevBuf<uint32_t> d_sorted, d_comp;
+    DevBuf<uint8_t> d_acc;
+    DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff;
+    DevBuf<int64_t> d_total;
+    int64_t nmax = 0;            // attempts per sub-batch
+    double acc_rate = 0.25;      // running estimate of the cut acceptance
+    // k_main timing with HIP events on the context's stream
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_live, ev_pool;
+    std::vector<int64_t> ev_points;
+    double main_ms = 0, fwd_ms = 0;
+    int64_t main_launches = 0, main_points = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> fev_live;
+    ~chili_mc() {
+        for (auto& e : ev_live) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        for (auto& e : fev_live) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        for (auto& e : ev_pool) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    }
+};
+
+namespace {
+
+int next_pow2(int x) { int p = 1; while (p < x) p <<= 1; return p; }
+
+// Host-fed uniforms must lie in [0, 1): 1.0 would map to bin B (past the
+// grid) exactly as it would index past the reference's edge array.
+void check_uniforms(const double* u, size_t cnt) {
+    for (size_t i = 0; i < cnt; ++i)
+        if (!(u[i] >= 0.0 && u[i] < 1.0)) throw chili_error(CHILI_EINVAL, "host uniforms must lie in [0, 1)");
+}
+
+void upload_state(chili_mc* m) {
+    const int C = m->C, d = m->d, B = m->B, P2 = m->P2;
+    std::vector<double> srch((size_t)C * d * P2, std::numeric_limits<double>::infinity());
+    for (int c = 0; c < C; ++c)
+        for (int k = 0; k < d; ++k)
+            for (int b = 0; b < B; ++b) srch[((size_t)c * d + k) * P2 + b] = m->grids[((size_t)c * d + k) * (B + 1) + b];
+    m->d_edges.upload(m->grids, m->stream);
+    m->d_search.upload(srch, m->stream);
+    // cumulative alpha as std::discrete_distribution builds it (random.tcc:2656-2677)
+    std::vector<double> cum(C, 1.0);
+    if (C >= 2) {
+        double sum = 0.0;
+        for (double a : m->alpha) sum += a;
+        double acc = 0;
+        for (int i = 0; i < C; ++i) { double p = m->alpha[i] / sum; acc = (i == 0) ? p : acc + p; cum[i] = acc; }
+        cum[C - 1] = 1.0;
+    }
+    m->d_alpha.upload(m->alpha, m->stream);
+    m->d_cum.upload(cum, m->stream);
+}
+
+void ensure_capacity(chili_mc* m, int64_t n) {
+    if (n <= m->nmax) return;
+    const int64_t nblk = (n + kTile - 1) / kTile;
+    const int64_t ntiles = nblk + m->C + 1;
+    m->d_pick.resize(n);
+    m->d_sorted.resize(n);
+    m->d_comp.resize(n);
+    m->d_acc.resize(n);
+    m->d_mom.resize((size_t)n * m->NT * 4);
+    m->d_counts.resize((size_t)nblk * m->C);
+    m->d_offs.resize((size_t)nblk * m->C);
+    m->d_blkacc.resize(nblk);
+    m->d_kept.resize(ntiles);
+    m->d_keptoff.resize(ntiles);
+    const int64_t rec = kStatW + m->C + (int64_t)m->d * m->B;
+    m->d_part.resize((size_t)ntiles * rec);
+    m->d_ps.resize((size_t)ntiles * kTile * m->C);
+    m->nmax = n;
+}
+
+MCArgs make_args(chili_mc* m, const double* u, int64_t ustride, uint64_t seed, uint32_t stream, uint64_t a0) {
+    MCArgs A{};
+    A.progs = m->d_progs.ptr;
+    A.edges = m->d_edges.ptr;
+    A.search = m->d_search.ptr;
+    A.alpha = m->d_alpha.ptr;
+    A.cum = m->d_cum.ptr;
+    A.sext = m->d_sext.ptr;
+    A.f = m->d_f.ptr;
+    A.u = u;
+    A.ustride = ustride;
+    A.seed = seed;
+    A.a0 = a0;
+    A.stream = stream;
+    A.C = m->C;
+    A.d = m->d;
+    A.B = m->B;
+    A.P2 = m->P2;
+    A.sqrts = m->set.model.sqrts;
+    A.ptmax = m->set.model.sqrts / 2.0;
+    return A;
+}
+
+template <int NT>
+void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R) {
+    hipLaunchKernelGGL(k_forward<NT>, dim3(grid), dim3(kTile), 0, m->stream, A, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr,
+                       m->d_mom.ptr, m->nmax, m->d_acc.ptr, R);
+}
+template <int NT>
+void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals, int64_t vals_off, int64_t vals_cap,
+                 const PointRec& R) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    const size_t lds = sizeof(double) * (D * m->B + kWaves * (8 + m->C)) + sizeof(short) * D * kTile;
+    if (lds > 160 * 1024) throw chili_error(CHILI_ERANGE, "channel count x bins exceed the LDS of one workgroup");
+    const int64_t rec = kStatW + m->C + (int64_t)m->d * m->B;
+    hipLaunchKernelGGL(k_main<NT>, dim3(grid), dim3(kTile), lds, m->stream, A, m->d_seg2.ptr, m->d_tiles2.ptr, m->d_comp.ptr,
+                       m->d_mom.ptr, m->nmax, m->d_ps.ptr, (int64_t)grid * kTile, train, m->d_part.ptr, rec, vals, vals_off, vals_cap,
+                       m->d_sorted.ptr, R);
+}
+
+#define CHILI_DISPATCH_NT(NTV, CALL)           \
+    switch (NTV) {                             \
+        case 4: { constexpr int NT = 4; CALL; } break;  \
+        case 5: { constexpr int NT = 5; CALL; } break;  \
+        case 6: { constexpr int NT = 6; CALL; } break;  \
+        case 7: { constexpr int NT = 7; CALL; } break;  \
+        case 8: { constexpr int NT = 8; CALL; } break;  \
+        default: throw chili_error(CHILI_EINVAL, "unsupported number of external particles"); \
+    }
+
+std::pair<hipEvent_t, hipEvent_t> take_events(chili_mc* m) {
+    if (!m->ev_pool.empty()) {
+        auto e = m->ev_pool.back();
+        m->ev_pool.pop_back();
+        return e;
+    }
+    std::pair<hipEvent_t, hipEvent_t> e;
+    HIP_CHECK(hipEventCreate(&e.first));
+    HIP_CHECK(hipEventCreate(&e.second));
+    return e;
+}
+// Folds finished k_main / k_forward event pairs into the totals (stream synchronised).
+void harvest_events(chili_mc* m) {
+    for (size_t i = 0; i < m->ev_live.size(); ++i) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, m->ev_live[i].first, m->ev_live[i].second));
+        m->main_ms += ms;
+        m->main_launches += 1;
+        m->main_points += m->ev_points[i];
+        m->ev_pool.push_back(m->ev_live[i]);
+    }
+    for (auto& e : m->fev_live) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+        m->fwd_ms += ms;
+        m->ev_pool.push_back(e);
+    }
+    m->ev_live.clear();
+    m->ev_points.clear();
+    m->fev_live.clear();
+}
+
+// One sub-batch of attempts [a0, a0 + n): returns kept (accepted below the
+// cutoff), the cutoff (attempts consumed) and NaN count, accumulating into d_tot.
+void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, uint64_t seed, uint32_t stream, uint64_t a0,
+                  int64_t n, int64_t need, int train, double* vals, int64_t vals_off, int64_t vals_cap, int64_t& kept, int64_t& used,
+                  const PointRec& R = PointRec{}) {
+    ensure_capacity(m, n);
+    const int C = m->C;
+    const int nblk = (int)((n + kTile - 1) / kTile);
+    MCArgs A = make_args(m, u, ustride, seed, stream, a0);
+    A.ua0 = ua0;
+    hipLaunchKernelGGL(k_pick, dim3(nblk), dim3(kTile), sizeof(int) * C, m->stream, A, n, m->d_pick.ptr, m->d_counts.ptr, nblk);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, m->stream, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * C, m->d_total.ptr);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kTile), sizeof(int) * kWaves * C, m->stream, C, n, m->d_pick.ptr, m->d_offs.ptr, nblk,
+                       m->d_sorted.ptr);
+    hipLaunchKernelGGL(k_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_offs.ptr, nblk, n, m->d_seg.ptr, m->d_tiles.ptr);
+    const int tgrid = nblk + C;
+    std::pair<hipEvent_t, hipEvent_t> fev{};
+    if (m->timing) {
+        fev = take_events(m);
+        HIP_CHECK(hipEventRecord(fev.first, m->stream));
+    }
+    CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R));
+    if (m->timing) {
+        HIP_CHECK(hipEventRecord(fev.second, m->stream));
+        m->fev_live.push_back(fev);
+    }
+    hipLaunchKernelGGL(k_acc_count, dim3(nblk), dim3(kTile), 0, m->stream, m->d_acc.ptr, n, m->d_blkacc.ptr);
+    // cutoff: the attempt at which `need` accepted attempts are reached
+    std::vector<int> blkacc(nblk);
+    m->d_blkacc.download(blkacc.data(), nblk, m->stream);
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+    int64_t run = 0;
+    int64_t cut = n;
+    kept = 0;
+    for (int b = 0; b < nblk; ++b) {
+        if (run + blkacc[b] >= need) {
+            std::vector<uint8_t> fl(kTile);
+            const int64_t base = (int64_t)b * kTile;
+            const int64_t cnt = std::min<int64_t>(kTile, n - base);
+            HIP_CHECK(hipMemcpyAsync(fl.data(), m->d_acc.ptr + base, cnt, hipMemcpyDeviceToHost, m->stream));
+            HIP_CHECK(hipStreamSynchronize(m->stream));
+            for (int64_t i = 0; i < cnt; ++i) {
+                run += fl[i];
+                if (run == need) { cut = base + i + 1; break; }
+            }
+            break;
+        }
+        run += blkacc[b];
+    }
+    kept = std::min<int64_t>(run, need);
+    used = cut;
+    hipLaunchKernelGGL(k_keep_count, dim3(tgrid), dim3(kTile), 0, m->stream, C, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr, m->d_acc.ptr,
+                       (uint32_t)cut, m->d_kept.ptr);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, m->stream, m->d_kept.ptr, m->d_keptoff.ptr, (int64_t)tgrid, m->d_total.ptr);
+    hipLaunchKernelGGL(k_keep_scatter, dim3(tgrid), dim3(kTile), 0, m->stream, C, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr, m->d_acc.ptr,
+                       (uint32_t)cut, m->d_keptoff.ptr, m->d_comp.ptr);
+    hipLaunchKernelGGL(k_keep_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_tiles.ptr, m->d_keptoff.ptr, kept, m->d_seg2.ptr,
+                       m->d_tiles2.ptr);
+    if (kept > 0) {
+        const int mgrid = (int)((kept + kTile - 1) / kTile) + C;
+        std::pair<hipEvent_t, hipEvent_t> ev{};
+        if (m->timing) {
+            ev = take_events(m);
+            HIP_CHECK(hipEventRecord(ev.first, m->stream));
+        }
+        CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, train, vals, vals_off, vals_cap, R));
+        if (m->timing) {
+            HIP_CHECK(hipEventRecord(ev.second, m->stream));
+            m->ev_live.push_back(ev);
+            m->ev_points.push_back(kept);
+        }
+        const int64_t rec = kStatW + C + (int64_t)m->d * m->B;
+        const int64_t nout = 1 + (int64_t)C + (int64_t)C * m->d * m->B;
+        hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, m->stream, C, m->d * m->B, m->d_tiles2.ptr,
+                           m->d_part.ptr, rec, m->d_tot.ptr, train);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int chili_construct_channels(const chili_process* proc, chili_channels** out) {
+    return guarded([&] {
+        if (!proc || !out) throw chili_error(CHILI_EINVAL, "null argument");
+        auto* ch = new chili_channels();
+        std::unique_ptr<chili_channels> hold(ch);
+        auto& S = ch->set;
+        S.model = chili_host::model_from_abi(*proc, S.cuts);
+        S.topo = chili_host::construct_topologies(S.model, S.cuts);
+        S.ntot = (int)S.model.flavs.size();
+        S.ndims = 3 * (S.ntot - 2) - 2;
+        S.sexternal.assign(S.ntot, 0.0);
+        for (int i = 2; i < S.ntot; ++i) S.sexternal[i] = S.cuts.sexternal[i - 2];
+        for (const auto& t : S.topo) S.progs.push_back(chili_host::compile_channel(t, S.cuts, S.model.sqrts, S.ntot));
+        *out = hold.release();
+    });
+}
+int chili_channels_count(const chili_channels* ch) { return ch ? (int)ch->set.topo.size() : CHILI_EINVAL; }
+int chili_channels_ndims(const chili_channels* ch) { return ch ? ch->set.ndims : CHILI_EINVAL; }
+int chili_channels_get(const chili_channels* ch, int i, chili_channel_desc* out) {
+    return guarded([&] {
+        if (!ch || !out || i < 0 || i >= (int)ch->set.topo.size()) throw chili_error(CHILI_EINVAL, "bad channel index");
+        std::memset(out, 0, sizeof(*out));
+        const auto& t = ch->set.topo[i];
+        auto cv = [](const chili_host::Particle& p) { chili_particle_info q; q.idx = p.idx; q.pid = p.pid; q.mass = p.mass; q.width = p.width; return q; };
+        out->ninfo = (int)t.tchan.size();
+        for (size_t k = 0; k < t.tchan.size(); ++k) out->info[k] = cv(t.tchan[k]);
+        out->ndecay = (int)t.decays.size();
+        int k = 0;
+        for (const auto& dk : t.decays) {
+            out->decay[k][0] = cv(dk.first);
+            out->decay[k][1] = cv(dk.second.first);
+            out->decay[k][2] = cv(dk.second.second);
+            ++k;
+        }
+    });
+}
+void chili_channels_free(chili_channels* ch) { delete ch; }
+
+int chili_mc_create(const chili_channels* ch, int nbins, const chili_integrand* f, int device, chili_mc** out) {
+    return guarded([&] {
+        if (!ch || !f || !out) throw chili_error(CHILI_EINVAL, "null argument");
+        if (nbins < 2 || nbins > 256) throw chili_error(CHILI_ERANGE, "nbins must be in [2, 256]");
+        if (ch->set.topo.empty()) throw chili_error(CHILI_EINVAL, "process has no channels");
+        if (ch->set.ntot < 4 || ch->set.ntot > 8) throw chili_error(CHILI_ERANGE, "device path supports 2->2 .. 2->6");
+        if (ch->set.topo.size() > 4096) throw chili_error(CHILI_ERANGE, "at most 4096 channels");
+        select_device(device);
+        auto m = std::make_unique<chili_mc>();
+        m->device = device;
+        HIP_CHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        m->set = ch->set;
+        m->f = *f;
+        m->C = (int)ch->set.topo.size();
+        m->d = ch->set.ndims;
+        m->NT = ch->set.ntot;
+        m->B = nbins;
+        m->P2 = next_pow2(nbins);
+        m->d_progs.upload(m->set.progs, m->stream);
+        m->d_sext.upload(m->set.sexternal, m->stream);
+        std::vector<chili_integrand> fv(1, *f);
+        m->d_f.upload(fv, m->stream);
+        m->d_seg.resize(m->C + 1);
+        m->d_tiles.resize(m->C + 1);
+        m->d_seg2.resize(m->C + 1);
+        m->d_tiles2.resize(m->C + 1);
+        m->d_total.resize(1);
+        m->d_tot.resize(kStatW + m->C + (size_t)m->C * m->d * m->B);
+        // uniform grids and alpha = 1/C (AdaptiveMap.hh:29-40, MultiChannel.cc:62-67)
+        m->grids.resize((size_t)m->C * m->d * (m->B + 1));
+        for (int c = 0; c < m->C; ++c)
+            for (int k = 0; k < m->d; ++k)
+                for (int b = 0; b <= m->B; ++b) m->grids[((size_t)c * m->d + k) * (m->B + 1) + b] = (double)b / (double)m->B;
+        m->alpha.assign(m->C, 1.0 / (double)m->C);
+        upload_state(m.get());
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+        *out = m.release();
+    });
+}
+
+int chili_mc_destroy(chili_mc* m) {
+    return guarded([&] {
+        if (!m) return;
+        select_device(m->device);
+        if (m->stream) hipStreamSynchronize(m->stream);
+        hipStream_t s = m->stream;
+        delete m;
+        if (s) hipStreamDestroy(s);
+    });
+}
+
+int chili_mc_info(const chili_mc* m, int* nchannels, int* ndims, int* nbins, int* ntot) {
+    return guarded([&] {
+        if (!m) throw chili_error(CHILI_EINVAL, "null context");
+        if (nchannels) *nchannels = m->C;
+        if (ndims) *ndims = m->d;
+        if (nbins) *nbins = m->B;
+        if (ntot) *ntot = m->NT;
+    });
+}
+
+int chili_mc_set_grids(chili_mc* m, const double* grids, int nbins) {
+    return guarded([&] {
+        if (!m || !grids) throw chili_error(CHILI_EINVAL, "null argument");
+        if (nbins < 2 || nbins > 256) throw chili_error(CHILI_ERANGE, "nbins must be in [2, 256]");
+        select_device(m->device);
+        if (nbins != m->B) {
+            m->B = nbins;
+            m->P2 = next_pow2(nbins);
+            m->d_tot.resize(kStatW + m->C + (size_t)m->C * m->d * m->B);
+            m->nmax = 0;   // record sizes changed
+        }
+        m->grids.assign(grids, grids + (size_t)m->C * m->d * (m->B + 1));
+        upload_state(m);
+    });
+}
+int chili_mc_get_grids(const chili_mc* m, double* grids) {
+    return guarded([&] {
+        if (!m || !grids) throw chili_error(CHILI_EINVAL, "null argument");
+        std::copy(m->grids.begin(), m->grids.end(), grids);
+    });
+}
+int chili_mc_set_alpha(chili_mc* m, const double* alpha) {
+    return guarded([&] {
+        if (!m || !alpha) throw chili_error(CHILI_EINVAL, "null argument");
+        select_device(m->device);
+        m->alpha.assign(alpha, alpha + m->C);
+        upload_state(m);
+    });
+}
+int chili_mc_get_alpha(const chili_mc* m, double* alpha) {
+    return guarded([&] {
+        if (!m || !alpha) throw chili_error(CHILI_EINVAL, "null argument");
+        std::copy(m->alpha.begin(), m->alpha.end(), alpha);
+    });
+}
+
+// One iteration: attempts from a0 until ncalls pass the cut with a finite
+// value (MultiChannel.hh:214-248).  stats/T/hist are host outputs (T, hist
+// only when train != 0).  vals_dev (device pointer, optional) receives the
+// value of every kept point (channel-sorted order), at most vals_cap of them.
+int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, int train, chili_stats* stats,
+                     double* T, double* hist, int64_t* n_attempts, double* vals_dev, int64_t vals_cap) {
+    return guarded([&] {
+        if (!m || !uni || !stats) throw chili_error(CHILI_EINVAL, "null argument");
+        if (ncalls <= 0) throw chili_error(CHILI_EINVAL, "ncalls must be positive");
+        select_device(m->device);
+        const double* ud = nullptr;
+        int64_t ustride = 0;
+        if (uni->u) {
+            if (uni->stride <= 0) throw chili_error(CHILI_EINVAL, "uniform stride must be positive");
+            const size_t cnt = (size_t)(m->d + 1) * uni->stride;
+            check_uniforms(uni->u, cnt);
+            m->d_u.resize(cnt);
+            HIP_CHECK(hipMemcpyAsync(m->d_u.ptr, uni->u, cnt * sizeof(double), hipMemcpyHostToDevice, m->stream));
+            ud = m->d_u.ptr;
+            ustride = uni->stride;
+        }
+        const size_t ntot = kStatW + m->C + (size_t)m->C * m->d * m->B;
+        std::vector<double> tot(ntot, 0.0);
+        tot[1] = std::numeric_limits<double>::max();
+        tot[2] = std::numeric_limits<double>::min();
+        m->d_tot.upload(tot, m->stream);
+        int64_t need = ncalls, att = 0, rejected = 0, got = 0;
+        double prev_nan = 0;
+        const int64_t cap = std::max<int64_t>(int64_t(1) << 16, std::min<int64_t>(int64_t(1) << 23, 8 * ncalls + 4096));
+        while (need > 0) {
+            int64_t n = (int64_t)((double)need / std::max(m->acc_rate, 0.02) * 1.08) + 2048;
+            n = std::min(n, cap);
+            if (ud) {
+                n = std::min<int64_t>(n, ustride - att);
+                if (n <= 0) throw chili_error(CHILI_ERANGE, "host uniforms exhausted before ncalls points passed");
+            }
+            int64_t kept = 0, used = 0;
+            run_subbatch(m, ud, ustride, a0, uni->seed, uni->stream, a0 + (uint64_t)att, n, need, train, vals_dev, got, vals_cap,
+                         kept, used);
+            double nanc = 0;
+            HIP_CHECK(hipMemcpyAsync(&nanc, m->d_tot.ptr + 6, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+            HIP_CHECK(hipStreamSynchronize(m->stream));
+            const int64_t nan_new = (int64_t)(nanc - prev_nan);
+            prev_nan = nanc;
+            rejected += used - kept;
+            need -= kept - nan_new;
+            got += kept;
+            att += used;
+            if (used > 0) m->acc_rate = 0.5 * m->acc_rate + 0.5 * std::max(0.005, (double)kept / (double)used);
+        }
+        HIP_CHECK(hipMemcpyAsync(tot.data(), m->d_tot.ptr, ntot * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+        if (m->timing) harvest_events(m);
+        // rejected attempts add a zero each (MultiChannel.hh:219-223)
+        double n = tot[0] + (double)rejected;
+        double mn = tot[1], mx = tot[2];
+        if (rejected > 0) { mn = std::min(mn, 0.0); mx = std::max(mx, 0.0); }
+        stats->n = n; stats->min = mn; stats->max = mx; stats->sum = tot[3]; stats->sum2 = tot[4]; stats->n_finite = tot[5];
+        if (T && train) std::copy(tot.begin() + kStatW, tot.begin() + kStatW + m->C, T);
+        if (hist && train) std::copy(tot.begin() + kStatW + m->C, tot.end(), hist);
+        if (n_attempts) *n_attempts = att;
+    });
+}
+
+// Kernel timing on the context's stream (HIP events around every k_main and
+// k_forward launch).  enable: 1 on, 0 off; reset clears the totals.  Outputs
+// (each may be NULL): summed k_main milliseconds, its launch count and the
+// points it processed, summed k_forward milliseconds.
+int chili_mc_timing(chili_mc* m, int enable, int reset, double* main_ms, int64_t* launches, int64_t* points, double* fwd_ms) {
+    return guarded([&] {
+        if (!m) throw chili_error(CHILI_EINVAL, "null context");
+        select_device(m->device);
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+        harvest_events(m);
+        if (main_ms) *main_ms = m->main_ms;
+        if (launches) *launches = m->main_launches;
+        if (points) *points = m->main_points;
+        if (fwd_ms) *fwd_ms = m->fwd_ms;
+        if (reset) { m->main_ms = 0; m->fwd_ms = 0; m->main_launches = 0; m->main_points = 0; }
+        m->timing = enable != 0;
+    });
+}
+
+// Per-attempt records for attempts [a0, a0+n) without the retry loop (every
+// attempt evaluated once): the batch FFI's view of the path
+// (include/Interfaces/nanobind.hh:23-50) extended to the multichannel weight.
+// Host outputs (each may be NULL): status[n] (0 cut-rejected, 1 accepted,
+// 2 NaN value), pick[n], w[n], f[n], val[n], ps[C][n], x[d][n] (inverse point
+// of the picked channel), bins[d][n] (its FindBin), mom[ntot*4][n].
+int chili_mc_points(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n, int32_t* status, int32_t* pick, double* w,
+                    double* f, double* val, double* ps, double* x, int32_t* bins, double* mom) {
+    return guarded([&] {
+        if (!m || !uni) throw chili_error(CHILI_EINVAL, "null argument");
+        if (n <= 0 || n > (int64_t(1) << 24)) throw chili_error(CHILI_ERANGE, "n must be in [1, 2^24]");
+        select_device(m->device);
+        const double* ud = nullptr;
+        int64_t ustride = 0;
+        if (uni->u) {
+            if (uni->stride < n) throw chili_error(CHILI_ERANGE, "host uniforms shorter than n");
+            const size_t cnt = (size_t)(m->d + 1) * uni->stride;
+            check_uniforms(uni->u, cnt);
+            m->d_u.resize(cnt);
+            HIP_CHECK(hipMemcpyAsync(m->d_u.ptr, uni->u, cnt * sizeof(double), hipMemcpyHostToDevice, m->stream));
+            ud = m->d_u.ptr;
+            ustride = uni->stride;
+        }
+        DevBuf<int32_t> st, pk, bn;
+        DevBuf<double> wv, fv, vv, pv, xv, mv;
+        st.resize(n); pk.resize(n); bn.resize((size_t)m->d * n);
+        wv.resize(n); fv.resize(n); vv.resize(n); pv.resize((size_t)m->C * n); xv.resize((size_t)m->d * n);
+        mv.resize((size_t)m->NT * 4 * n);
+        HIP_CHECK(hipMemsetAsync(wv.ptr, 0, n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(fv.ptr, 0, n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(vv.ptr, 0, n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(pv.ptr, 0, (size_t)m->C * n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(xv.ptr, 0, (size_t)m->d * n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(bn.ptr, 0xff, (size_t)m->d * n * sizeof(int32_t), m->stream));
+        PointRec R{st.ptr, pk.ptr, wv.ptr, fv.ptr, vv.ptr, pv.ptr, xv.ptr, bn.ptr, mv.ptr, n};
+        const size_t ntot = kStatW + m->C + (size_t)m->C * m->d * m->B;
+        std::vector<double> tot(ntot, 0.0);
+        m->d_tot.upload(tot, m->stream);
+        int64_t kept = 0, used = 0;
+        run_subbatch(m, ud, ustride, a0, uni->seed, uni->stream, a0, n, n, 0, nullptr, 0, 0, kept, used, R);
+        HIP_CHECK(hipGetLastError());
+        if (status) st.download(status, n, m->stream);
+        if (pick) pk.download(pick, n, m->stream);
+        if (w) wv.download(w, n, m->stream);
+        if (f) fv.download(f, n, m->stream);
+        if (val) vv.download(val, n, m->stream);
+        if (ps) pv.download(ps, (size_t)m->C * n, m->stream);
+        if (x) xv.download(x, (size_t)m->d * n, m->stream);
+        if (bins) bn.download(bins, (size_t)m->d * n, m->stream);
+        if (mom) mv.download(mom, (size_t)m->NT * 4 * n, m->stream);
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+"""ctypes binding of libchili_gpu.so (include/chili_gpu.h).
+
+The product path is the HIP library: loading fails loudly when it is
+missing, and nothing here falls back to a CPU implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchili_gpu.so")
+_lib = None
+
+EXPORTS = [
+    "chili_last_error", "chili_device_count", "chili_philox_uniform",
+    "chili_construct_channels", "chili_channels_count", "chili_channels_ndims", "chili_channels_get", "chili_channels_free",
+    "chili_mc_create", "chili_mc_destroy", "chili_mc_info", "chili_mc_set_grids", "chili_mc_get_grids",
+    "chili_mc_set_alpha", "chili_mc_get_alpha", "chili_mc_iterate", "chili_mc_points", "chili_mc_timing",
+    "chili_vegas_create", "chili_vegas_destroy", "chili_vegas_set_grid", "chili_vegas_get_grid", "chili_vegas_iterate",
+    "chili_grid_adapt", "chili_grid_split", "chili_alpha_adapt", "chili_max_difference",
+]
+
+
+class ChiliError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ChiliError(f"{LIB_PATH} is not built (run `python -m chili_amd.build`); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    P, D, I32, I64, U64, U32 = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int64, C.c_uint64, C.c_uint32
+    L.chili_last_error.restype = C.c_char_p
+    L.chili_philox_uniform.restype = C.c_double
+    L.chili_philox_uniform.argtypes = [U64, U32, U64, U32]
+    L.chili_construct_channels.argtypes = [C.POINTER(abi.Process), C.POINTER(P)]
+    L.chili_channels_count.argtypes = [P]
+    L.chili_channels_ndims.argtypes = [P]
+    L.chili_channels_get.argtypes = [P, C.c_int, C.POINTER(abi.ChannelDesc)]
+    L.chili_channels_free.argtypes = [P]
+    L.chili_channels_free.restype = None
+    L.chili_mc_create.argtypes = [P, C.c_int, C.POINTER(abi.Integrand), C.c_int, C.POINTER(P)]
+    L.chili_mc_destroy.argtypes = [P]
+    L.chili_mc_info.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.chili_mc_set_grids.argtypes = [P, D, C.c_int]
+    L.chili_mc_get_grids.argtypes = [P, D]
+    L.chili_mc_set_alpha.argtypes = [P, D]
+    L.chili_mc_get_alpha.argtypes = [P, D]
+    L.chili_mc_iterate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_int, C.POINTER(abi.Stats), D, D,
+                                   C.POINTER(C.c_int64), C.c_void_p, I64]
+    L.chili_mc_points.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, I32, I32, D, D, D, D, D, I32, D]
+    L.chili_mc_timing.argtypes = [P, C.c_int, C.c_int, D, C.POINTER(C.c_int64), C.POINTER(C.c_int64), D]
+    L.chili_vegas_create.argtypes = [C.c_int, C.c_int, C.POINTER(abi.Integrand), C.c_int, C.POINTER(P)]
+    L.chili_vegas_destroy.argtypes = [P]
+    L.chili_vegas_set_grid.argtypes = [P, D, C.c_int]
+    L.chili_vegas_get_grid.argtypes = [P, D, C.POINTER(C.c_int)]
+    L.chili_vegas_iterate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), D, C.c_void_p, I64]
+    L.chili_grid_adapt.argtypes = [C.c_int, C.c_int, D, C.c_double, D]
+    L.chili_grid_split.argtypes = [C.c_int, C.c_int, D, C.c_int, D]
+    L.chili_alpha_adapt.argtypes = [C.c_int, D, D, C.c_double, C.c_double]
+    L.chili_max_difference.restype = C.c_double
+    L.chili_max_difference.argtypes = [C.c_int, D]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise ChiliError(f"{abi.ERRORS.get(rc, rc)}: {load().chili_last_error().decode()}")
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def device_count():
+    return load().chili_device_count()
+
+
+class Channels:
+    """ConstructChannels (src/Channel/ChannelNode.cc:98-291) on the host."""
+
+    def __init__(self, proc):
+        L = load()
+        h = C.c_void_p()
+        check(L.chili_construct_channels(C.byref(proc), C.byref(h)))
+        self.h = h
+        self.count = L.chili_channels_count(h)
+        self.ndims = L.chili_channels_ndims(h)
+        self.ntot = proc.nflav
+
+    def desc(self, i):
+        d = abi.ChannelDesc()
+        check(load().chili_channels_get(self.h, i, C.byref(d)))
+        return d
+
+    def descs(self):
+        return [self.desc(i).as_tuple() for i in range(self.count)]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().chili_channels_free(self.h)
+            self.h = None
+
+
+class MultiChannelGPU:
+    """Device context of one multichannel integrand (chili_mc_*)."""
+
+    def __init__(self, channels, nbins, integrand, device=0):
+        L = load()
+        h = C.c_void_p()
+        check(L.chili_mc_create(channels.h, nbins, C.byref(integrand), device, C.byref(h)))
+        self.h = h
+        nc, nd, nb, nt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(L.chili_mc_info(h, C.byref(nc), C.byref(nd), C.byref(nb), C.byref(nt)))
+        self.nchannels, self.ndims, self.nbins, self.ntot = nc.value, nd.value, nb.value, nt.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().chili_mc_destroy(self.h)
+            self.h = None
+
+    def set_grids(self, grids):
+        g = np.ascontiguousarray(grids, dtype=np.float64)
+        check(load().chili_mc_set_grids(self.h, _d(g), g.shape[-1] - 1))
+        self.nbins = g.shape[-1] - 1
+
+    def grids(self):
+        g = np.zeros((self.nchannels, self.ndims, self.nbins + 1))
+        check(load().chili_mc_get_grids(self.h, _d(g)))
+        return g
+
+    def set_alpha(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(load().chili_mc_set_alpha(self.h, _d(a)))
+
+    def alpha(self):
+        a = np.zeros(self.nchannels)
+        check(load().chili_mc_get_alpha(self.h, _d(a)))
+        return a
+
+    def iterate(self, uni, ncalls, a0=0, train=True, vals_ptr=None, vals_cap=0):
+        st = abi.Stats()
+        T = np.zeros(self.nchannels)
+        hist = np.zeros((self.nchannels, self.ndims, self.nbins))
+        na = C.c_int64()
+        check(load().chili_mc_iterate(self.h, C.byref(uni), a0, ncalls, 1 if train else 0, C.byref(st), _d(T), _d(hist),
+                                      C.byref(na), vals_ptr, vals_cap))
+        return st, T, hist, na.value
+
+    def timing(self, enable=True, reset=False):
+        ms, fms = C.c_double(), C.c_double()
+        nl, npts = C.c_int64(), C.c_int64()
+        check(load().chili_mc_timing(self.h, 1 if enable else 0, 1 if reset else 0, C.byref(ms), C.byref(nl), C.byref(npts),
+                                     C.byref(fms)))
+        return dict(main_ms=ms.value, launches=nl.value, points=npts.value, forward_ms=fms.value)
+
+    def points(self, uni, a0, n):
+        d, c, nt = self.ndims, self.nchannels, self.ntot
+        out = dict(status=np.zeros(n, np.int32), pick=np.zeros(n, np.int32), w=np.zeros(n), f=np.zeros(n), val=np.zeros(n),
+                   ps=np.zeros((c, n)), x=np.zeros((d, n)), bins=np.zeros((d, n), np.int32), mom=np.zeros((nt * 4, n)))
+        check(load().chili_mc_points(self.h, C.byref(uni), a0, n, _i(out["status"]), _i(out["pick"]), _d(out["w"]),
+                                     _d(out["f"]), _d(out["val"]), _d(out["ps"]), _d(out["x"]), _i(out["bins"]),
+                                     _d(out["mom"])))
+        # attempt-major views matching the oracle's layout
+        out["ps"] = out["ps"].T.copy()
+        out["xpick"] = out.pop("x").T.copy()
+        out["bins"] = out["bins"].T.copy()
+        out["mom"] = out["mom"].T.reshape(n, nt, 4).copy()
+        return out
+
+
+class VegasGPU:
+    """Device context of plain VEGAS (chili_vegas_*)."""
+
+    def __init__(self, ndims, nbins, integrand, device=0):
+        L = load()
+        h = C.c_void_p()
+        check(L.chili_vegas_create(ndims, nbins, C.byref(integrand), device, C.byref(h)))
+        self.h, self.ndims, self.nbins = h, ndims, nbins
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().chili_vegas_destroy(self.h)
+            self.h = None
+
+    def set_grid(self, edges):
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        check(load().chili_vegas_set_grid(self.h, _d(e), e.shape[1] - 1))
+        self.nbins = e.shape[1] - 1
+
+    def grid(self):
+        e = np.zeros((self.ndims, self.nbins + 1))
+        nb = C.c_int()
+        check(load().chili_vegas_get_grid(self.h, _d(e), C.byref(nb)))
+        return e
+
+    def iterate(self, uni, ncalls, a0=0, vals_ptr=None, vals_cap=0):
+        st = abi.Stats()
+        hist = np.zeros((self.ndims, self.nbins))
+        check(load().chili_vegas_iterate(self.h, C.byref(uni), a0, ncalls, C.byref(st), _d(hist), vals_ptr, vals_cap))
+        return st, hist
+
+
+def grid_adapt(edges, data, alpha=1.0):
+    e = np.array(edges, dtype=np.float64)
+    dd = np.ascontiguousarray(data, dtype=np.float64)
+    check(load().chili_grid_adapt(e.shape[0], e.shape[1] - 1, _d(e), alpha, _d(dd)))
+    return e
+
+
+def grid_split(edges, nsplit=2):
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    out = np.zeros((e.shape[0], nsplit * (e.shape[1] - 1) + 1))
+    check(load().chili_grid_split(e.shape[0], e.shape[1] - 1, _d(e), nsplit, _d(out)))
+    return out
+
+
+def alpha_adapt(alpha, train, beta=0.25, min_alpha=1e-5):
+    a = np.array(alpha, dtype=np.float64)
+    t = np.ascontiguousarray(train, dtype=np.float64)
+    check(load().chili_alpha_adapt(len(a), _d(a), _d(t), beta, min_alpha))
+    return a
+
+
+def max_difference(train):
+    t = np.ascontiguousarray(train, dtype=np.float64)
+    return load().chili_max_difference(len(t), _d(t))
+
+
+def philox_uniform(seed, stream, attempt, k):
+    return load().chili_philox_uniform(seed, stream, attempt, k)

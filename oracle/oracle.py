@@ -1,0 +1,227 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
+never by the product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from chili_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+
+def lib(count=False):
+    name = "liboracle_count.so" if count else "liboracle.so"
+    if name not in _LIBS:
+        path = os.path.join(_HERE, "_build", name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
+        P, D, I64, I32, U64, U32 = C.c_void_p, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int32), C.c_uint64, C.c_uint32
+        L.oracle_create.restype = P
+        L.oracle_create.argtypes = [C.POINTER(abi.Process), C.c_int, C.POINTER(abi.Integrand), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.oracle_destroy.argtypes = [P]
+        L.oracle_channel_desc.argtypes = [P, C.c_int, C.POINTER(abi.ChannelDesc)]
+        L.oracle_set_grids.argtypes = [P, D, C.c_int]
+        L.oracle_get_grids.argtypes = [P, D]
+        L.oracle_set_alpha.argtypes = [P, D]
+        L.oracle_get_alpha.argtypes = [P, D]
+        L.oracle_generate_point.argtypes = [P, C.c_int, D, D]
+        L.oracle_generate_weight.restype = C.c_double
+        L.oracle_generate_weight.argtypes = [P, C.c_int, D, D]
+        L.oracle_mc_points.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, I32, I32, D, D, D, D, D, I32, D]
+        L.oracle_mc_iterate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_int, C.POINTER(abi.Stats), D, D, C.POINTER(C.c_int64)]
+        L.oracle_alpha_adapt.argtypes = [C.c_int, D, D, C.c_double, C.c_double]
+        L.oracle_max_difference.restype = C.c_double
+        L.oracle_max_difference.argtypes = [C.c_int, D]
+        L.oracle_grid_adapt.argtypes = [C.c_int, C.c_int, D, C.c_double, D]
+        L.oracle_grid_split.argtypes = [C.c_int, C.c_int, D, C.c_int, D]
+        L.oracle_grid_find_bin.argtypes = [C.c_int, C.c_int, D, C.c_int, C.c_double]
+        L.oracle_grid_forward.restype = C.c_double
+        L.oracle_grid_forward.argtypes = [C.c_int, C.c_int, D, D]
+        L.oracle_vegas_iterate.argtypes = [C.c_int, C.c_int, D, C.POINTER(abi.Integrand), C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), D, D]
+        L.oracle_integrand.restype = C.c_double
+        L.oracle_integrand.argtypes = [C.POINTER(abi.Integrand), C.c_int, D, C.c_double, C.POINTER(C.c_int)]
+        L.oracle_eemumu.restype = C.c_double
+        L.oracle_eemumu.argtypes = [C.POINTER(abi.Integrand), D]
+        L.oracle_philox_uniform.restype = C.c_double
+        L.oracle_philox_uniform.argtypes = [U64, U32, U64, U32]
+        L.oracle_philox_block.argtypes = [C.POINTER(C.c_uint32), U32, U32]
+        L.oracle_count_points.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, D]
+        L.oracle_count_get.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
+        for nm in ("oracle_sqlam", "oracle_massless_prop_momenta", "oracle_massive_prop_momenta"):
+            getattr(L, nm).restype = C.c_double
+        L.oracle_sqlam.argtypes = [C.c_double] * 3
+        L.oracle_massless_prop_momenta.argtypes = [C.c_double] * 4
+        L.oracle_massive_prop_momenta.argtypes = [C.c_double] * 5
+        L.oracle_massless_prop_weight.restype = C.c_double
+        L.oracle_massless_prop_weight.argtypes = [C.c_double] * 4 + [D]
+        L.oracle_massive_prop_weight.restype = C.c_double
+        L.oracle_massive_prop_weight.argtypes = [C.c_double] * 5 + [D]
+        L.oracle_schannel_momenta.argtypes = [D, C.c_double, C.c_double, C.c_double, C.c_double, D, D]
+        L.oracle_schannel_weight.restype = C.c_double
+        L.oracle_schannel_weight.argtypes = [D, D, D, D]
+        _LIBS[name] = L
+    return _LIBS[name]
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Oracle:
+    """One process (channels + grids + alpha) in the CPU oracle."""
+
+    def __init__(self, proc, nbins, integrand=None, count=False):
+        self.L = lib(count)
+        self.f = integrand
+        nc, nd = C.c_int(), C.c_int()
+        self.h = self.L.oracle_create(C.byref(proc), nbins, C.byref(integrand) if integrand is not None else None,
+                                      C.byref(nc), C.byref(nd))
+        if not self.h:
+            raise RuntimeError("oracle_create failed")
+        self.nchannels, self.ndims, self.nbins = nc.value, nd.value, nbins
+        self.ntot = proc.nflav
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oracle_destroy(self.h)
+            self.h = None
+
+    def channel(self, i):
+        d = abi.ChannelDesc()
+        assert self.L.oracle_channel_desc(self.h, i, C.byref(d)) == 0
+        return d
+
+    def channels(self):
+        return [self.channel(i).as_tuple() for i in range(self.nchannels)]
+
+    def set_grids(self, grids):
+        g = np.ascontiguousarray(grids, dtype=np.float64)
+        self.nbins = g.shape[-1] - 1
+        self.L.oracle_set_grids(self.h, _d(g), self.nbins)
+
+    def grids(self):
+        g = np.zeros((self.nchannels, self.ndims, self.nbins + 1))
+        self.L.oracle_get_grids(self.h, _d(g))
+        return g
+
+    def set_alpha(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.L.oracle_set_alpha(self.h, _d(a))
+
+    def alpha(self):
+        a = np.zeros(self.nchannels)
+        self.L.oracle_get_alpha(self.h, _d(a))
+        return a
+
+    def generate_point(self, ch, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        mom = np.zeros((self.ntot, 4))
+        self.L.oracle_generate_point(self.h, ch, _d(x), _d(mom))
+        return mom
+
+    def generate_weight(self, ch, mom):
+        mom = np.ascontiguousarray(mom, dtype=np.float64)
+        x = np.zeros(self.ndims)
+        w = self.L.oracle_generate_weight(self.h, ch, _d(mom), _d(x))
+        return w, x
+
+    def points(self, uni, a0, n):
+        d, c, nt = self.ndims, self.nchannels, self.ntot
+        out = dict(pick=np.zeros(n, np.int32), status=np.zeros(n, np.int32), w=np.zeros(n), f=np.zeros(n),
+                   val=np.zeros(n), mom=np.zeros((n, nt, 4)), xpick=np.zeros((n, d)),
+                   bins=np.zeros((n, d), np.int32), ps=np.zeros((n, c)))
+        self.L.oracle_mc_points(self.h, C.byref(uni), a0, n, _i(out["pick"]), _i(out["status"]), _d(out["w"]),
+                                _d(out["f"]), _d(out["val"]), _d(out["mom"]), _d(out["xpick"]), _i(out["bins"]),
+                                _d(out["ps"]))
+        return out
+
+    def count_points(self, uni, a0, n):
+        out = np.zeros(5)
+        self.L.oracle_count_points(self.h, C.byref(uni), a0, n, _d(out))
+        return dict(fwd_arith=out[0], fwd_trans=out[1], wgt_arith=out[2], wgt_trans=out[3], accepted=out[4], attempts=n)
+
+    def iterate(self, uni, ncalls, a0=0, train=True):
+        st = abi.Stats()
+        T = np.zeros(self.nchannels)
+        hist = np.zeros((self.nchannels, self.ndims, self.nbins))
+        na = C.c_int64()
+        rc = self.L.oracle_mc_iterate(self.h, C.byref(uni), a0, ncalls, 1 if train else 0, C.byref(st), _d(T), _d(hist), C.byref(na))
+        if rc != 0:
+            raise RuntimeError(f"oracle_mc_iterate rc={rc}")
+        return st, T, hist, na.value
+
+
+def alpha_adapt(alpha, train, beta=0.25, min_alpha=1e-5):
+    a = np.array(alpha, dtype=np.float64)
+    t = np.ascontiguousarray(train, dtype=np.float64)
+    lib().oracle_alpha_adapt(len(a), _d(a), _d(t), beta, min_alpha)
+    return a
+
+
+def max_difference(train):
+    t = np.ascontiguousarray(train, dtype=np.float64)
+    return lib().oracle_max_difference(len(t), _d(t))
+
+
+def grid_adapt(edges, data, alpha=1.0):
+    e = np.array(edges, dtype=np.float64)
+    dims, bp1 = e.shape
+    dd = np.ascontiguousarray(data, dtype=np.float64)
+    lib().oracle_grid_adapt(dims, bp1 - 1, _d(e), alpha, _d(dd))
+    return e
+
+
+def grid_split(edges, nsplit=2):
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    dims, bp1 = e.shape
+    out = np.zeros((dims, nsplit * (bp1 - 1) + 1))
+    lib().oracle_grid_split(dims, bp1 - 1, _d(e), nsplit, _d(out))
+    return out
+
+
+def grid_find_bin(edges, dim, x):
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    return lib().oracle_grid_find_bin(e.shape[0], e.shape[1] - 1, _d(e), dim, x)
+
+
+def grid_forward(edges, r):
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    rr = np.array(r, dtype=np.float64)
+    jac = lib().oracle_grid_forward(e.shape[0], e.shape[1] - 1, _d(e), _d(rr))
+    return rr, jac
+
+
+def uniform_grid(dims, bins):
+    row = np.arange(bins + 1, dtype=np.float64) / bins
+    return np.tile(row, (dims, 1))
+
+
+def vegas_iterate(edges, integrand, uni, ncalls, a0=0, want_vals=False):
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    dims, bp1 = e.shape
+    st = abi.Stats()
+    hist = np.zeros((dims, bp1 - 1))
+    vals = np.zeros(ncalls) if want_vals else None
+    lib().oracle_vegas_iterate(dims, bp1 - 1, _d(e), C.byref(integrand), C.byref(uni), a0, ncalls, C.byref(st), _d(hist),
+                               _d(vals) if want_vals else None)
+    return st, hist, vals
+
+
+def philox_uniform(seed, stream, attempt, k):
+    return lib().oracle_philox_uniform(seed, stream, attempt, k)
+
+
+def philox_block(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    lib().oracle_philox_block(c, key[0], key[1])
+    return [c[i] for i in range(4)]
