@@ -41,31 +41,63 @@ __global__ __launch_bounds__(kTile) void k_pick(MCArgs A, int64_t n, uint16_t* _
 }
 
 // ---------------------------------------------------------------------------
-// Single-workgroup exclusive scan of an int array (len up to a few million);
-// out may alias in.  total receives the sum.
+// Exclusive scan of an int array in three passes: sums of 1024-element blocks,
+// a single-workgroup scan of those sums, then each block's scan plus its
+// offset.  out may alias in; total (device) receives the sum.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int* out, int64_t len, int64_t* total) {
-    __shared__ int64_t s_part[1024];
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ int64_t block_scan_1024(int64_t v, int64_t* s_tmp, int64_t& total) {
+    // inclusive Hillis-Steele scan over 1024 threads, returns the exclusive value
     const int t = threadIdx.x;
-    const int64_t chunk = (len + 1023) / 1024;
-    const int64_t b = t * chunk, e = std::min<int64_t>(len, b + chunk);
-    int64_t sum = 0;
-    for (int64_t i = b; i < e; ++i) sum += in[i];
-    s_part[t] = sum;
+    s_tmp[t] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int64_t v = t >= off ? s_part[t - off] : 0;
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        int64_t x = t >= off ? s_tmp[t - off] : 0;
         __syncthreads();
-        s_part[t] += v;
+        s_tmp[t] += x;
         __syncthreads();
     }
-    int64_t run = s_part[t] - sum;
-    for (int64_t i = b; i < e; ++i) {
-        int v = in[i];
-        out[i] = (int)run;
-        run += v;
+    total = s_tmp[kScanBlock - 1];
+    int64_t ex = s_tmp[t] - v;
+    __syncthreads();
+    return ex;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(const int* __restrict__ in, int64_t len, int64_t* __restrict__ bsum) {
+    __shared__ int64_t s[kScanBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    int64_t v = i < len ? in[i] : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int w = 0; w < kScanBlock / 64; ++w) t += s[w];
+        bsum[blockIdx.x] = t;
     }
-    if (t == 1023 && total) *total = s_part[1023];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_top(int64_t* __restrict__ bsum, int64_t nb, int64_t* total) {
+    __shared__ int64_t s_tmp[kScanBlock];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += kScanBlock) {
+        const int64_t i = base + threadIdx.x;
+        int64_t v = i < nb ? bsum[i] : 0, tot;
+        int64_t ex = block_scan_1024(v, s_tmp, tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int* in, int* out, int64_t len, const int64_t* __restrict__ bsum) {
+    __shared__ int64_t s_tmp[kScanBlock];
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    int64_t v = i < len ? in[i] : 0, tot;
+    int64_t ex = block_scan_1024(v, s_tmp, tot);
+    if (i < len) out[i] = (int)(bsum[blockIdx.x] + ex);
 }
 
 // Per-wave stable ranks of equal keys: rank among earlier lanes with the same
@@ -123,7 +155,7 @@ __global__ void k_segments(int C, const int* __restrict__ offs, int nblk, int64_
 }
 
 __device__ __forceinline__ bool locate_tile(int C, const int* __restrict__ seg_start, const int* __restrict__ tile_start, int tile,
-                                            int& ch, int& e0, int& e1) {
+                                            int& ch, int& e0, int& e1, int tsize = kTile) {
     if (tile >= tile_start[C]) return false;
     int lo = 0, hi = C - 1;
     while (lo < hi) {            // last c with tile_start[c] <= tile
@@ -131,8 +163,8 @@ __device__ __forceinline__ bool locate_tile(int C, const int* __restrict__ seg_s
         if (tile_start[mid] <= tile) lo = mid; else hi = mid - 1;
     }
     ch = lo;
-    e0 = seg_start[lo] + (tile - tile_start[lo]) * kTile;
-    e1 = std::min(e0 + kTile, seg_start[lo + 1]);
+    e0 = seg_start[lo] + (tile - tile_start[lo]) * tsize;
+    e1 = std::min(e0 + tsize, seg_start[lo + 1]);
     return true;
 }
 
@@ -167,7 +199,7 @@ __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restri
         r[i] = left + loc * size;
     }
     const ChanProg& p = A.progs[ch];
-    V4 P[2 * NT];
+    MomRF<2 * NT> P;
     double m2[2 * NT];
     generate_point<NT>(A, p, r, P, m2);
     bool ok = cut_pass<NT>(A.f, P, A.sqrts);
@@ -177,19 +209,19 @@ __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restri
         R.pick[a] = ch;
 #pragma unroll
         for (int i = 0; i < NT; ++i) {
-            R.mom[(int64_t)(i * 4 + 0) * R.n + a] = P[i].t;
-            R.mom[(int64_t)(i * 4 + 1) * R.n + a] = P[i].x;
-            R.mom[(int64_t)(i * 4 + 2) * R.n + a] = P[i].y;
-            R.mom[(int64_t)(i * 4 + 3) * R.n + a] = P[i].z;
+            R.mom[(int64_t)(i * 4 + 0) * R.n + a] = P.v[4 * i + 0];
+            R.mom[(int64_t)(i * 4 + 1) * R.n + a] = P.v[4 * i + 1];
+            R.mom[(int64_t)(i * 4 + 2) * R.n + a] = P.v[4 * i + 2];
+            R.mom[(int64_t)(i * 4 + 3) * R.n + a] = P.v[4 * i + 3];
         }
     }
     if (ok) {
 #pragma unroll
         for (int i = 0; i < NT; ++i) {
-            mom[(int64_t)(i * 4 + 0) * nmax + e] = P[i].t;
-            mom[(int64_t)(i * 4 + 1) * nmax + e] = P[i].x;
-            mom[(int64_t)(i * 4 + 2) * nmax + e] = P[i].y;
-            mom[(int64_t)(i * 4 + 3) * nmax + e] = P[i].z;
+            mom[(int64_t)(i * 4 + 0) * nmax + e] = P.v[4 * i + 0];
+            mom[(int64_t)(i * 4 + 1) * nmax + e] = P.v[4 * i + 1];
+            mom[(int64_t)(i * 4 + 2) * nmax + e] = P.v[4 * i + 2];
+            mom[(int64_t)(i * 4 + 3) * nmax + e] = P.v[4 * i + 3];
         }
     }
 }
@@ -256,7 +288,7 @@ __global__ __launch_bounds__(kTile) void k_keep_scatter(int C, const int* __rest
 
 // Segments of the compacted list: per channel, kept_off at its first tile.
 __global__ void k_keep_segments(int C, const int* __restrict__ tile_start, const int* __restrict__ kept_off, int64_t kept_total,
-                                int* seg2, int* tile2) {
+                                int* seg2, int* tile2, int tsize) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int t = 0;
     for (int c = 0; c < C; ++c) {
@@ -265,7 +297,7 @@ __global__ void k_keep_segments(int C, const int* __restrict__ tile_start, const
         int e = (c + 1 < C && tile_start[c + 1] < tile_start[C]) ? kept_off[tile_start[c + 1]] : (int)kept_total;
         seg2[c] = s;
         tile2[c] = t;
-        t += (e - s + kTile - 1) / kTile;
+        t += (e - s + tsize - 1) / tsize;
     }
     seg2[C] = (int)kept_total;
     tile2[C] = t;
@@ -300,6 +332,9 @@ __device__ __forceinline__ double wave_max(double v) {
 // StatsData, alpha-training T_j += val^2 W / ps_j and the picked channel's
 // f^2 histogram (Integrand.hh:89-123, MultiChannel.hh:167-180, 224-247).
 // ---------------------------------------------------------------------------
+constexpr int kRounds = 4;                 // sub-tiles of kTile points per k_main workgroup
+constexpr int kMainTile = kTile * kRounds;
+
 template <int NT>
 __global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict__ seg2, const int* __restrict__ tile2,
                                                 const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
@@ -308,83 +343,118 @@ __global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict_
                                                 int64_t vals_cap, const uint32_t* __restrict__ sorted, PointRec R) {
     constexpr int D = 3 * (NT - 2) - 2;
     extern __shared__ double s_mem[];
-    // LDS: hist [D][B] doubles, per-wave sums [kWaves][8 + C] doubles, bins [D][kTile] shorts
+    // LDS: hist [D][B], per-wave sums [kWaves][8 + C], bins [D][kTile] shorts
     const int B = A.B, C = A.C;
     double* s_hist = s_mem;
     double* s_small = s_hist + D * B;
     short* s_bins = (short*)(s_small + kWaves * (8 + C));
     int ch, e0, e1;
-    if (!locate_tile(C, seg2, tile2, blockIdx.x, ch, e0, e1)) return;
-    const int e = e0 + threadIdx.x;
-    const bool valid = e < e1;
+    if (!locate_tile(C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t gt = (int64_t)blockIdx.x * kTile + threadIdx.x;
-
-    V4 P[2 * NT];
-    double val = 0, W = 0, fv = 0;
-    bool isnanv = false;
-    uint32_t att = 0;
-    if (valid) {
-        const uint32_t sp = comp[e];
-        if (R.status) att = sorted[sp];
+    for (int i = threadIdx.x; i < D * B; i += kTile) s_hist[i] = 0.0;
+    for (int i = threadIdx.x; i < kWaves * (8 + C); i += kTile) s_small[i] = 0.0;
+    // per-lane StatsData moments, in round order
+    double st_n = 0, st_s1 = 0, st_s2 = 0, st_nf = 0, st_nn = 0;
+    double st_mn = std::numeric_limits<double>::max(), st_mx = std::numeric_limits<double>::min();
+    __syncthreads();
+    for (int rnd = 0; rnd < kRounds; ++rnd) {
+        const int e = e0 + rnd * kTile + threadIdx.x;
+        if (e0 + rnd * kTile >= e1) break;           // uniform: no points left in this super-tile
+        const bool valid = e < e1;
+        MomRF<2 * NT> P;
+        double val = 0, W = 0, fv = 0;
+        bool isnanv = false;
+        uint32_t att = 0;
+        if (valid) {
+            const uint32_t sp = comp[e];
+            if (R.status) att = sorted[sp];
 #pragma unroll
-        for (int i = 0; i < NT; ++i) {
-            P[i].t = mom[(int64_t)(i * 4 + 0) * nmax + sp];
-            P[i].x = mom[(int64_t)(i * 4 + 1) * nmax + sp];
-            P[i].y = mom[(int64_t)(i * 4 + 2) * nmax + sp];
-            P[i].z = mom[(int64_t)(i * 4 + 3) * nmax + sp];
-        }
-        fv = integrand<NT>(A.f, P);
-    } else {
+            for (int i = 0; i < NT; ++i) {
+                P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
+                P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
+                P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
+                P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+            }
+            fv = integrand<NT>(A.f, P);
+        } else {
 #pragma unroll
-        for (int i = 0; i < NT; ++i) P[i] = mk(1., 0., 0., 1.);
-    }
-    // Integrand::GenerateWeight (Integrand.hh:112-123)
-    double wsum = 0;
-    for (int j = 0; j < C; ++j) {
-        double vw;
-        short* bo = (j == ch) ? (s_bins + threadIdx.x) : nullptr;
-        const bool recx = R.status && valid && j == ch;
-        double ps = generate_weight<NT>(A, A.progs[j], j, P, &vw, bo, kTile, recx ? R.x + att : nullptr,
-                                        recx ? R.bins + att : nullptr, R.n);
-        wsum += A.alpha[j] / ps / vw;
-        psbuf[(int64_t)j * psstride + gt] = ps;
-        if (R.status && valid) R.ps[(int64_t)j * R.n + att] = ps;
-    }
-    if (valid) {
-        W = 1.0 / wsum;
-        val = (W == 0) ? 0.0 : fv * W;
-        isnanv = isnan(val);
-        if (isnanv) { val = 0; W = 0; }
-        if (vals_out && vals_off + e < vals_cap) vals_out[vals_off + e] = val;
-        if (R.status) {
-            R.status[att] = isnanv ? 2 : 1;
-            R.w[att] = W;
-            R.f[att] = fv;
-            R.val[att] = val;
+            for (int i = 0; i < NT; ++i) P.set(i, mk(1., 0., 0., 1.));
         }
-    }
-    // ---- StatsData partial (Statistics.hh:92-102) ----
-    const double v2 = val * val;
-    {
-        double n = wave_sum(valid ? 1.0 : 0.0);
-        double s1 = wave_sum(val);
-        double s2 = wave_sum(v2);
-        double nf = wave_sum((valid && val != 0) ? 1.0 : 0.0);
-        double nn = wave_sum(isnanv ? 1.0 : 0.0);
-        double mn = wave_min(valid ? val : std::numeric_limits<double>::max());
-        double mx = wave_max(valid ? val : std::numeric_limits<double>::min());
-        if (lane == 0) {
-            double* q = s_small + w * (8 + C);
-            q[0] = n; q[1] = mn; q[2] = mx; q[3] = s1; q[4] = s2; q[5] = nf; q[6] = nn;
+        // Integrand::GenerateWeight (Integrand.hh:112-123)
+        double wsum = 0;
+        for (int j = 0; j < C; ++j) {
+            double vw;
+            short* bo = (j == ch) ? (s_bins + threadIdx.x) : nullptr;
+            const bool recx = R.status && valid && j == ch;
+            double ps = generate_weight<NT>(A, A.progs[j], j, P, &vw, bo, kTile, recx ? R.x + att : nullptr,
+                                            recx ? R.bins + att : nullptr, R.n);
+            wsum += A.alpha[j] / ps / vw;
+            psbuf[(int64_t)j * psstride + gt] = ps;
+            if (R.status && valid) R.ps[(int64_t)j * R.n + att] = ps;
         }
-    }
-    // ---- alpha training T_j (MultiChannel.hh:174-177) ----
-    if (train) {
+        if (valid) {
+            W = 1.0 / wsum;
+            val = (W == 0) ? 0.0 : fv * W;
+            isnanv = isnan(val);
+            if (isnanv) { val = 0; W = 0; }
+            if (vals_out && vals_off + e < vals_cap) vals_out[vals_off + e] = val;
+            if (R.status) {
+                R.status[att] = isnanv ? 2 : 1;
+                R.w[att] = W;
+                R.f[att] = fv;
+                R.val[att] = val;
+            }
+            // StatsData::operator+= (Statistics.hh:92-102)
+            st_n += 1.0; st_s1 += val; st_s2 += val * val;
+            if (val != 0) st_nf += 1.0;
+            if (isnanv) st_nn += 1.0;
+            st_mn = (val < st_mn) ? val : st_mn;
+            st_mx = (st_mx < val) ? val : st_mx;
+        }
+        if (!train) continue;
+        const double v2 = val * val;
+        // alpha training T_j (MultiChannel.hh:174-177): wave sums added per round
         for (int j = 0; j < C; ++j) {
             double t = valid ? v2 * W / psbuf[(int64_t)j * psstride + gt] : 0.0;
             t = wave_sum(t);
-            if (lane == 0) s_small[w * (8 + C) + 8 + j] = t;
+            if (lane == 0) s_small[w * (8 + C) + 8 + j] += t;
+        }
+        // f^2 histogram of the picked channel (Integrand.hh:89-96): lane l of a
+        // wave owns bins l, l+64, ...; wave values in lane order, waves in turn.
+        __syncthreads();
+        for (int k = 0; k < D; ++k) {
+            const int bk = valid ? s_bins[k * kTile + threadIdx.x] : 0;
+            const double vk = valid ? v2 : 0.0;
+            double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+            for (int src = 0; src < 64; ++src) {
+                const int b = __builtin_amdgcn_readlane(bk, src);
+                const double v = readlane_f64(vk, src);
+                if ((b & 63) == lane) {
+                    const int hi = b >> 6;
+                    if (hi == 0) h0 += v; else if (hi == 1) h1 += v; else if (hi == 2) h2 += v; else h3 += v;
+                }
+            }
+            for (int ww = 0; ww < kWaves; ++ww) {
+                if (w == ww) {
+                    double hq[4] = {h0, h1, h2, h3};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int b = lane + 64 * q;
+                        if (b < B) s_hist[k * B + b] += hq[q];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // ---- StatsData partial of the workgroup ----
+    {
+        double n = wave_sum(st_n), s1 = wave_sum(st_s1), s2 = wave_sum(st_s2), nf = wave_sum(st_nf), nn = wave_sum(st_nn);
+        double mn = wave_min(st_mn), mx = wave_max(st_mx);
+        if (lane == 0) {
+            double* q = s_small + w * (8 + C);
+            q[0] = n; q[1] = mn; q[2] = mx; q[3] = s1; q[4] = s2; q[5] = nf; q[6] = nn;
         }
     }
     double* out = part + (int64_t)blockIdx.x * rec;
@@ -400,62 +470,68 @@ __global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict_
     }
     if (!train) return;
     for (int j = threadIdx.x; j < C; j += kTile) {
-        double s = 0;
-        for (int ww = 0; ww < kWaves; ++ww) s += s_small[ww * (8 + C) + 8 + j];
-        out[kStatW + j] = s;
-    }
-    // ---- f^2 histogram of the picked channel (Integrand.hh:89-96) ----
-    // Lane l of a wave owns bins l, l+64, ... and adds the wave's values in
-    // lane order; the waves then add into LDS one after the other.
-    for (int k = 0; k < D; ++k) {
-        const int bk = valid ? s_bins[k * kTile + threadIdx.x] : 0;
-        const double vk = valid ? v2 : 0.0;
-        double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
-        for (int src = 0; src < 64; ++src) {
-            const int b = __builtin_amdgcn_readlane(bk, src);
-            const double v = readlane_f64(vk, src);
-            if ((b & 63) == lane) {
-                const int hi = b >> 6;
-                if (hi == 0) h0 += v; else if (hi == 1) h1 += v; else if (hi == 2) h2 += v; else h3 += v;
-            }
-        }
-        for (int ww = 0; ww < kWaves; ++ww) {
-            if (w == ww) {
-                double hq[4] = {h0, h1, h2, h3};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int b = lane + 64 * q;
-                    if (b < B) s_hist[k * B + b] = (ww == 0) ? hq[q] : s_hist[k * B + b] + hq[q];
-                }
-            }
-            __syncthreads();
-        }
+        double sj = 0;
+        for (int ww = 0; ww < kWaves; ++ww) sj += s_small[ww * (8 + C) + 8 + j];
+        out[kStatW + j] = sj;
     }
     for (int i = threadIdx.x; i < D * B; i += kTile) out[kStatW + C + i] = s_hist[i];
 }
 
 // ---------------------------------------------------------------------------
-// k_reduce: fixed-order sums of the workgroup partials into the running
-// totals: tot = [stats(7), T(C), hist(C x d x B)].
+// Fixed-order reduction of the workgroup partials into the running totals
+// tot = [stats(7), T(C), hist(C x d x B)], in two levels so that no thread
+// walks every partial: k_reduce_groups sums groups of kRedGroup consecutive
+// partials (stats and T); k_reduce_final adds the group sums in order and sums
+// each histogram entry over its channel's partials.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_reduce(int C, int DB, const int* __restrict__ tile2, const double* __restrict__ part, int64_t rec,
-                                                double* __restrict__ tot, int train) {
+constexpr int kRedGroup = 64;
+
+__device__ __forceinline__ void stat_merge(double* a, const double* q) {
+    a[0] += q[0];
+    a[1] = (q[1] < a[1]) ? q[1] : a[1];
+    a[2] = (a[2] < q[2]) ? q[2] : a[2];
+    a[3] += q[3]; a[4] += q[4]; a[5] += q[5]; a[6] += q[6];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_groups(int C, const int* __restrict__ tile2, const double* __restrict__ part,
+                                                       int64_t rec, double* __restrict__ gsum, int train) {
+    const int ntile = tile2[C];
+    const int g = blockIdx.x;
+    const int t0 = g * kRedGroup;
+    if (t0 >= ntile) return;
+    const int t1 = min(ntile, t0 + kRedGroup);
+    const int W = kStatW + C;
+    double* out = gsum + (int64_t)g * W;
+    if (threadIdx.x == 0) {
+        double a[7] = {0, std::numeric_limits<double>::max(), std::numeric_limits<double>::min(), 0, 0, 0, 0};
+        for (int t = t0; t < t1; ++t) stat_merge(a, part + (int64_t)t * rec);
+        for (int k = 0; k < kStatW; ++k) out[k] = a[k];
+    }
+    if (!train) return;
+    for (int j = threadIdx.x; j < C; j += blockDim.x) {
+        double s = 0;
+        for (int t = t0; t < t1; ++t) s += part[(int64_t)t * rec + kStatW + j];
+        out[kStatW + j] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(int C, int DB, const int* __restrict__ tile2, const double* __restrict__ part,
+                                                      int64_t rec, const double* __restrict__ gsum, double* __restrict__ tot, int train) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int ntile = tile2[C];
+    const int ng = (ntile + kRedGroup - 1) / kRedGroup;
+    const int W = kStatW + C;
     if (i == 0) {
-        double n = tot[0], mn = tot[1], mx = tot[2], s1 = tot[3], s2 = tot[4], nf = tot[5], nn = tot[6];
-        for (int t = 0; t < ntile; ++t) {
-            const double* q = part + (int64_t)t * rec;
-            n += q[0]; mn = (q[1] < mn) ? q[1] : mn; mx = (mx < q[2]) ? q[2] : mx; s1 += q[3]; s2 += q[4]; nf += q[5]; nn += q[6];
-        }
-        tot[0] = n; tot[1] = mn; tot[2] = mx; tot[3] = s1; tot[4] = s2; tot[5] = nf; tot[6] = nn;
+        double a[7] = {tot[0], tot[1], tot[2], tot[3], tot[4], tot[5], tot[6]};
+        for (int g = 0; g < ng; ++g) stat_merge(a, gsum + (int64_t)g * W);
+        for (int k = 0; k < kStatW; ++k) tot[k] = a[k];
         return;
     }
     if (!train) return;
     const int64_t j = i - 1;
     if (j < C) {
         double s = tot[kStatW + j];
-        for (int t = 0; t < ntile; ++t) s += part[(int64_t)t * rec + kStatW + j];
+        for (int g = 0; g < ng; ++g) s += gsum[(int64_t)g * W + kStatW + j];
         tot[kStatW + j] = s;
         return;
     }
@@ -483,13 +559,13 @@ struct chili_mc {
     bool have_grids = false;
     // device buffers
     DevBuf<ChanProg> d_progs;
-    DevBuf<double> d_edges, d_search, d_alpha, d_cum, d_sext, d_tot, d_part, d_mom, d_ps, d_u;
+    DevBuf<double> d_edges, d_search, d_alpha, d_cum, d_sext, d_tot, d_part, d_mom, d_ps, d_u, d_gsum;
     DevBuf<chili_integrand> d_f;
     DevBuf<uint16_t> d_pick;
     DevBuf<uint32_t> d_sorted, d_comp;
     DevBuf<uint8_t> d_acc;
     DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff;
-    DevBuf<int64_t> d_total;
+    DevBuf<int64_t> d_total, d_bsum;
     int64_t nmax = 0;            // attempts per sub-batch
     double acc_rate = 0.25;      // running estimate of the cut acceptance
     // k_main timing with HIP events on the context's stream
@@ -510,6 +586,16 @@ namespace {
 
 int next_pow2(int x) { int p = 1; while (p < x) p <<= 1; return p; }
 
+void scan_int(chili_mc* m, const int* in, int* out, int64_t len);
+
+// Grid edges must be non-decreasing (AdaptiveMap's Adapt and Split keep
+// them so); the device bin search relies on it.
+void check_edges(const double* e, size_t rows, int B) {
+    for (size_t r = 0; r < rows; ++r)
+        for (int b = 0; b < B; ++b)
+            if (!(e[r * (B + 1) + b] <= e[r * (B + 1) + b + 1])) throw chili_error(CHILI_EINVAL, "grid edges must be non-decreasing");
+}
+
 // Host-fed uniforms must lie in [0, 1): 1.0 would map to bin B (past the
 // grid) exactly as it would index past the reference's edge array.
 void check_uniforms(const double* u, size_t cnt) {
@@ -522,7 +608,7 @@ void upload_state(chili_mc* m) {
     std::vector<double> srch((size_t)C * d * P2, std::numeric_limits<double>::infinity());
     for (int c = 0; c < C; ++c)
         for (int k = 0; k < d; ++k)
-            for (int b = 0; b < B; ++b) srch[((size_t)c * d + k) * P2 + b] = m->grids[((size_t)c * d + k) * (B + 1) + b];
+            for (int b = 1; b < B; ++b) srch[((size_t)c * d + k) * P2 + b - 1] = m->grids[((size_t)c * d + k) * (B + 1) + b];
     m->d_edges.upload(m->grids, m->stream);
     m->d_search.upload(srch, m->stream);
     // cumulative alpha as std::discrete_distribution builds it (random.tcc:2656-2677)
@@ -608,6 +694,14 @@ void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals
         default: throw chili_error(CHILI_EINVAL, "unsupported number of external particles"); \
     }
 
+void scan_int(chili_mc* m, const int* in, int* out, int64_t len) {
+    const int64_t nb = (len + kScanBlock - 1) / kScanBlock;
+    m->d_bsum.resize(nb);
+    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nb), dim3(kScanBlock), 0, m->stream, in, len, m->d_bsum.ptr);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, m->stream, m->d_bsum.ptr, nb, m->d_total.ptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanBlock), 0, m->stream, in, out, len, m->d_bsum.ptr);
+}
+
 std::pair<hipEvent_t, hipEvent_t> take_events(chili_mc* m) {
     if (!m->ev_pool.empty()) {
         auto e = m->ev_pool.back();
@@ -651,7 +745,7 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
     MCArgs A = make_args(m, u, ustride, seed, stream, a0);
     A.ua0 = ua0;
     hipLaunchKernelGGL(k_pick, dim3(nblk), dim3(kTile), sizeof(int) * C, m->stream, A, n, m->d_pick.ptr, m->d_counts.ptr, nblk);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, m->stream, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * C, m->d_total.ptr);
+    scan_int(m, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * C);
     hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kTile), sizeof(int) * kWaves * C, m->stream, C, n, m->d_pick.ptr, m->d_offs.ptr, nblk,
                        m->d_sorted.ptr);
     hipLaunchKernelGGL(k_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_offs.ptr, nblk, n, m->d_seg.ptr, m->d_tiles.ptr);
@@ -693,13 +787,13 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
     used = cut;
     hipLaunchKernelGGL(k_keep_count, dim3(tgrid), dim3(kTile), 0, m->stream, C, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr, m->d_acc.ptr,
                        (uint32_t)cut, m->d_kept.ptr);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, m->stream, m->d_kept.ptr, m->d_keptoff.ptr, (int64_t)tgrid, m->d_total.ptr);
+    scan_int(m, m->d_kept.ptr, m->d_keptoff.ptr, (int64_t)tgrid);
     hipLaunchKernelGGL(k_keep_scatter, dim3(tgrid), dim3(kTile), 0, m->stream, C, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr, m->d_acc.ptr,
                        (uint32_t)cut, m->d_keptoff.ptr, m->d_comp.ptr);
     hipLaunchKernelGGL(k_keep_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_tiles.ptr, m->d_keptoff.ptr, kept, m->d_seg2.ptr,
-                       m->d_tiles2.ptr);
+                       m->d_tiles2.ptr, kMainTile);
     if (kept > 0) {
-        const int mgrid = (int)((kept + kTile - 1) / kTile) + C;
+        const int mgrid = (int)((kept + kMainTile - 1) / kMainTile) + C;
         std::pair<hipEvent_t, hipEvent_t> ev{};
         if (m->timing) {
             ev = take_events(m);
@@ -713,8 +807,12 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
         }
         const int64_t rec = kStatW + C + (int64_t)m->d * m->B;
         const int64_t nout = 1 + (int64_t)C + (int64_t)C * m->d * m->B;
-        hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, m->stream, C, m->d * m->B, m->d_tiles2.ptr,
-                           m->d_part.ptr, rec, m->d_tot.ptr, train);
+        const int ngroups = (mgrid + kRedGroup - 1) / kRedGroup;
+        m->d_gsum.resize((size_t)ngroups * (kStatW + C));
+        hipLaunchKernelGGL(k_reduce_groups, dim3(ngroups), dim3(256), 0, m->stream, C, m->d_tiles2.ptr, m->d_part.ptr, rec,
+                           m->d_gsum.ptr, train);
+        hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, m->stream, C, m->d * m->B,
+                           m->d_tiles2.ptr, m->d_part.ptr, rec, m->d_gsum.ptr, m->d_tot.ptr, train);
     }
 }
 
@@ -832,6 +930,7 @@ int chili_mc_set_grids(chili_mc* m, const double* grids, int nbins) {
             m->d_tot.resize(kStatW + m->C + (size_t)m->C * m->d * m->B);
             m->nmax = 0;   // record sizes changed
         }
+        check_edges(grids, (size_t)m->C * m->d, nbins);
         m->grids.assign(grids, grids + (size_t)m->C * m->d * (m->B + 1));
         upload_state(m);
     });
@@ -886,7 +985,11 @@ int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_
         int64_t need = ncalls, att = 0, rejected = 0, got = 0;
         double prev_nan = 0;
         const int64_t cap = std::max<int64_t>(int64_t(1) << 16, std::min<int64_t>(int64_t(1) << 23, 8 * ncalls + 4096));
+        // the reference loops until ncalls points pass; stop instead of spinning
+        // when the cut (or NaN values) leave essentially nothing
+        const int64_t max_attempts = 100000 * ncalls + (int64_t(1) << 24);
         while (need > 0) {
+            if (att > max_attempts) throw chili_error(CHILI_ERANGE, "no attempts pass the cut: retry limit reached");
             int64_t n = (int64_t)((double)need / std::max(m->acc_rate, 0.02) * 1.08) + 2048;
             n = std::min(n, cap);
             if (ud) {

@@ -47,7 +47,7 @@ struct PointRec {
 struct MCArgs {
     const ChanProg* progs;
     const double* edges;     // [C][d][B+1]
-    const double* search;    // [C][d][P2] lower edges e_0..e_{B-1} padded with +inf
+    const double* search;    // [C][d][P2] interior edges e_1..e_{B-1} padded with +inf
     const double* alpha;     // [C]
     const double* cum;       // [C] cumulative alpha (last entry 1.0)
     const double* sext;      // [ntot] external masses^2 (0 for incoming)
@@ -60,6 +60,19 @@ struct MCArgs {
     uint32_t stream;
     int C, d, B, P2;
     double sqrts, ptmax;
+};
+
+// --------------------------------------------------------------------------
+// Momentum slots of one point, indexed by wave-uniform slot numbers: four
+// scalar arrays touched only through get/set so that the compiler can keep
+// them in VGPRs (indexed register moves) instead of private memory.
+// --------------------------------------------------------------------------
+template <int N>
+struct MomRF {
+    double v[4 * N];
+    CHILI_DEV V4 get(int i) const { return mk(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]); }
+    CHILI_DEV void set(int i, const V4& q) { v[4 * i] = q.t; v[4 * i + 1] = q.x; v[4 * i + 2] = q.y; v[4 * i + 3] = q.z; }
+    CHILI_DEV double mass2(int i) const { return vmass2(get(i)); }
 };
 
 // --------------------------------------------------------------------------
@@ -97,14 +110,15 @@ CHILI_DEV int pick_channel(const MCArgs& A, double u) {      // lower_bound(cum,
     return lo;
 }
 
-// FindBin (AdaptiveMap.cc:34-38): number of lower edges e_0..e_{B-1} below x,
-// minus one; x <= e_0 or NaN is clamped to bin 0.  Branch-free search over
-// the +inf-padded power-of-two table.
+// FindBin (AdaptiveMap.cc:34-38): lower_bound over e_0..e_{B-1} minus one,
+// with x <= e_0 or NaN clamped to bin 0.  For non-decreasing edges that is the
+// number of interior edges e_1..e_{B-1} below x, found by a branch-free
+// search over the +inf-padded power-of-two table (P2 >= B).
 CHILI_DEV int find_bin(const double* __restrict__ s, int P2, double x) {
     int lo = 0;
     for (int step = P2 >> 1; step > 0; step >>= 1)
         lo += (s[lo + step - 1] < x) ? step : 0;
-    return lo == 0 ? 0 : lo - 1;
+    return lo;
 }
 
 // --------------------------------------------------------------------------
@@ -112,7 +126,7 @@ CHILI_DEV int find_bin(const double* __restrict__ s, int P2, double x) {
 // P: momentum slots, m2: masses^2 per slot, r: the VEGAS-mapped point.
 // --------------------------------------------------------------------------
 template <int NT>
-CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* r, V4 (&P)[2 * NT], double (&m2)[2 * NT]) {
+CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* r, MomRF<2 * NT>& P, double (&m2)[2 * NT]) {
     constexpr int ND = NT - 3;       // max decays
     constexpr int NTT = NT - 2;      // max t-channel objects
 #pragma unroll
@@ -151,7 +165,7 @@ CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* 
             double sp, cp;
             sincos(phi, &sp, &cp);
             V4 q = mk(mt * coshy, pt * cp, pt * sp, mt * sinhy);
-            P[p.t_slot[i]] = q;
+            P.set(p.t_slot[i], q);
             psum = vadd(psum, q);
         }
     }
@@ -170,12 +184,12 @@ CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* 
         double sinhy = sinh(yv);
         double coshy = sqrt(1 + sinhy * sinhy);
         V4 q = mk(qt * coshy, -psum.x, -psum.y, qt * sinhy);
-        P[last] = q;
+        P.set(last, q);
         V4 tot = vadd(q, psum);
         double pp = tot.t + tot.z;
         double pm = tot.t - tot.z;
-        P[0] = mk(-pp / 2, 0., 0., -pp / 2);
-        P[1] = mk(-pm / 2, 0., 0., pm / 2);
+        P.set(0, mk(-pp / 2, 0., 0., -pp / 2));
+        P.set(1, mk(-pm / 2, 0., 0., pm / 2));
     }
     // GenSChan / DecayParts, Channel.cc:204-245
 #pragma unroll
@@ -184,9 +198,9 @@ CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* 
             double r1 = r[ir++];
             double r2 = r[ir++];
             V4 pa, pb;
-            schannel_momenta(P[p.s_par[k]], m2[p.s_a[k]], m2[p.s_b[k]], pa, pb, r1, r2);
-            P[p.s_a[k]] = pa;
-            P[p.s_b[k]] = pb;
+            schannel_momenta(P.get(p.s_par[k]), m2[p.s_a[k]], m2[p.s_b[k]], pa, pb, r1, r2);
+            P.set(p.s_a[k], pa);
+            P.set(p.s_b[k], pb);
         }
     }
 }
@@ -198,7 +212,7 @@ CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* 
 // picked channel the bins of its inverse point go to `bins_out` (LDS).
 // --------------------------------------------------------------------------
 template <int NT>
-CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, V4 (&P)[2 * NT], double* vw_out,
+CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, MomRF<2 * NT>& P, double* vw_out,
                                  short* bins_out, int bstride, double* x_out = nullptr, int32_t* bin_rec = nullptr,
                                  int64_t xstride = 0) {
     constexpr int ND = NT - 3;
@@ -220,16 +234,16 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, V4 
     // composite momenta, ascending decays (Channel.cc:47-50)
 #pragma unroll
     for (int k = 0; k < ND; ++k)
-        if (k < nd) P[p.dec_par[k]] = vadd(P[p.dec_c1[k]], P[p.dec_c2[k]]);
+        if (k < nd) P.set(p.dec_par[k], vadd(P.get(p.dec_c1[k]), P.get(p.dec_c2[k])));
     // WgtDecays, Channel.cc:83-113
     double wd = 1;
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
         if (k < nd) {
-            double a = sqrt(fabs(vmass2(P[p.dec_c1[k]]))) + sqrt(fabs(vmass2(P[p.dec_c2[k]])));
+            double a = sqrt(fabs(P.mass2(p.dec_c1[k]))) + sqrt(fabs(P.mass2(p.dec_c2[k])));
             double smin1 = a * a;
             double smin = dmax(dmax(smin1, p.dec_smin2[k]), p.dec_smincut[k]);
-            double s = vmass2(P[p.dec_par[k]]);
+            double s = P.mass2(p.dec_par[k]);
             double xr;
             if (p.dec_massive[k]) wd *= massive_prop_weight(p.dec_mass[k], p.dec_width[k], smin, p.dec_smax[k], s, xr);
             else wd *= massless_prop_weight(smin, p.dec_smax[k], s, xr);
@@ -245,7 +259,7 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, V4 
     for (int i = 0; i < NTT - 1; ++i) {
         if (i < nt - 1) {
             wt *= 1.0 / (16 * kPi3);
-            const V4 m = P[p.t_slot[i]];
+            const V4 m = P.get(p.t_slot[i]);
             const double ptmin = p.t_ptmin[i];
             double pt = vpt(m);
             double tp = 2 * ptmin + pt;
@@ -264,7 +278,7 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, V4 
         }
     }
     {
-        const V4 ml = P[p.t_slot[nt - 1]];
+        const V4 ml = P.get(p.t_slot[nt - 1]);
         double mjets2 = vmass2(psum);
         double yjets = vrap(psum);
         double ptj2 = vpt2(psum);
@@ -292,7 +306,7 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, V4 
     for (int k = 0; k < ND; ++k) {
         if (k < nd) {
             double r1, r2;
-            ws *= schannel_weight(P[p.s_a[k]], P[p.s_b[k]], r1, r2);
+            ws *= schannel_weight(P.get(p.s_a[k]), P.get(p.s_b[k]), r1, r2);
             take(r1);
             take(r2);
         }
@@ -307,7 +321,10 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, V4 
 CHILI_DEV double pair_s(const V4& a, const V4& b) { return vmass2(vadd(a, b)); }
 
 template <int NT>
-CHILI_DEV bool cut_pass(const chili_integrand* __restrict__ f, const V4 (&P)[2 * NT], double sqrts) {
+CHILI_DEV bool cut_pass(const chili_integrand* __restrict__ f, const MomRF<2 * NT>& PR, double sqrts) {
+    V4 P[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) P[i] = PR.get(i);
     if (f->cuts == CHILI_CUTS_NONE) return true;
     if (isnan(P[0].t)) return false;
     // physical incoming momenta: x1, x2 <= 1 (incoming legs carry E = -x sqrt(s)/2)
@@ -336,7 +353,10 @@ CHILI_DEV bool cut_pass(const chili_integrand* __restrict__ f, const V4 (&P)[2 *
 }
 
 template <int NT>
-CHILI_DEV double integrand(const chili_integrand* __restrict__ f, const V4 (&P)[2 * NT]) {
+CHILI_DEV double integrand(const chili_integrand* __restrict__ f, const MomRF<2 * NT>& PR) {
+    V4 P[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) P[i] = PR.get(i);
     const int kind = f->kind;
     if (kind == CHILI_F_UNIT) return 1.0;
     constexpr int NP = (NT - 2) * (NT - 3) / 2;

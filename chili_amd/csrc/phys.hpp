@@ -38,8 +38,10 @@ CHILI_DEV double vphi(const V4& a) {                                            
 }
 CHILI_DEV double dmax(double a, double b) { return (a < b) ? b : a; }   // std::max
 CHILI_DEV double dmin(double a, double b) { return (b < a) ? b : a; }   // std::min
-CHILI_DEV double comp(const V4& v, int i) { return i == 1 ? v.x : (i == 2 ? v.y : v.z); }
-CHILI_DEV void set_comp(V4& v, int i, double s) { if (i == 1) v.x = s; else if (i == 2) v.y = s; else v.z = s; }
+// Spatial component i in {1, 2, 3} of a four-vector, by value selects only (no
+// address arithmetic, so the vector stays in registers).
+CHILI_DEV double comp(const V4 v, int i) { return i == 1 ? v.x : (i == 2 ? v.y : v.z); }
+CHILI_DEV V4 with_comp(const V4 v, int i, double s) { return mk(v.t, i == 1 ? s : v.x, i == 2 ? s : v.y, i == 3 ? s : v.z); }
 
 CHILI_DEV double small_omct(const V4& a, const V4& v) {                                                        // FourVector.cc:57-65
     double mag = sqrt(vp2(a) * vp2(v));
@@ -75,16 +77,16 @@ CHILI_DEV V4 rotation_pt(const V4& v1, const V4& v2) {
     V4 t = vadd(b, vscale(l, lb));
     double mt = vp2(t);
     if (mt != 0.) t = vscale(t, 1. / sqrt(mt));
+    // the reference's three-compare sorting network over |l_i|, carrying the
+    // index together with its value
     int i0 = 1, i1 = 2, i2 = 3;
-    double a1 = fabs(l.x), a2 = fabs(l.y), a3 = fabs(l.z);
-    // the reference's three-compare sorting network over |l_i| (indices by value)
-    auto ml = [&](int i) { return i == 1 ? a1 : (i == 2 ? a2 : a3); };
-    if (ml(i2) > ml(i1)) { int s = i1; i1 = i2; i2 = s; }
-    if (ml(i1) > ml(i0)) { int s = i0; i0 = i1; i1 = s; }
-    if (ml(i2) > ml(i1)) { int s = i1; i1 = i2; i2 = s; }
+    double m0 = fabs(l.x), m1 = fabs(l.y), m2 = fabs(l.z);
+    if (m2 > m1) { int s = i1; i1 = i2; i2 = s; double q = m1; m1 = m2; m2 = q; }
+    if (m1 > m0) { int s = i0; i0 = i1; i1 = s; double q = m0; m0 = m1; m1 = q; }
+    if (m2 > m1) { int s = i1; i1 = i2; i2 = s; double q = m1; m1 = m2; m2 = q; }
     double tdp = comp(t, i1) * comp(l, i1) + comp(t, i2) * comp(l, i2);
-    if (tdp != 0.) set_comp(t, i0, -tdp / comp(l, i0));
-    if (vp2(t) == 0.) set_comp(t, i1, 1.);
+    if (tdp != 0.) t = with_comp(t, i0, -tdp / comp(l, i0));
+    if (vp2(t) == 0.) t = with_comp(t, i1, 1.);
     return t;
 }
 

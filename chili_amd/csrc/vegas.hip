@@ -199,7 +199,7 @@ namespace {
 void vupload(chili_vegas* v) {
     std::vector<double> s((size_t)v->d * v->P2, std::numeric_limits<double>::infinity());
     for (int k = 0; k < v->d; ++k)
-        for (int b = 0; b < v->B; ++b) s[(size_t)k * v->P2 + b] = v->edges[(size_t)k * (v->B + 1) + b];
+        for (int b = 1; b < v->B; ++b) s[(size_t)k * v->P2 + b - 1] = v->edges[(size_t)k * (v->B + 1) + b];
     v->d_edges.upload(v->edges, v->stream);
     v->d_search.upload(s, v->stream);
 }
@@ -264,6 +264,10 @@ int chili_vegas_set_grid(chili_vegas* v, const double* edges, int nbins) {
         int p = 1;
         while (p < nbins) p <<= 1;
         v->P2 = p;
+        for (int k = 0; k < v->d; ++k)
+            for (int b = 0; b < nbins; ++b)
+                if (!(edges[(size_t)k * (nbins + 1) + b] <= edges[(size_t)k * (nbins + 1) + b + 1]))
+                    throw chili_error(CHILI_EINVAL, "grid edges must be non-decreasing");
         v->edges.assign(edges, edges + (size_t)v->d * (nbins + 1));
         v->cap_tiles = 0;
         vupload(v);

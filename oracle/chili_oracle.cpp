@@ -966,6 +966,16 @@ static size_t pick_channel(const std::vector<double>& w, double u) {
 
 extern "C" {
 
+int oracle_pick_channel(int n, const double* alpha, double u) {
+    return (int)pick_channel(std::vector<double>(alpha, alpha + n), u);
+}
+double oracle_stats(int n, const double* x, chili_stats* out) {
+    Stats s;
+    for (int i = 0; i < n; ++i) s.add(x[i]);
+    s.out(out);
+    return s.err();
+}
+
 double oracle_philox_uniform(uint64_t seed, uint32_t stream, uint64_t attempt, uint32_t k) {
     return philox_uniform(seed, stream, attempt, k);
 }
@@ -1081,15 +1091,22 @@ struct Attempt {
     std::vector<Real> xpick;
     std::vector<double> ps;
 };
-static void run_attempt(OProc* o, const chili_uniforms* U, uint64_t a, Attempt& A, size_t d) {
+// mom_in (optional, ntot x 4): use these momenta instead of the forward map's,
+// so that the weight stage can be compared on identical inputs.
+static void run_attempt(OProc* o, const chili_uniforms* U, uint64_t a, Attempt& A, size_t d, const double* mom_in = nullptr) {
     const size_t C = o->maps.size();
     std::vector<Real> r(d);
     for (size_t k = 0; k < d; ++k) r[k] = pick_uniform(U, a, (uint32_t)k);
     double uch = pick_uniform(U, a, (uint32_t)d);
     A.pick = pick_channel(o->alpha, uch);                                                                          // MultiChannel.hh:147
-    grid_forward(o->grids[A.pick], r.data());                                                                      // Integrand.hh:108 (jacobian dropped)
-    o->maps[A.pick].generate_point(A.mom, r.data());
-    if (!cut_pass(o->f, A.mom, o->sqrts)) { A.status = 0; return; }                                                          // MultiChannel.hh:219-223
+    if (mom_in) {
+        A.mom.resize(o->ntot);
+        for (size_t i = 0; i < o->ntot; ++i) A.mom[i] = mk(mom_in[4 * i], mom_in[4 * i + 1], mom_in[4 * i + 2], mom_in[4 * i + 3]);
+    } else {
+        grid_forward(o->grids[A.pick], r.data());                                                                  // Integrand.hh:108 (jacobian dropped)
+        o->maps[A.pick].generate_point(A.mom, r.data());
+    }
+    if (!cut_pass(o->f, A.mom, o->sqrts)) { A.status = 0; return; }                                                // MultiChannel.hh:219-223
     // Integrand::GenerateWeight, Integrand.hh:112-123 (shared _rans vector)
     std::vector<Real> xr(o->ntot, Real(0.));
     A.ps.assign(C, 0.0);
@@ -1112,14 +1129,15 @@ static void run_attempt(OProc* o, const chili_uniforms* U, uint64_t a, Attempt& 
 // Per-attempt records for attempts [a0, a0+n): pick, status, W, f, val,
 // momenta (ntot*4), x of the picked channel's inverse (d), its FindBin bins
 // (d) and the phase-space weights of every channel (C).
-int oracle_mc_points(void* h, const chili_uniforms* U, uint64_t a0, int64_t n,
-                     int32_t* pick, int32_t* status, double* w, double* f, double* val,
-                     double* mom, double* xpick, int32_t* bins, double* ps) {
+// mom_in (optional, n x ntot x 4) replaces the forward map's momenta.
+int oracle_mc_points2(void* h, const chili_uniforms* U, uint64_t a0, int64_t n, const double* mom_in,
+                      int32_t* pick, int32_t* status, double* w, double* f, double* val,
+                      double* mom, double* xpick, int32_t* bins, double* ps) {
     OProc* o = (OProc*)h;
     const size_t C = o->maps.size(), d = o->maps[0].ndims(), nt = o->ntot;
     for (int64_t i = 0; i < n; ++i) {
         Attempt A;
-        run_attempt(o, U, a0 + (uint64_t)i, A, d);
+        run_attempt(o, U, a0 + (uint64_t)i, A, d, mom_in ? mom_in + (size_t)i * nt * 4 : nullptr);
         if (pick) pick[i] = (int32_t)A.pick;
         if (status) status[i] = A.status;
         if (w) w[i] = A.w;
@@ -1180,13 +1198,20 @@ int oracle_count_points(void* h, const chili_uniforms* U, uint64_t a0, int64_t n
     return 0;
 }
 
+int oracle_mc_points(void* h, const chili_uniforms* U, uint64_t a0, int64_t n,
+                     int32_t* pick, int32_t* status, double* w, double* f, double* val,
+                     double* mom, double* xpick, int32_t* bins, double* ps) {
+    return oracle_mc_points2(h, U, a0, n, nullptr, pick, status, w, f, val, mom, xpick, bins, ps);
+}
+
 // One iteration of MultiChannel::operator() (MultiChannel.hh:196-251) with
 // should_optimize == train: accumulates StatsData, the alpha training sums
 // T[C] (MultiChannel.hh:167-180) and every channel's VEGAS f^2 histogram
 // hist[c][dim][bin] (Integrand.hh:89-96).  Attempts start at a0; the number
 // used is returned in *n_attempts.
-int oracle_mc_iterate(void* h, const chili_uniforms* U, uint64_t a0, int64_t ncalls, int train,
-                      chili_stats* stats, double* T, double* hist, int64_t* n_attempts) {
+int oracle_mc_iterate2(void* h, const chili_uniforms* U, uint64_t a0, int64_t ncalls, int train,
+                       const double* mom_in, int64_t n_in,
+                       chili_stats* stats, double* T, double* hist, int64_t* n_attempts) {
     OProc* o = (OProc*)h;
     const size_t C = o->maps.size(), d = o->maps[0].ndims(), B = o->grids[0].bins;
     Stats res;
@@ -1194,8 +1219,10 @@ int oracle_mc_iterate(void* h, const chili_uniforms* U, uint64_t a0, int64_t nca
     uint64_t a = a0;
     for (int64_t i = 0; i < ncalls; ++i) {
         if (U->u && (int64_t)(a - a0) >= U->stride) return -2;
+        if (mom_in && (int64_t)(a - a0) >= n_in) return -2;
         Attempt A;
-        run_attempt(o, U, a++, A, d);
+        run_attempt(o, U, a, A, d, mom_in ? mom_in + (size_t)(a - a0) * o->ntot * 4 : nullptr);
+        ++a;
         if (A.status == 0) { --i; res.add(0); continue; }
         if (A.status == 2) --i;
         if (train) {
@@ -1211,6 +1238,11 @@ int oracle_mc_iterate(void* h, const chili_uniforms* U, uint64_t a0, int64_t nca
     if (hist) std::copy(hh.begin(), hh.end(), hist);
     *n_attempts = (int64_t)(a - a0);
     return 0;
+}
+
+int oracle_mc_iterate(void* h, const chili_uniforms* U, uint64_t a0, int64_t ncalls, int train,
+                      chili_stats* stats, double* T, double* hist, int64_t* n_attempts) {
+    return oracle_mc_iterate2(h, U, a0, ncalls, train, nullptr, 0, stats, T, hist, n_attempts);
 }
 
 // MultiChannel::Adapt, src/Channel/MultiChannel.cc:109-136

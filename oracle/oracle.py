@@ -34,6 +34,8 @@ def lib(count=False):
         L.oracle_generate_weight.restype = C.c_double
         L.oracle_generate_weight.argtypes = [P, C.c_int, D, D]
         L.oracle_mc_points.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, I32, I32, D, D, D, D, D, I32, D]
+        L.oracle_mc_points2.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, D, I32, I32, D, D, D, D, D, I32, D]
+        L.oracle_mc_iterate2.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_int, D, I64, C.POINTER(abi.Stats), D, D, C.POINTER(C.c_int64)]
         L.oracle_mc_iterate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_int, C.POINTER(abi.Stats), D, D, C.POINTER(C.c_int64)]
         L.oracle_alpha_adapt.argtypes = [C.c_int, D, D, C.c_double, C.c_double]
         L.oracle_max_difference.restype = C.c_double
@@ -51,6 +53,9 @@ def lib(count=False):
         L.oracle_philox_uniform.restype = C.c_double
         L.oracle_philox_uniform.argtypes = [U64, U32, U64, U32]
         L.oracle_philox_block.argtypes = [C.POINTER(C.c_uint32), U32, U32]
+        L.oracle_pick_channel.argtypes = [C.c_int, D, C.c_double]
+        L.oracle_stats.restype = C.c_double
+        L.oracle_stats.argtypes = [C.c_int, D, C.POINTER(abi.Stats)]
         L.oracle_count_points.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, D]
         L.oracle_count_get.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
         for nm in ("oracle_sqlam", "oracle_massless_prop_momenta", "oracle_massive_prop_momenta"):
@@ -135,14 +140,16 @@ class Oracle:
         w = self.L.oracle_generate_weight(self.h, ch, _d(mom), _d(x))
         return w, x
 
-    def points(self, uni, a0, n):
+    def points(self, uni, a0, n, mom_in=None):
+        """Per-attempt records; mom_in (n, ntot, 4) replaces the forward map."""
         d, c, nt = self.ndims, self.nchannels, self.ntot
         out = dict(pick=np.zeros(n, np.int32), status=np.zeros(n, np.int32), w=np.zeros(n), f=np.zeros(n),
                    val=np.zeros(n), mom=np.zeros((n, nt, 4)), xpick=np.zeros((n, d)),
                    bins=np.zeros((n, d), np.int32), ps=np.zeros((n, c)))
-        self.L.oracle_mc_points(self.h, C.byref(uni), a0, n, _i(out["pick"]), _i(out["status"]), _d(out["w"]),
-                                _d(out["f"]), _d(out["val"]), _d(out["mom"]), _d(out["xpick"]), _i(out["bins"]),
-                                _d(out["ps"]))
+        mi = None if mom_in is None else np.ascontiguousarray(mom_in, dtype=np.float64)
+        self.L.oracle_mc_points2(self.h, C.byref(uni), a0, n, _d(mi) if mi is not None else None, _i(out["pick"]),
+                                 _i(out["status"]), _d(out["w"]), _d(out["f"]), _d(out["val"]), _d(out["mom"]),
+                                 _d(out["xpick"]), _i(out["bins"]), _d(out["ps"]))
         return out
 
     def count_points(self, uni, a0, n):
@@ -150,12 +157,16 @@ class Oracle:
         self.L.oracle_count_points(self.h, C.byref(uni), a0, n, _d(out))
         return dict(fwd_arith=out[0], fwd_trans=out[1], wgt_arith=out[2], wgt_trans=out[3], accepted=out[4], attempts=n)
 
-    def iterate(self, uni, ncalls, a0=0, train=True):
+    def iterate(self, uni, ncalls, a0=0, train=True, mom_in=None):
+        """One MultiChannel iteration; mom_in (n, ntot, 4) replaces the forward map."""
         st = abi.Stats()
         T = np.zeros(self.nchannels)
         hist = np.zeros((self.nchannels, self.ndims, self.nbins))
         na = C.c_int64()
-        rc = self.L.oracle_mc_iterate(self.h, C.byref(uni), a0, ncalls, 1 if train else 0, C.byref(st), _d(T), _d(hist), C.byref(na))
+        mi = None if mom_in is None else np.ascontiguousarray(mom_in, dtype=np.float64)
+        rc = self.L.oracle_mc_iterate2(self.h, C.byref(uni), a0, ncalls, 1 if train else 0,
+                                       _d(mi) if mi is not None else None, 0 if mi is None else mi.shape[0],
+                                       C.byref(st), _d(T), _d(hist), C.byref(na))
         if rc != 0:
             raise RuntimeError(f"oracle_mc_iterate rc={rc}")
         return st, T, hist, na.value
@@ -225,3 +236,62 @@ def philox_block(ctr, key):
     c = (C.c_uint32 * 4)(*ctr)
     lib().oracle_philox_block(c, key[0], key[1])
     return [c[i] for i in range(4)]
+
+
+def pick_channel(alpha, u):
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    return lib().oracle_pick_channel(len(a), _d(a), u)
+
+
+def stats(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    st = abi.Stats()
+    err = lib().oracle_stats(len(x), _d(x), C.byref(st))
+    return st, err
+
+
+def integrand(f, mom, sqrts):
+    m = np.ascontiguousarray(mom, dtype=np.float64)
+    ok = C.c_int()
+    v = lib().oracle_integrand(C.byref(f), m.shape[0], _d(m), sqrts, C.byref(ok))
+    return v, bool(ok.value)
+
+
+def eemumu(f, x):
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().oracle_eemumu(C.byref(f), _d(xx))
+
+
+def massless_prop(smin, smax, ran, sexp=0.99):
+    L = lib()
+    s = L.oracle_massless_prop_momenta(sexp, smin, smax, ran)
+    r = C.c_double()
+    w = L.oracle_massless_prop_weight(sexp, smin, smax, s, C.byref(r))
+    return s, w, r.value
+
+
+def massive_prop(m, g, smin, smax, ran):
+    L = lib()
+    s = L.oracle_massive_prop_momenta(m, g, smin, smax, ran)
+    r = C.c_double()
+    w = L.oracle_massive_prop_weight(m, g, smin, smax, s, C.byref(r))
+    return s, w, r.value
+
+
+def schannel_momenta(p, s1, s2, r1, r2):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    p1, p2 = np.zeros(4), np.zeros(4)
+    lib().oracle_schannel_momenta(_d(p), s1, s2, r1, r2, _d(p1), _d(p2))
+    return p1, p2
+
+
+def schannel_weight(p1, p2):
+    a = np.ascontiguousarray(p1, dtype=np.float64)
+    b = np.ascontiguousarray(p2, dtype=np.float64)
+    r1, r2 = C.c_double(), C.c_double()
+    w = lib().oracle_schannel_weight(_d(a), _d(b), C.byref(r1), C.byref(r2))
+    return w, r1.value, r2.value
+
+
+def sqlam(s, s1, s2):
+    return lib().oracle_sqlam(s, s1, s2)

@@ -1,0 +1,447 @@
+"""Pins of the CPU oracle (oracle/chili_oracle.cpp).
+
+The reference ships no tests, golden vectors or fixtures for this path and
+cannot be built here (its build fetches fmt/spdlog/yaml-cpp), so every oracle
+function is pinned by a known-answer vector, a closed form, an invariant, a
+library routine or brute force -- chosen so that a dropped term, a wrong
+sign or a transposed index fails:
+
+  Philox4x32-10 ............ Random123 known-answer vectors
+  channel pick ............. libstdc++ std::discrete_distribution itself (compiled probe)
+  ConstructChannels ........ set-partition counts (Bell numbers, brute force), reference example
+  FindBin .................. bisect_left (textbook lower_bound), edge cases
+  grid map / weight ........ identity on the uniform grid, numpy.interp, product of widths
+  grid Adapt ............... fixed point on uniform data, all-zero row, equal-mass invariant
+  grid Split ............... closed form
+  MultiChannel::Adapt ...... hand-computed closed form
+  StatsData ................ numpy mean / unbiased variance
+  s sampling (both) ........ weight == |ds/dr| (finite differences), round trip
+  two-body decay ........... momentum conservation, on-shell masses, round trip,
+                             weight == sqrt(lambda)/(8 pi s)
+  FSMapper ................. round trip x -> p -> x, sum p = 0, masses, channel
+                             consistency of the phase-space integral
+  integrands / cut ......... explicit numpy restatement of the definitions
+  plain VEGAS loop ......... integral vs scipy quadrature of the closed-form sigma(s)
+  retry loop ............... iterate() against the per-attempt records
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from chili_amd import abi
+from chili_amd import processes as P
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# ---------------------------------------------------------------------------
+def test_philox_known_answers():
+    # Random123 kat_vectors for philox4x32_10
+    assert O.philox_block([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert O.philox_block([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert O.philox_block([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_philox_uniform_layout():
+    seed, stream, att = 0x1234567890ABCDEF, 7, (5 << 32) + 11
+    for k in range(6):
+        c = O.philox_block([att & 0xffffffff, att >> 32, k >> 1, stream], [seed & 0xffffffff, seed >> 32])
+        hi, lo = (c[2], c[3]) if k & 1 else (c[0], c[1])
+        want = ((hi << 21) | (lo >> 11)) * 2.0 ** -53
+        assert O.philox_uniform(seed, stream, att, k) == want
+    u = np.array([O.philox_uniform(1, 0, a, 0) for a in range(20000)])
+    assert 0 <= u.min() and u.max() < 1 and abs(u.mean() - 0.5) < 0.01
+
+
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def discrete_probe(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("probe") / "discrete_ref")
+    subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(HERE, "golden", "discrete_ref.cpp"), "-o", exe], check=True)
+    return exe
+
+
+def test_channel_pick_matches_libstdcxx(discrete_probe):
+    rng = np.random.default_rng(3)
+    for n in (2, 3, 15, 52):
+        w = rng.uniform(0.0, 1.0, n)
+        w[rng.integers(0, n)] = 0.0          # a zero-weight channel is never picked
+        words = rng.integers(0, 2 ** 32, (400, 2), dtype=np.uint64)
+        words[0] = (0, 0)                     # u = 0
+        words[1] = (2 ** 32 - 1, 2 ** 32 - 1)  # u just below 1
+        inp = f"{n}\n" + " ".join(repr(float(x)) for x in w) + f"\n{len(words)}\n" + \
+            "\n".join(f"{a} {b}" for a, b in words) + "\n"
+        out = subprocess.run([discrete_probe], input=inp, capture_output=True, text=True, check=True).stdout.split("\n")
+        for line in out:
+            if not line.strip():
+                continue
+            us, ks = line.split()
+            assert O.pick_channel(w, float.fromhex(us)) == int(ks)
+
+
+# ---------------------------------------------------------------------------
+def partitions_with_cost(n, smax):
+    """Brute-force count of set partitions of n labelled items with
+    sum(|block| - 1) <= smax."""
+    def rec(items, cost):
+        if not items:
+            return 1
+        first, rest = items[0], items[1:]
+        tot = 0
+        for r in range(len(rest) + 1):
+            for combo in __import__("itertools").combinations(rest, r):
+                c = cost + r
+                if c > smax:
+                    continue
+                tot += rec([x for x in rest if x not in combo], c)
+        return tot
+    return rec(list(range(n)), 0)
+
+
+@pytest.mark.parametrize("n,smax", [(2, 0), (2, 1), (3, 1), (3, 2), (4, 1), (4, 2), (4, 3), (5, 2), (5, 4)])
+def test_construct_channels_counts(n, smax):
+    o = O.Oracle(P.gluon_process(n, smax=smax), 8, P.integrand_unit())
+    assert o.nchannels == partitions_with_cost(n, smax)
+    assert o.ndims == 3 * n - 2
+    if smax == n - 1:
+        bell = [1, 1, 2, 5, 15, 52, 203]
+        assert o.nchannels == bell[n]
+
+
+def test_construct_channels_reference_example():
+    # src/example.cc:98 -- g g -> g g at smax = 0: one channel, t-objects 8 > 4
+    o = O.Oracle(P.example_process(), 2, P.integrand_unit())
+    assert o.channels() == [(((8, 21, 0.0, 0.0), (4, 21, 0.0, 0.0)), ())]
+
+
+def test_construct_channels_structure():
+    o = O.Oracle(P.gluon_process(5, smax=4), 8, P.integrand_unit())
+    full = sum(1 << i for i in range(2, 7))
+    for info, dec in o.channels():
+        idx = [p[0] for p in info]
+        assert idx == sorted(idx, reverse=True)
+        assert sum(idx) == full and all((a & b) == 0 for i, a in enumerate(idx) for b in idx[i + 1:])
+        parents = [d[0][0] for d in dec]
+        assert parents == sorted(parents)
+        for par, c1, c2 in dec:
+            assert c1[0] | c2[0] == par[0] and (c1[0] & c2[0]) == 0
+        comp = [x for x in idx if bin(x).count("1") > 1]
+        assert len(dec) == sum(bin(x).count("1") - 1 for x in comp)
+
+
+def test_tt_channels():
+    o = O.Oracle(P.tt_process(smax=5), 64, P.integrand_resonant())
+    ch = o.channels()
+    assert len(ch) == 2 and o.ndims == 16
+    tchan = [tuple(p[0] for p in c[0]) for c in ch]
+    assert (252,) in tchan and (224, 28) in tchan
+
+
+# ---------------------------------------------------------------------------
+def random_edges(rng, dims, B):
+    return np.array([np.concatenate([[0.0], np.sort(rng.uniform(0, 1, B - 1)), [1.0]]) for _ in range(dims)])
+
+
+def test_find_bin_lower_bound():
+    import bisect
+    rng = np.random.default_rng(1)
+    e = random_edges(rng, 3, 17)
+    for dim in range(3):
+        row = list(e[dim, :17])           # lower edges e_0..e_{B-1}
+        for x in list(rng.uniform(0, 1, 400)) + list(e[dim, 1:17]) + [1.0, 0.5]:
+            k = bisect.bisect_left(row, x) - 1
+            assert O.grid_find_bin(e, dim, x) == max(k, 0)
+        # x on an interior edge lands in the bin to its left: bins are (e_k, e_{k+1}]
+        assert O.grid_find_bin(e, dim, e[dim, 5]) == 4
+        assert O.grid_find_bin(e, dim, 0.0) == 0        # reference: size_t(-1), clamped
+        assert O.grid_find_bin(e, dim, float("nan")) == 0
+
+
+def test_grid_forward_map():
+    rng = np.random.default_rng(2)
+    u = O.uniform_grid(4, 8)
+    r = rng.uniform(0, 1, 4)
+    x, jac = O.grid_forward(u, r)
+    assert np.allclose(x, r, rtol=0, atol=1e-15) and jac == 1.0
+    e = random_edges(rng, 4, 8)
+    r = rng.uniform(0, 1, 4)
+    x, jac = O.grid_forward(e, r)
+    for k in range(4):
+        assert abs(x[k] - np.interp(r[k], np.linspace(0, 1, 9), e[k])) < 1e-15
+    idx = np.floor(r * 8).astype(int)
+    assert math.isclose(jac, np.prod([(e[k, idx[k] + 1] - e[k, idx[k]]) * 8 for k in range(4)]), rel_tol=1e-14)
+
+
+def test_grid_adapt_invariants():
+    rng = np.random.default_rng(4)
+    B = 16
+    # uniform data: fixed point (up to rounding)
+    e = O.uniform_grid(2, B)
+    assert np.allclose(O.grid_adapt(e, np.ones((2, B))), e, atol=1e-14)
+    # an all-zero row is left with every edge at zero (AdaptiveMap.cc:69,93)
+    d = rng.uniform(0.1, 2.0, (2, B))
+    d[1] = 0
+    out = O.grid_adapt(e, d)
+    assert np.all(out[1] == 0) and out[0, 0] == 0 and out[0, -1] == 1
+    # equal-mass invariant: with smoothed, damped weights w_i the new edges
+    # split the piecewise-linear cumulative weight into B equal parts
+    e = random_edges(rng, 1, B)
+    d = rng.uniform(0.0, 3.0, (1, B))
+    out = O.grid_adapt(e, d)
+    t = d[0].copy()
+    sm = np.empty(B)
+    sm[0] = (t[0] + t[1]) / 2
+    sm[1:-1] = (t[:-2] + t[1:-1] + t[2:]) / 3
+    sm[-1] = (t[-2] + t[-1]) / 2
+    rr = sm / sm.sum()
+    w = (rr - 1) / np.log(rr)
+    cum = np.concatenate([[0], np.cumsum(w)])
+    target = np.interp(out[0], e[0], cum)
+    assert np.allclose(target, np.arange(B + 1) / B * cum[-1], rtol=0, atol=1e-9 * cum[-1])
+    assert np.all(np.diff(out[0]) > 0)
+
+
+def test_grid_split():
+    rng = np.random.default_rng(5)
+    e = random_edges(rng, 2, 5)
+    for ns in (2, 3, 4):
+        out = O.grid_split(e, ns)
+        assert out.shape == (2, 5 * ns + 1)
+        for k in range(2):
+            want = np.concatenate([e[k, b] + np.arange(ns) * (e[k, b + 1] - e[k, b]) / ns for b in range(5)] + [[1.0]])
+            assert np.allclose(out[k], want, rtol=0, atol=1e-16)
+
+
+def test_alpha_adapt_closed_form():
+    a = np.array([0.5, 0.25, 0.25])
+    t = np.array([16.0, 1.0, 0.0])
+    out = O.alpha_adapt(a, t, beta=0.25, min_alpha=1e-5)
+    # new = [0.5*2, 0.25*1, 0] -> normalised [0.8, 0.2, 0] -> zeros stay zero
+    assert np.allclose(out, [0.8, 0.2, 0.0], rtol=1e-15)
+    out = O.alpha_adapt([0.5, 0.5], [1.0, 1e-40], beta=0.25, min_alpha=0.01)
+    # [0.5, 0.5e-10] -> [1, 1e-10] -> clamp [1, 0.01] -> renormalise
+    assert np.allclose(out, [1 / 1.01, 0.01 / 1.01], rtol=1e-12)
+    assert O.max_difference([3.0, -1.0, 2.5]) == 4.0
+
+
+def test_stats_textbook():
+    x = np.random.default_rng(6).normal(2.0, 3.0, 1000)
+    x[::7] = 0.0
+    st, err = O.stats(x)
+    assert st.n == 1000 and st.n_finite == np.count_nonzero(x)
+    assert math.isclose(st.sum / st.n, x.mean(), rel_tol=1e-13)
+    # Error() is the error of the mean: (sum2/n - mean^2)/(n-1) = s^2/n (Statistics.hh:90,125)
+    assert math.isclose(err, np.sqrt(np.var(x, ddof=1) / len(x)), rel_tol=1e-10)
+    assert st.min == x.min() and st.max == x.max()
+
+
+# ---------------------------------------------------------------------------
+def test_massless_prop_jacobian():
+    smin, smax = 12.97, 1.0e6
+    for r in (1e-3, 0.1, 0.5, 0.9, 0.999):
+        s, w, rback = O.massless_prop(smin, smax, r)
+        assert smin <= s <= smax and abs(rback - r) < 1e-12
+        h = 1e-6
+        ds = (O.massless_prop(smin, smax, r + h)[0] - O.massless_prop(smin, smax, r - h)[0]) / (2 * h)
+        assert math.isclose(w, abs(ds), rel_tol=1e-6)
+
+
+def test_massive_prop_jacobian():
+    m, g, smin, smax = 80.4, 2.08, 100.0, 1.0e5
+    for r in (1e-3, 0.2, 0.5, 0.8, 0.999):
+        s, w, rback = O.massive_prop(m, g, smin, smax, r)
+        assert smin <= s <= smax and abs(rback - r) < 1e-12
+        h = 1e-7
+        ds = (O.massive_prop(m, g, smin, smax, r + h)[0] - O.massive_prop(m, g, smin, smax, r - h)[0]) / (2 * h)
+        assert math.isclose(w, abs(ds), rel_tol=1e-5)
+    # r = 0 gives smax: the reference's ymin/ymax names are swapped
+    assert math.isclose(O.massive_prop(m, g, smin, smax, 0.0)[0], smax, rel_tol=1e-12)
+
+
+def kallen(a, b, c):
+    return a * a + b * b + c * c - 2 * a * b - 2 * a * c - 2 * b * c
+
+
+def test_sqlam():
+    for s, s1, s2 in [(100.0, 1.0, 4.0), (1e4, 0.0, 0.0), (100.0, 20.0, 20.0)]:
+        assert math.isclose(O.sqlam(s, s1, s2), math.sqrt(kallen(s, s1, s2)) / s, rel_tol=1e-14)
+    assert O.sqlam(10.0, 9.0, 9.0) == 0.0
+
+
+@pytest.mark.parametrize("s1,s2", [(0.0, 0.0), (25.0, 0.0), (100.0, 400.0)])
+def test_schannel_decay(s1, s2):
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        m = 60.0 + 100 * rng.uniform()
+        pv = rng.normal(0, 80, 3)
+        p = np.array([math.sqrt(m * m + pv @ pv), *pv])
+        r1, r2 = rng.uniform(0, 1, 2)
+        p1, p2 = O.schannel_momenta(p, s1, s2, r1, r2)
+        mink = lambda v: v[0] ** 2 - v[1:] @ v[1:]
+        assert np.allclose(p1 + p2, p, rtol=0, atol=1e-12 * p[0])
+        assert abs(mink(p1) - s1) < 1e-9 * p[0] ** 2 and abs(mink(p2) - s2) < 1e-9 * p[0] ** 2
+        w, q1, q2 = O.schannel_weight(p1, p2)
+        assert abs(q1 - r1) < 1e-9 and abs(q2 - r2) < 1e-9
+        want = math.sqrt(kallen(m * m, s1, s2)) / (m * m) / (8 * math.pi)
+        assert math.isclose(w, want, rel_tol=1e-8)
+
+
+@pytest.mark.parametrize("n,smax", [(2, 1), (3, 2), (4, 3), (5, 4)])
+def test_mapper_round_trip(n, smax):
+    o = O.Oracle(P.gluon_process(n, smax=smax), 8, P.integrand_unit())
+    rng = np.random.default_rng(9 + n)
+    for c in range(o.nchannels):
+        done = 0
+        for _ in range(200):
+            x = rng.uniform(0.02, 0.98, o.ndims)
+            mom = o.generate_point(c, x)
+            if not np.all(np.isfinite(mom)):
+                continue
+            tot = mom.sum(axis=0)
+            E = np.abs(mom[:, 0]).max()
+            xin = -2 * mom[:2, 0] / 1000.0
+            if np.any(xin > 1):          # outside the beam: not a physical point
+                continue
+            assert np.allclose(tot, 0, atol=1e-10 * E)
+            m2 = mom[2:, 0] ** 2 - (mom[2:, 1:] ** 2).sum(axis=1)
+            assert np.all(np.abs(m2) < 1e-8 * E * E)
+            w, xb = o.generate_weight(c, mom)
+            assert w > 0
+            # WgtDecays takes sqrt(|m^2|) of massless children (Channel.cc:167-168):
+            # the rounding noise of m^2 ~ 1e-16 E^2 moves s_min by ~1e-8 relative
+            assert np.allclose(xb, x, rtol=0, atol=1e-6), (c, x, xb)
+            done += 1
+            if done >= 20:
+                break
+        assert done >= 5
+
+
+def test_channel_consistency_of_phase_space():
+    """Every channel's weight is a Jacobian of the same measure: the
+    integral of one cut-bounded test function through each channel alone
+    must agree within errors."""
+    proc = P.gluon_process(3, smax=2)
+    f = P.integrand_propprod()
+    o = O.Oracle(proc, 8, f)
+    rng = np.random.default_rng(12)
+    est = []
+    for c in range(o.nchannels):
+        vals = []
+        for _ in range(30000):
+            x = rng.uniform(0, 1, o.ndims)
+            mom = o.generate_point(c, x)
+            if not np.all(np.isfinite(mom)):
+                vals.append(0.0)
+                continue
+            fv, ok = O.integrand(f, mom, 1000.0)
+            if not ok:
+                vals.append(0.0)
+                continue
+            w, _ = o.generate_weight(c, mom)
+            vals.append(fv * w)
+        v = np.array(vals)
+        est.append((v.mean(), v.std() / math.sqrt(len(v))))
+    m = np.array([e[0] for e in est])
+    s = np.array([e[1] for e in est])
+    mean = np.sum(m / s ** 2) / np.sum(1 / s ** 2)
+    chi2 = np.sum((m - mean) ** 2 / s ** 2)
+    assert chi2 < 4 * (len(m) - 1) + 10, (m, s)
+
+
+# ---------------------------------------------------------------------------
+def test_integrand_definitions():
+    rng = np.random.default_rng(13)
+    mom = np.zeros((6, 4))
+    for i in range(2, 6):
+        v = rng.normal(0, 100, 3)
+        mom[i] = [np.linalg.norm(v), *v]
+    tot = mom[2:].sum(axis=0)
+    mom[0] = [-(tot[0] + tot[3]) / 2, 0, 0, -(tot[0] + tot[3]) / 2]
+    mom[1] = [-(tot[0] - tot[3]) / 2, 0, 0, (tot[0] - tot[3]) / 2]
+    mink = lambda v: v[0] ** 2 - v[1:] @ v[1:]
+    s = [mink(mom[i] + mom[j]) for i in range(2, 6) for j in range(i + 1, 6)]
+    want = sum(1e8 / (s[a] * s[b]) for a in range(6) for b in range(a + 1, 6))
+    f = P.integrand_propprod(s0=1e4, cuts=None)
+    v, ok = O.integrand(f, mom, 1e6)
+    assert ok and math.isclose(v, want, rel_tol=1e-12)
+    # resonant: Breit-Wigners of the four currents times x^T Q x
+    mom8 = np.zeros((8, 4))
+    for i in range(2, 8):
+        vv = rng.normal(0, 100, 3)
+        mom8[i] = [np.linalg.norm(vv), *vv]
+    q = P.resonant_q()
+    fr = P.integrand_resonant(q=q, cuts=None)
+    bw = 1.0
+    for mask, M, G in [(0b11000, P.MW, P.GW), (0b11100, P.MT, P.GT), (0b11000000, P.MW, P.GW), (0b11100000, P.MT, P.GT)]:
+        sv = mink(sum(mom8[i] for i in range(8) if mask >> i & 1))
+        bw *= (M * G) ** 2 / ((sv - M * M) ** 2 + (M * G) ** 2)
+    x = np.array([mink(mom8[i] + mom8[j]) for i in range(2, 8) for j in range(i + 1, 8)]) / 1e5
+    v, ok = O.integrand(fr, mom8, 1e6)
+    assert ok and math.isclose(v, bw * (x @ q @ x), rel_tol=1e-10)
+
+
+def test_cut_definition():
+    f = P.integrand_unit((20.0, 2.5, 0.4))
+
+    def event(pts, ys, phis):
+        mom = np.zeros((2 + len(pts), 4))
+        for i, (pt, y, ph) in enumerate(zip(pts, ys, phis)):
+            mom[2 + i] = [pt * math.cosh(y), pt * math.cos(ph), pt * math.sin(ph), pt * math.sinh(y)]
+        tot = mom[2:].sum(axis=0)
+        mom[0] = [-(tot[0] + tot[3]) / 2, 0, 0, -(tot[0] + tot[3]) / 2]
+        mom[1] = [-(tot[0] - tot[3]) / 2, 0, 0, (tot[0] - tot[3]) / 2]
+        return mom
+
+    assert O.integrand(f, event([30, 40], [0.1, -0.3], [0.5, 3.5]), 1000.0)[1]
+    assert not O.integrand(f, event([19.9, 40], [0.1, -0.3], [0.5, 3.5]), 1000.0)[1]
+    assert not O.integrand(f, event([30, 40], [2.6, -0.3], [0.5, 3.5]), 1000.0)[1]
+    # dR uses the wrapped azimuth difference: phi = 0.1 and 2 pi - 0.1 are 0.2 apart
+    assert not O.integrand(f, event([30, 40], [0.1, 0.1], [0.1, 2 * math.pi - 0.1]), 1000.0)[1]
+    assert O.integrand(f, event([30, 40], [0.1, 0.6], [0.1, 2 * math.pi - 0.1]), 1000.0)[1]
+    # x1 > 1 (beyond the beam energy) fails
+    assert not O.integrand(f, event([300, 300], [1.5, 1.4], [0.0, 3.1]), 1000.0)[1]
+
+
+def test_vegas_integral_vs_quadrature():
+    from scipy.integrate import quad
+    f = P.integrand_eemumu()
+    exact = quad(P.eemumu_sigma, P.EE_SMIN, P.EE_SMAX, points=[P.EE_MZ ** 2], limit=200, epsabs=0, epsrel=1e-10)[0]
+    edges = O.uniform_grid(2, 50)
+    uni = abi.philox_uniforms(0xC0FFEE00)
+    tot = np.zeros(3)
+    for it in range(6):
+        st, hist, _ = O.vegas_iterate(edges, f, uni, 10000, a0=it * 10000)
+        edges = O.grid_adapt(edges, hist)
+        tot += [st.n, st.sum, st.sum2]
+    mean = tot[1] / tot[0]
+    err = math.sqrt((tot[2] / tot[0] - mean ** 2) / (tot[0] - 1))
+    assert abs(mean - exact) < 4 * err, (mean, err, exact)
+    assert err / exact < 0.01
+
+
+def test_retry_loop_accounting():
+    proc, f = P.gluon_process(3, smax=2), P.integrand_propprod()
+    o = O.Oracle(proc, 16, f)
+    uni = abi.philox_uniforms(77, 2)
+    ncalls = 500
+    st, T, hist, na = o.iterate(uni, ncalls, a0=3)
+    rec = o.points(uni, 3, na)
+    acc = rec["status"] == 1
+    assert rec["status"][na - 1] == 1 and acc.sum() == ncalls     # stops at the ncalls-th accepted attempt
+    assert st.n == na and st.n_finite == np.count_nonzero(rec["val"][acc])
+    assert math.isclose(st.sum, rec["val"][acc].sum(), rel_tol=1e-12)
+    assert st.min == min(0.0, rec["val"][acc].min())               # rejected attempts add zeros
+    # alpha training and picked-channel histograms from the records
+    v2 = rec["val"] ** 2
+    Tw = (v2[acc, None] * rec["w"][acc, None] / rec["ps"][acc]).sum(axis=0)
+    assert np.allclose(T, Tw, rtol=1e-12)
+    H = np.zeros_like(hist)
+    for i in np.nonzero(acc)[0]:
+        for k in range(o.ndims):
+            H[rec["pick"][i], k, rec["bins"][i, k]] += v2[i]
+    assert np.allclose(hist, H, rtol=1e-12, atol=0)
