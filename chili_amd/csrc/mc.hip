@@ -332,7 +332,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // StatsData, alpha-training T_j += val^2 W / ps_j and the picked channel's
 // f^2 histogram (Integrand.hh:89-123, MultiChannel.hh:167-180, 224-247).
 // ---------------------------------------------------------------------------
-constexpr int kRounds = 4;                 // sub-tiles of kTile points per k_main workgroup
+constexpr int kRounds = 1;                 // sub-tiles of kTile points per k_main workgroup
 constexpr int kMainTile = kTile * kRounds;
 
 template <int NT>
@@ -515,8 +515,41 @@ __global__ __launch_bounds__(256) void k_reduce_groups(int C, const int* __restr
     }
 }
 
+// Histogram groups: per channel, runs of kRedGroupH consecutive partials.
+constexpr int kRedGroupH = 16;
+
+__global__ void k_hist_groups_table(int C, const int* __restrict__ tile2, int* __restrict__ grp2) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int g = 0;
+    for (int c = 0; c < C; ++c) {
+        grp2[c] = g;
+        g += (tile2[c + 1] - tile2[c] + kRedGroupH - 1) / kRedGroupH;
+    }
+    grp2[C] = g;
+}
+
+__global__ __launch_bounds__(256) void k_hist_groups(int C, int DB, const int* __restrict__ tile2, const int* __restrict__ grp2,
+                                                     const double* __restrict__ part, int64_t rec, double* __restrict__ hsum) {
+    const int g = blockIdx.x;
+    if (g >= grp2[C]) return;
+    int lo = 0, hi = C - 1;
+    while (lo < hi) {            // last c with grp2[c] <= g
+        int mid = (lo + hi + 1) >> 1;
+        if (grp2[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int c = lo;
+    const int t0 = tile2[c] + (g - grp2[c]) * kRedGroupH;
+    const int t1 = min(tile2[c + 1], t0 + kRedGroupH);
+    for (int k = blockIdx.y * 256 + threadIdx.x; k < DB && k < (blockIdx.y + 1) * 256; k += 256) {
+        double s = 0;
+        for (int t = t0; t < t1; ++t) s += part[(int64_t)t * rec + kStatW + C + k];
+        hsum[(int64_t)g * DB + k] = s;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_reduce_final(int C, int DB, const int* __restrict__ tile2, const double* __restrict__ part,
-                                                      int64_t rec, const double* __restrict__ gsum, double* __restrict__ tot, int train) {
+                                                      int64_t rec, const double* __restrict__ gsum, double* __restrict__ tot, int train,
+                                                      const int* __restrict__ grp2, const double* __restrict__ hsum) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int ntile = tile2[C];
     const int ng = (ntile + kRedGroup - 1) / kRedGroup;
@@ -539,7 +572,7 @@ __global__ __launch_bounds__(256) void k_reduce_final(int C, int DB, const int* 
     if (h >= (int64_t)C * DB) return;
     const int c = (int)(h / DB), k = (int)(h % DB);
     double s = tot[kStatW + C + h];
-    for (int t = tile2[c]; t < tile2[c + 1]; ++t) s += part[(int64_t)t * rec + kStatW + C + k];
+    for (int g = grp2[c]; g < grp2[c + 1]; ++g) s += hsum[(int64_t)g * DB + k];
     tot[kStatW + C + h] = s;
 }
 
@@ -559,12 +592,12 @@ struct chili_mc {
     bool have_grids = false;
     // device buffers
     DevBuf<ChanProg> d_progs;
-    DevBuf<double> d_edges, d_search, d_alpha, d_cum, d_sext, d_tot, d_part, d_mom, d_ps, d_u, d_gsum;
+    DevBuf<double> d_edges, d_search, d_alpha, d_cum, d_sext, d_tot, d_part, d_mom, d_ps, d_u, d_gsum, d_hsum;
     DevBuf<chili_integrand> d_f;
     DevBuf<uint16_t> d_pick;
     DevBuf<uint32_t> d_sorted, d_comp;
     DevBuf<uint8_t> d_acc;
-    DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff;
+    DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff, d_grp2;
     DevBuf<int64_t> d_total, d_bsum;
     int64_t nmax = 0;            // attempts per sub-batch
     double acc_rate = 0.25;      // running estimate of the cut acceptance
@@ -811,8 +844,16 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
         m->d_gsum.resize((size_t)ngroups * (kStatW + C));
         hipLaunchKernelGGL(k_reduce_groups, dim3(ngroups), dim3(256), 0, m->stream, C, m->d_tiles2.ptr, m->d_part.ptr, rec,
                            m->d_gsum.ptr, train);
+        if (train) {
+            const int DB = m->d * m->B;
+            const int hgroups = mgrid / kRedGroupH + C + 1;
+            m->d_hsum.resize((size_t)hgroups * DB);
+            hipLaunchKernelGGL(k_hist_groups_table, dim3(1), dim3(1), 0, m->stream, C, m->d_tiles2.ptr, m->d_grp2.ptr);
+            hipLaunchKernelGGL(k_hist_groups, dim3(hgroups, (DB + 255) / 256), dim3(256), 0, m->stream, C, DB, m->d_tiles2.ptr,
+                               m->d_grp2.ptr, m->d_part.ptr, rec, m->d_hsum.ptr);
+        }
         hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, m->stream, C, m->d * m->B,
-                           m->d_tiles2.ptr, m->d_part.ptr, rec, m->d_gsum.ptr, m->d_tot.ptr, train);
+                           m->d_tiles2.ptr, m->d_part.ptr, rec, m->d_gsum.ptr, m->d_tot.ptr, train, m->d_grp2.ptr, m->d_hsum.ptr);
     }
 }
 
@@ -884,6 +925,7 @@ int chili_mc_create(const chili_channels* ch, int nbins, const chili_integrand* 
         m->d_tiles.resize(m->C + 1);
         m->d_seg2.resize(m->C + 1);
         m->d_tiles2.resize(m->C + 1);
+        m->d_grp2.resize(m->C + 1);
         m->d_total.resize(1);
         m->d_tot.resize(kStatW + m->C + (size_t)m->C * m->d * m->B);
         // uniform grids and alpha = 1/C (AdaptiveMap.hh:29-40, MultiChannel.cc:62-67)
