@@ -13,7 +13,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchili_gpu.so")
 ARCH = "gfx950"
 
-SOURCES = ["mc.hip", "vegas.hip", "channels.cpp", "adapt.cpp", "common.cpp"]
+SOURCES = ["mc.hip", "vegas.hip", "flow.hip", "channels.cpp", "adapt.cpp", "common.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,0 +1,684 @@
+// flow.hip -- learned proposal on gfx950: per-channel coupling-layer flows with
+// rational-quadratic splines over the VEGAS input hypercube.  MLPs run on
+// hand-written BF16 MFMA GEMMs (FP32 accumulation); spline transforms,
+// log-Jacobians and the training loss run in FP64.
+//
+// Density direction (r -> z): layer l keeps x[A_l], maps x[B_l] through a
+// spline whose parameters an MLP computes from x[A_l]; log q(r) = sum of the
+// spline log-derivatives.  Sampling direction (z -> r) inverts the layers in
+// reverse order.  Training minimises the weighted-variance loss of the
+// multichannel weight over samples drawn by the method itself:
+//   L = mean_i w_i^2 / sum_j beta_ij q_j(r_ij),
+// w_i the point's weight under the sampler that drew it, beta_ij = alpha_j
+// g_j(p_i) / g(p_i) that sampler's channel shares and r_ij the point's
+// coordinates in channel j's hypercube.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "common.hpp"
+#include "flow.hpp"
+#include "flow_kernels.hpp"
+
+using namespace chili_nf;
+
+namespace {
+
+constexpr int kSplineBins = 16;
+constexpr int kTh = 256;
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+// xa[i][c] = bf16(x[i][A[c]]) (padded columns zero)
+__global__ void k_gather_bf16(int64_t n, int d, const double* __restrict__ x, const int* __restrict__ A, int nA, int ld,
+                              bf16* __restrict__ xa) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < ld; ++c) xa[i * ld + c] = f2bf(c < nA ? (float)x[i * d + A[c]] : 0.f);
+}
+
+// Forward (density direction) spline over the B dims of one layer.
+template <int K>
+__global__ void k_spline_fwd(int64_t n, int d, const double* __restrict__ x, const float* __restrict__ raw, int ldr,
+                             const int* __restrict__ Bd, int nB, float shift, double* __restrict__ y, double* __restrict__ lq) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < d; ++c) y[i * d + c] = x[i * d + c];
+    double ld = 0;
+    for (int b = 0; b < nB; ++b) {
+        double l;
+        const int c = Bd[b];
+        y[i * d + c] = rqs_forward<K>(raw + i * ldr + b * (3 * K - 1), shift, x[i * d + c], l);
+        ld += l;
+    }
+    lq[i] += ld;
+}
+
+// Sampling-direction spline: x[B] = inverse(y[B]); lq accumulates log dy/dx.
+template <int K>
+__global__ void k_spline_inv(int64_t n, int d, double* __restrict__ xy, const float* __restrict__ raw, int ldr,
+                             const int* __restrict__ Bd, int nB, float shift, double* __restrict__ lq) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ld = 0;
+    for (int b = 0; b < nB; ++b) {
+        double l;
+        const int c = Bd[b];
+        xy[i * d + c] = rqs_inverse<K>(raw + i * ldr + b * (3 * K - 1), shift, xy[i * d + c], l);
+        ld += l;
+    }
+    lq[i] += ld;
+}
+
+// Spline backward: gx holds dL/dy on entry and dL/dx (B dims) on exit; draw
+// receives dL/draw (bf16 for the next GEMM and fp32 for the bias sums).
+template <int K>
+__global__ void k_spline_bwd(int64_t n, int d, const double* __restrict__ x, const float* __restrict__ raw, int ldr,
+                             const int* __restrict__ Bd, int nB, float shift, const double* __restrict__ glq,
+                             double* __restrict__ gx, float* __restrict__ draw, bf16* __restrict__ drawb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int b = 0; b < nB; ++b) {
+        const int c = Bd[b];
+        float* du = draw + i * ldr + b * (3 * K - 1);
+        gx[i * d + c] = rqs_backward<K>(raw + i * ldr + b * (3 * K - 1), shift, x[i * d + c], gx[i * d + c], glq[i], du);
+        for (int q = 0; q < 3 * K - 1; ++q) drawb[i * ldr + b * (3 * K - 1) + q] = f2bf(du[q]);
+    }
+}
+
+// gh (bf16) = relu'(h) * g (fp32)
+__global__ void k_relu_mask(int64_t n, const float* g, const bf16* __restrict__ h, bf16* __restrict__ out, float* outf) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = bf2f(h[i]) > 0.f ? g[i] : 0.f;
+    out[i] = f2bf(v);
+    outf[i] = v;
+}
+
+// gx[i][A[c]] += dxa[i][c]
+__global__ void k_scatter_add(int64_t n, int d, const float* __restrict__ dxa, int ld, const int* __restrict__ A, int nA,
+                              double* __restrict__ gx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < nA; ++c) gx[i * d + A[c]] += (double)dxa[i * ld + c];
+}
+
+// Column sums of g [n x m] in fixed order: partials over row chunks.
+__global__ void k_colsum_part(int64_t n, int m, const float* __restrict__ g, int64_t chunk, float* __restrict__ part) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    const int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = std::min<int64_t>(n, r0 + chunk);
+    float s = 0;
+    for (int64_t r = r0; r < r1; ++r) s += g[r * m + c];
+    part[(int64_t)blockIdx.y * m + c] = s;
+}
+
+// dst[k] += sum_z part[z][k], z in order
+__global__ void k_sum_parts(int64_t cnt, int nz, const float* __restrict__ part, float* __restrict__ dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    float s = 0;
+    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * cnt + k];
+    dst[k] += s;
+}
+
+// Weighted-variance loss over C flows; loss partial per block; dlq[j][i].
+__global__ void k_loss(int64_t n, int C, const double* __restrict__ lq, const double* __restrict__ beta,
+                       const double* __restrict__ w2, double inv_n, double* __restrict__ dlq, double* __restrict__ lpart) {
+    __shared__ double s[kTh];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double li = 0;
+    if (i < n) {
+        double ratio = 0;
+        for (int j = 0; j < C; ++j) ratio += beta[(int64_t)j * n + i] * exp(lq[(int64_t)j * n + i]);
+        li = w2[i] / ratio;
+        for (int j = 0; j < C; ++j)
+            dlq[(int64_t)j * n + i] = -inv_n * w2[i] * beta[(int64_t)j * n + i] * exp(lq[(int64_t)j * n + i]) / (ratio * ratio);
+    }
+    s[threadIdx.x] = li;
+    __syncthreads();
+    for (int off = kTh / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = s[0];
+}
+
+__global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1 - b1) * gi;
+    const float vi = b2 * v[i] + (1 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copies of a weight W [in x out]: Wb (same layout) and Wt [out x in]
+__global__ void k_weights_bf16(int in, int out, const float* __restrict__ W, bf16* __restrict__ Wb, bf16* __restrict__ Wt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= in * out) return;
+    const int r = i / out, c = i % out;
+    Wb[i] = f2bf(W[i]);
+    Wt[c * in + r] = f2bf(W[i]);
+}
+
+__global__ void k_fill_f64(int64_t n, double v, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+__global__ void k_fill_f32(int64_t n, float v, float* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+// out[i] = in[perm[i]] rows of width w (fp64)
+__global__ void k_gather_rows(int64_t n, int w, const double* __restrict__ in, const int64_t* __restrict__ perm, int64_t base,
+                              double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t src = perm[base + i];
+    for (int c = 0; c < w; ++c) out[i * w + c] = in[src * w + c];
+}
+
+inline unsigned nblk(int64_t n, int t = kTh) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+// ===========================================================================
+// host engine
+// ===========================================================================
+namespace chili_nf {
+
+void FlowEngine::build_layers(uint64_t seed) {
+    // Coupling masks: pairs of layers transform a seeded random half of the
+    // dimensions and then its complement, so every dimension is transformed
+    // (and conditions the others) in every pair of layers.
+    std::mt19937_64 rng(seed);
+    layers.clear();
+    std::vector<int> half;
+    for (int l = 0; l < L; ++l) {
+        std::vector<int> B;
+        if (l % 2 == 0) {
+            std::vector<int> perm(d);
+            for (int i = 0; i < d; ++i) perm[i] = i;
+            std::shuffle(perm.begin(), perm.end(), rng);
+            half.assign(perm.begin(), perm.begin() + d / 2);
+            std::sort(half.begin(), half.end());
+            B = half;
+        } else {
+            for (int i = 0; i < d; ++i)
+                if (!std::binary_search(half.begin(), half.end(), i)) B.push_back(i);
+        }
+        Layer ly;
+        ly.B = B;
+        for (int i = 0; i < d; ++i)
+            if (!std::binary_search(B.begin(), B.end(), i)) ly.A.push_back(i);
+        ly.nA = (int)ly.A.size();
+        ly.nB = (int)ly.B.size();
+        ly.out = ly.nB * (3 * kSplineBins - 1);
+        ly.ldA = ((ly.nA + 7) / 8) * 8;
+        layers.push_back(std::move(ly));
+    }
+    nparams = 0;
+    for (auto& ly : layers) {
+        ly.oW1 = nparams; nparams += (size_t)ly.nA * H;
+        ly.ob1 = nparams; nparams += H;
+        ly.oW2 = nparams; nparams += (size_t)H * H;
+        ly.ob2 = nparams; nparams += H;
+        ly.oW3 = nparams; nparams += (size_t)H * H;
+        ly.ob3 = nparams; nparams += H;
+        ly.oW4 = nparams; nparams += (size_t)H * ly.out;
+        ly.ob4 = nparams; nparams += ly.out;
+    }
+    for (auto& ly : layers) {
+        std::vector<int> a = ly.A, b = ly.B;
+        ly.dA.upload(a, stream);
+        ly.dB.upload(b, stream);
+    }
+}
+
+void FlowEngine::init_params(uint64_t seed) {
+    std::mt19937_64 rng(seed ^ 0x9E3779B97F4A7C15ull);
+    for (int f = 0; f < nflows; ++f) {
+        std::vector<float> p(nparams, 0.f);
+        for (auto& ly : layers) {
+            auto fill = [&](size_t off, int in, int out) {
+                const double bnd = 1.0 / std::sqrt((double)in);
+                std::uniform_real_distribution<double> u(-bnd, bnd);
+                for (size_t k = 0; k < (size_t)in * out; ++k) p[off + k] = (float)u(rng);
+            };
+            fill(ly.oW1, ly.nA, H);
+            fill(ly.oW2, H, H);
+            fill(ly.oW3, H, H);
+            // last layer zero: every spline starts as the identity
+        }
+        flows[f].params.upload(p, stream);
+        std::vector<float> z(nparams, 0.f);
+        flows[f].grads.upload(z, stream);
+        flows[f].m.upload(z, stream);
+        flows[f].v.upload(z, stream);
+        flows[f].steps = 0;
+        refresh_bf16(f);
+    }
+}
+
+void FlowEngine::refresh_bf16(int f) {
+    Flow& F = flows[f];
+    size_t wt = 0;
+    for (auto& ly : layers) wt += (size_t)ly.nA * H + 2 * (size_t)H * H + (size_t)H * ly.out;
+    F.wb.resize(wt);
+    F.wt.resize(wt);
+    size_t o = 0;
+    for (auto& ly : layers) {
+        const int ins[4] = {ly.nA, H, H, H}, outs[4] = {H, H, H, ly.out};
+        const size_t offs[4] = {ly.oW1, ly.oW2, ly.oW3, ly.oW4};
+        for (int q = 0; q < 4; ++q) {
+            ly.wo[q] = o;
+            hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((int64_t)ins[q] * outs[q])), dim3(kTh), 0, stream, ins[q], outs[q],
+                               F.params.ptr + offs[q], F.wb.ptr + o, F.wt.ptr + o);
+            o += (size_t)ins[q] * outs[q];
+        }
+    }
+}
+
+void FlowEngine::ensure_ws(int64_t nb, bool keep) {
+    // keep: activations of every layer stay for the backward pass; otherwise
+    // the layers share one slot (and two for x)
+    const size_t Ls = keep ? (size_t)L : 1;
+    const size_t Lx = keep ? (size_t)L + 1 : 2;
+    const int maxout = ws_out(), maxA = ws_lda();
+    ws.x.resize(Lx * nb * d);
+    ws.xa.resize(Ls * nb * maxA);
+    ws.h.resize(Ls * 3 * nb * H);
+    ws.raw.resize(Ls * nb * maxout);
+    if (keep) {
+        ws.draw.resize((size_t)nb * maxout);
+        ws.drawb.resize((size_t)nb * maxout);
+        ws.gf.resize((size_t)nb * std::max(H, maxout));
+        ws.gb.resize((size_t)nb * std::max(H, maxout));
+        ws.gx.resize((size_t)nb * d);
+        ws.dxa.resize((size_t)nb * maxA);
+    }
+}
+
+// GEMM launch helpers ------------------------------------------------------
+void FlowEngine::gemm_fwd(int64_t M, int N, int K, const bf16* A, int lda, const bf16* Bt, int ldb, const float* bias, int relu,
+                          float* C, int ldc, bf16* Cb, int ldcb) {
+    dim3 g((N + kGT - 1) / kGT, (unsigned)((M + kGT - 1) / kGT), 1);
+    hipLaunchKernelGGL((k_gemm<false, true>), g, dim3(kGThreads), 0, stream, (int)M, N, K, A, lda, Bt, ldb, C, ldc, bias, relu, Cb,
+                       ldcb, K, (int64_t)0);
+}
+// dW [in x out] += A^T G with A [n x in] (bf16), G [n x out] (bf16), split over n
+void FlowEngine::gemm_wgrad(int64_t n, int in, int out, const bf16* A, int lda, const bf16* G, int ldg, float* dW) {
+    const int chunk = 4096;
+    const int nz = (int)((n + chunk - 1) / chunk);
+    part.resize((size_t)nz * in * out);
+    dim3 g((out + kGT - 1) / kGT, (in + kGT - 1) / kGT, nz);
+    hipLaunchKernelGGL((k_gemm<true, false>), g, dim3(kGThreads), 0, stream, in, out, (int)n, A, lda, G, ldg, part.ptr, out,
+                       (const float*)nullptr, 0, (bf16*)nullptr, 0, chunk, (int64_t)in * out);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblk((int64_t)in * out)), dim3(kTh), 0, stream, (int64_t)in * out, nz, part.ptr, dW);
+}
+void FlowEngine::colsum(int64_t n, int m, const float* g, float* db) {
+    const int64_t chunk = 2048;
+    const int nz = (int)((n + chunk - 1) / chunk);
+    part2.resize((size_t)nz * m);
+    hipLaunchKernelGGL(k_colsum_part, dim3((m + 63) / 64, nz), dim3(64), 0, stream, n, m, g, chunk, part2.ptr);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(m)), dim3(kTh), 0, stream, (int64_t)m, nz, part2.ptr, db);
+}
+
+// Density direction over nb points: x0 = r (fp64 [nb x d]) -> z = x_L; lq.
+// keep = true stores every layer's activations for backward().
+void FlowEngine::density(int f, int64_t nb, const double* r, double* lq, double* zout, bool keep) {
+    ensure_ws(nb, keep);
+    Flow& F = flows[f];
+    HIP_CHECK(hipMemcpyAsync(ws.x.ptr, r, (size_t)nb * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, 0.0, lq);
+    const int lda = ws_lda(), ldo = ws_out();
+    for (int l = 0; l < L; ++l) {
+        Layer& ly = layers[l];
+        const size_t sl = keep ? (size_t)l : 0;
+        double* xl = ws.x.ptr + (keep ? (size_t)l : (size_t)(l & 1)) * nb * d;
+        double* xn = ws.x.ptr + (keep ? (size_t)l + 1 : (size_t)((l + 1) & 1)) * nb * d;
+        bf16* xa = ws.xa.ptr + sl * nb * lda;
+        bf16* h = ws.h.ptr + sl * 3 * nb * H;
+        float* raw = ws.raw.ptr + sl * nb * ldo;
+        mlp_forward(F, ly, nb, xl, xa, h, raw);
+        hipLaunchKernelGGL(k_spline_fwd<kSplineBins>, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, d, xl, raw, ly.out, ly.dB.ptr, ly.nB,
+                           shift, xn, lq);
+    }
+    const double* zl = ws.x.ptr + (keep ? (size_t)L : (size_t)(L & 1)) * nb * d;
+    if (zout) HIP_CHECK(hipMemcpyAsync(zout, zl, (size_t)nb * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
+}
+
+// Conditioner of one coupling layer: xa = bf16(x[A]); three ReLU layers; raw
+// spline parameters (fp32).
+void FlowEngine::mlp_forward(Flow& F, Layer& ly, int64_t nb, const double* x, bf16* xa, bf16* h, float* raw) {
+    hipLaunchKernelGGL(k_gather_bf16, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, d, x, ly.dA.ptr, ly.nA, ly.ldA, xa);
+    gemm_fwd(nb, H, ly.nA, xa, ly.ldA, F.wt.ptr + ly.wo[0], ly.nA, F.params.ptr + ly.ob1, 1, nullptr, 0, h, H);
+    gemm_fwd(nb, H, H, h, H, F.wt.ptr + ly.wo[1], H, F.params.ptr + ly.ob2, 1, nullptr, 0, h + (size_t)nb * H, H);
+    gemm_fwd(nb, H, H, h + (size_t)nb * H, H, F.wt.ptr + ly.wo[2], H, F.params.ptr + ly.ob3, 1, nullptr, 0, h + 2 * (size_t)nb * H, H);
+    gemm_fwd(nb, ly.out, H, h + 2 * (size_t)nb * H, H, F.wt.ptr + ly.wo[3], H, F.params.ptr + ly.ob4, 0, raw, ly.out, nullptr, 0);
+}
+
+// Backward of density(): dlq (fp64 [nb]) -> gradients added to flow f's grads.
+void FlowEngine::backward(int f, int64_t nb, const double* dlq) {
+    Flow& F = flows[f];
+    HIP_CHECK(hipMemsetAsync(ws.gx.ptr, 0, (size_t)nb * d * sizeof(double), stream));
+    for (int l = L - 1; l >= 0; --l) {
+        Layer& ly = layers[l];
+        double* xl = ws.x.ptr + (size_t)l * nb * d;
+        bf16* xa = ws.xa.ptr + (size_t)l * nb * ws_lda();
+        bf16* h = ws.h.ptr + (size_t)l * 3 * nb * H;
+        float* raw = ws.raw.ptr + (size_t)l * nb * ws_out();
+        hipLaunchKernelGGL(k_spline_bwd<kSplineBins>, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, d, xl, raw, ly.out, ly.dB.ptr, ly.nB,
+                           shift, dlq, ws.gx.ptr, ws.draw.ptr, ws.drawb.ptr);
+        // last linear layer
+        gemm_wgrad(nb, H, ly.out, h + 2 * (size_t)nb * H, H, ws.drawb.ptr, ly.out, F.grads.ptr + ly.oW4);
+        colsum(nb, ly.out, ws.draw.ptr, F.grads.ptr + ly.ob4);
+        const bf16* gin = ws.drawb.ptr;
+        int gin_ld = ly.out, gin_w = ly.out;
+        const size_t wq[3] = {ly.wo[3], ly.wo[2], ly.wo[1]};
+        const size_t ow[3] = {ly.oW3, ly.oW2, ly.oW1};
+        const size_t ob[3] = {ly.ob3, ly.ob2, ly.ob1};
+        for (int q = 0; q < 3; ++q) {
+            // g_h = (gin W^T) * relu'(h), h = hidden activation 2 - q
+            const bf16* hq = h + (size_t)(2 - q) * nb * H;
+            dim3 g((H + kGT - 1) / kGT, (unsigned)((nb + kGT - 1) / kGT), 1);
+            hipLaunchKernelGGL((k_gemm<false, true>), g, dim3(kGThreads), 0, stream, (int)nb, H, gin_w, gin, gin_ld,
+                               F.wb.ptr + wq[q], gin_w, ws.gf.ptr, H, (const float*)nullptr, 0, (bf16*)nullptr, 0, gin_w, (int64_t)0);
+            hipLaunchKernelGGL(k_relu_mask, dim3(nblk(nb * H)), dim3(kTh), 0, stream, nb * H, ws.gf.ptr, hq, ws.gb.ptr, ws.gf.ptr);
+            // weight / bias grads of the layer feeding h_{2-q}
+            const bf16* ain = (q < 2) ? h + (size_t)(1 - q) * nb * H : xa;
+            const int ain_w = (q < 2) ? H : ly.nA, ain_ld = (q < 2) ? H : ly.ldA;
+            gemm_wgrad(nb, ain_w, H, ain, ain_ld, ws.gb.ptr, H, F.grads.ptr + ow[q]);
+            colsum(nb, H, ws.gf.ptr, F.grads.ptr + ob[q]);
+            gin = ws.gb.ptr;
+            gin_ld = H;
+            gin_w = H;
+        }
+        // input gradient of the first linear layer: dxa = g_h1 W1^T
+        {
+            dim3 g((ly.nA + kGT - 1) / kGT, (unsigned)((nb + kGT - 1) / kGT), 1);
+            hipLaunchKernelGGL((k_gemm<false, true>), g, dim3(kGThreads), 0, stream, (int)nb, ly.nA, H, ws.gb.ptr, H, F.wb.ptr + ly.wo[0],
+                               H, ws.dxa.ptr, ly.ldA, (const float*)nullptr, 0, (bf16*)nullptr, 0, H, (int64_t)0);
+            hipLaunchKernelGGL(k_scatter_add, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, d, ws.dxa.ptr, ly.ldA, ly.dA.ptr, ly.nA, ws.gx.ptr);
+        }
+    }
+}
+
+// Sampling direction: y (fp64 [nb x d], in place) from z to r; lq = log q(r).
+void FlowEngine::sample(int f, int64_t nb, double* xy, double* lq) {
+    ensure_ws(nb, false);
+    Flow& F = flows[f];
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, 0.0, lq);
+    for (int l = L - 1; l >= 0; --l) {
+        Layer& ly = layers[l];
+        mlp_forward(F, ly, nb, xy, ws.xa.ptr, ws.h.ptr, ws.raw.ptr);
+        hipLaunchKernelGGL(k_spline_inv<kSplineBins>, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, d, xy, ws.raw.ptr, ly.out, ly.dB.ptr, ly.nB,
+                           shift, lq);
+    }
+}
+
+void FlowEngine::density_chunked(int f, int64_t n, const double* r, double* lq) {
+    for (int64_t o = 0; o < n; o += kInferChunk) density(f, std::min(kInferChunk, n - o), r + o * d, lq + o, nullptr, false);
+}
+void FlowEngine::sample_chunked(int f, int64_t n, double* zr, double* lq) {
+    for (int64_t o = 0; o < n; o += kInferChunk) sample(f, std::min(kInferChunk, n - o), zr + o * d, lq + o);
+}
+
+double FlowEngine::loss_grad(int64_t nb, const double* r, const double* beta, const double* w2) {
+    // r: [C][nb][d], beta: [C][nb], w2: [nb]
+    lqb.resize((size_t)nflows * nb);
+    dlqb.resize((size_t)nflows * nb);
+    const unsigned nbk = nblk(nb);
+    lpart.resize(nbk);
+    for (int f = 0; f < nflows; ++f) {
+        HIP_CHECK(hipMemsetAsync(flows[f].grads.ptr, 0, nparams * sizeof(float), stream));
+    }
+    // density passes keep only one workspace, so each flow runs forward then
+    // backward before the next; the loss needs every flow's log-density first
+    for (int f = 0; f < nflows; ++f) density(f, nb, r + (size_t)f * nb * d, lqb.ptr + (size_t)f * nb, nullptr, false);
+    hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, dlqb.ptr, lpart.ptr);
+    std::vector<double> lp(nbk);
+    lpart.download(lp.data(), nbk, stream);
+    for (int f = 0; f < nflows; ++f) {
+        density(f, nb, r + (size_t)f * nb * d, lqb.ptr + (size_t)f * nb, nullptr, true);
+        backward(f, nb, dlqb.ptr + (size_t)f * nb);
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
+    double s = 0;
+    for (double v : lp) s += v;
+    return s / (double)nb;
+}
+
+void FlowEngine::adam_step(double lr) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int f = 0; f < nflows; ++f) {
+        Flow& F = flows[f];
+        F.steps += 1;
+        const float c1 = 1.f - std::pow(b1, (float)F.steps), c2 = 1.f - std::pow(b2, (float)F.steps);
+        hipLaunchKernelGGL(k_adam, dim3(nblk((int64_t)nparams)), dim3(kTh), 0, stream, (int64_t)nparams, F.params.ptr, F.grads.ptr, F.m.ptr,
+                           F.v.ptr, (float)lr, b1, b2, eps, c1, c2);
+        refresh_bf16(f);
+    }
+}
+
+}  // namespace chili_nf
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int chili_flow_create(int ndims, int nlayers, int hidden, int nflows, uint64_t seed, int device, chili_flow** out) {
+    return guarded([&] {
+        if (!out) throw chili_error(CHILI_EINVAL, "null argument");
+        if (ndims < 2 || ndims > CHILI_MAX_DIMS || nlayers < 1 || nlayers > 32 || hidden != 64 || nflows < 1 || nflows > 256)
+            throw chili_error(CHILI_ERANGE, "flow sizes out of range (hidden width must be 64)");
+        select_device(device);
+        auto F = std::make_unique<chili_flow>();
+        auto& E = F->eng;
+        E.device = device;
+        HIP_CHECK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+        E.d = ndims;
+        E.L = nlayers;
+        E.H = hidden;
+        E.nflows = nflows;
+        E.shift = (float)std::log(std::expm1(1.0 - kMinD));
+        E.build_layers(seed);
+        E.flows.resize(nflows);
+        E.init_params(seed);
+        HIP_CHECK(hipStreamSynchronize(E.stream));
+        *out = F.release();
+    });
+}
+
+int chili_flow_destroy(chili_flow* f) {
+    return guarded([&] {
+        if (!f) return;
+        select_device(f->eng.device);
+        hipStream_t s = f->eng.stream;
+        if (s) hipStreamSynchronize(s);
+        delete f;
+        if (s) hipStreamDestroy(s);
+    });
+}
+
+int chili_flow_info(const chili_flow* f, int64_t* nparams, int* ndims, int* nlayers, int* nflows) {
+    return guarded([&] {
+        if (!f) throw chili_error(CHILI_EINVAL, "null flow");
+        if (nparams) *nparams = (int64_t)f->eng.nparams;
+        if (ndims) *ndims = f->eng.d;
+        if (nlayers) *nlayers = f->eng.L;
+        if (nflows) *nflows = f->eng.nflows;
+    });
+}
+
+// Coupling masks: transformed dimensions of layer l (nB entries).
+int chili_flow_mask(const chili_flow* f, int layer, int32_t* B, int* nB) {
+    return guarded([&] {
+        if (!f || layer < 0 || layer >= f->eng.L) throw chili_error(CHILI_EINVAL, "bad layer");
+        const auto& ly = f->eng.layers[layer];
+        if (nB) *nB = ly.nB;
+        if (B) for (int i = 0; i < ly.nB; ++i) B[i] = ly.B[i];
+    });
+}
+
+int chili_flow_get_params(chili_flow* f, int flow, float* out) {
+    return guarded([&] {
+        if (!f || !out || flow < 0 || flow >= f->eng.nflows) throw chili_error(CHILI_EINVAL, "bad argument");
+        select_device(f->eng.device);
+        f->eng.flows[flow].params.download(out, f->eng.nparams, f->eng.stream);
+        HIP_CHECK(hipStreamSynchronize(f->eng.stream));
+    });
+}
+
+int chili_flow_set_params(chili_flow* f, int flow, const float* in) {
+    return guarded([&] {
+        if (!f || !in || flow < 0 || flow >= f->eng.nflows) throw chili_error(CHILI_EINVAL, "bad argument");
+        select_device(f->eng.device);
+        std::vector<float> p(in, in + f->eng.nparams);
+        f->eng.flows[flow].params.upload(p, f->eng.stream);
+        f->eng.refresh_bf16(flow);
+        HIP_CHECK(hipStreamSynchronize(f->eng.stream));
+    });
+}
+
+int chili_flow_get_grads(chili_flow* f, int flow, float* out) {
+    return guarded([&] {
+        if (!f || !out || flow < 0 || flow >= f->eng.nflows) throw chili_error(CHILI_EINVAL, "bad argument");
+        select_device(f->eng.device);
+        f->eng.flows[flow].grads.download(out, f->eng.nparams, f->eng.stream);
+        HIP_CHECK(hipStreamSynchronize(f->eng.stream));
+    });
+}
+
+// log q(r) (and z) for n points of flow `flow`; device pointers, r/z [n][d].
+int chili_flow_log_density(chili_flow* f, int flow, int64_t n, const double* r_dev, double* lq_dev, double* z_dev) {
+    return guarded([&] {
+        if (!f || !r_dev || !lq_dev || flow < 0 || flow >= f->eng.nflows) throw chili_error(CHILI_EINVAL, "bad argument");
+        if (n <= 0 || n > (int64_t(1) << 24)) throw chili_error(CHILI_ERANGE, "n out of range");
+        select_device(f->eng.device);
+        const int64_t chunk = kInferChunk;
+        const int d = f->eng.d;
+        for (int64_t o = 0; o < n; o += chunk) {
+            const int64_t m = std::min(chunk, n - o);
+            f->eng.density(flow, m, r_dev + o * d, lq_dev + o, z_dev ? z_dev + o * d : nullptr, false);
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(f->eng.stream));
+    });
+}
+
+// z -> r in place for n points (device [n][d]); lq = log q(r).
+int chili_flow_sample(chili_flow* f, int flow, int64_t n, double* zr_dev, double* lq_dev) {
+    return guarded([&] {
+        if (!f || !zr_dev || !lq_dev || flow < 0 || flow >= f->eng.nflows) throw chili_error(CHILI_EINVAL, "bad argument");
+        if (n <= 0 || n > (int64_t(1) << 24)) throw chili_error(CHILI_ERANGE, "n out of range");
+        select_device(f->eng.device);
+        f->eng.sample_chunked(flow, n, zr_dev, lq_dev);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(f->eng.stream));
+    });
+}
+
+// Weighted-variance loss and parameter gradients (all flows) on one batch
+// (device: r [C][n][d], beta [C][n], w2 [n]).  Gradients stay on the device
+// (chili_flow_get_grads); nothing is updated.
+int chili_flow_loss_grad(chili_flow* f, int64_t n, const double* r_dev, const double* beta_dev, const double* w2_dev, double* loss) {
+    return guarded([&] {
+        if (!f || !r_dev || !beta_dev || !w2_dev || !loss) throw chili_error(CHILI_EINVAL, "null argument");
+        if (n <= 0 || n > (int64_t(1) << 20)) throw chili_error(CHILI_ERANGE, "batch out of range");
+        select_device(f->eng.device);
+        *loss = f->eng.loss_grad(n, r_dev, beta_dev, w2_dev);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+// Training: epochs over n points in batches (Adam); loss_hist (host, optional)
+// receives the mean batch loss of every epoch.
+int chili_flow_train(chili_flow* f, int64_t n, const double* r_dev, const double* beta_dev, const double* w2_dev, int64_t batch,
+                     int epochs, double lr, uint64_t seed, double* loss_hist) {
+    return guarded([&] {
+        if (!f || !r_dev || !beta_dev || !w2_dev) throw chili_error(CHILI_EINVAL, "null argument");
+        if (n <= 0 || batch <= 0 || batch > (int64_t(1) << 20) || epochs < 0) throw chili_error(CHILI_ERANGE, "sizes out of range");
+        auto& E = f->eng;
+        select_device(E.device);
+        const int C = E.nflows, d = E.d;
+        batch = std::min(batch, n);
+        const int64_t nbat = n / batch;
+        // one permutation of the points, fixed by the seed
+        std::vector<int64_t> perm(n);
+        for (int64_t i = 0; i < n; ++i) perm[i] = i;
+        std::mt19937_64 rng(seed);
+        std::shuffle(perm.begin(), perm.end(), rng);
+        DevBuf<int64_t> dperm;
+        dperm.upload(perm, E.stream);
+        DevBuf<double> br, bb, bw;
+        br.resize((size_t)C * batch * d);
+        bb.resize((size_t)C * batch);
+        bw.resize(batch);
+        std::vector<int64_t> order(nbat);
+        for (int64_t b = 0; b < nbat; ++b) order[b] = b;
+        for (int ep = 0; ep < epochs; ++ep) {
+            std::shuffle(order.begin(), order.end(), rng);
+            double ls = 0;
+            for (int64_t bi = 0; bi < nbat; ++bi) {
+                const int64_t base = order[bi] * batch;
+                for (int c = 0; c < C; ++c) {
+                    hipLaunchKernelGGL(k_gather_rows, dim3(nblk(batch)), dim3(kTh), 0, E.stream, batch, d, r_dev + (size_t)c * n * d,
+                                       dperm.ptr, base, br.ptr + (size_t)c * batch * d);
+                    hipLaunchKernelGGL(k_gather_rows, dim3(nblk(batch)), dim3(kTh), 0, E.stream, batch, 1, beta_dev + (size_t)c * n,
+                                       dperm.ptr, base, bb.ptr + (size_t)c * batch);
+                }
+                hipLaunchKernelGGL(k_gather_rows, dim3(nblk(batch)), dim3(kTh), 0, E.stream, batch, 1, w2_dev, dperm.ptr, base, bw.ptr);
+                ls += E.loss_grad(batch, br.ptr, bb.ptr, bw.ptr);
+                E.adam_step(lr);
+            }
+            if (loss_hist) loss_hist[ep] = nbat ? ls / (double)nbat : 0.0;
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(E.stream));
+    });
+}
+
+// MFMA GEMM probe for tests: C [M x N] fp32 = A [M x K] (bf16 bits, row
+// major) x B^T with B [N x K] (bf16 bits); host buffers.
+int chili_gemm_bf16_test(int M, int N, int K, const uint16_t* A, const uint16_t* B, float* C, int transA, int transB) {
+    return guarded([&] {
+        if (M <= 0 || N <= 0 || K <= 0 || M > 4096 || N > 4096 || K > 1 << 20) throw chili_error(CHILI_ERANGE, "sizes out of range");
+        select_device(0);
+        DevBuf<uint16_t> a, b;
+        DevBuf<float> c;
+        std::vector<uint16_t> av(A, A + (size_t)M * K), bv(B, B + (size_t)N * K);
+        a.upload(av, 0);
+        b.upload(bv, 0);
+        c.resize((size_t)M * N);
+        dim3 g((N + kGT - 1) / kGT, (M + kGT - 1) / kGT, 1);
+        const bf16* ap = (const bf16*)a.ptr;
+        const bf16* bp = (const bf16*)b.ptr;
+        // A stored [M][K] (transA = 0) or [K][M]; B stored [N][K] (transB = 1) or [K][N]
+        const int lda = transA ? M : K, ldb = transB ? K : N;
+        if (!transA && transB)
+            hipLaunchKernelGGL((k_gemm<false, true>), g, dim3(kGThreads), 0, 0, M, N, K, ap, lda, bp, ldb, c.ptr, N, (const float*)nullptr, 0,
+                               (bf16*)nullptr, 0, K, (int64_t)0);
+        else if (transA && !transB)
+            hipLaunchKernelGGL((k_gemm<true, false>), g, dim3(kGThreads), 0, 0, M, N, K, ap, lda, bp, ldb, c.ptr, N, (const float*)nullptr, 0,
+                               (bf16*)nullptr, 0, K, (int64_t)0);
+        else
+            throw chili_error(CHILI_EINVAL, "unsupported operand layout");
+        HIP_CHECK(hipGetLastError());
+        c.download(C, (size_t)M * N, 0);
+        HIP_CHECK(hipDeviceSynchronize());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// flow.hpp -- host engine of the learned proposal (coupling-layer spline flows).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "flow_kernels.hpp"
+
+namespace chili_nf {
+
+// One coupling layer: x[A] conditions an MLP  nA -> H -> H -> H -> nB (3K-1)
+// whose output parametrises the splines of x[B].  Parameter offsets index the
+// flat fp32 parameter vector; W stored [in][out].
+struct Layer {
+    std::vector<int> A, B;
+    int nA = 0, nB = 0, out = 0, ldA = 0;
+    size_t oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0, oW4 = 0, ob4 = 0;
+    size_t wo[4] = {0, 0, 0, 0};   // offsets of the four weights in the bf16 copies
+    DevBuf<int> dA, dB;
+};
+
+struct Flow {
+    DevBuf<float> params, grads, m, v;
+    DevBuf<bf16> wb, wt;           // bf16 weights [in][out] and transposed [out][in]
+    int64_t steps = 0;
+};
+
+struct Workspace {
+    DevBuf<double> x;              // layer inputs x_0..x_L  [L+1][n][d]
+    DevBuf<bf16> xa;               // conditioning inputs    [L][n][ldA]
+    DevBuf<bf16> h;                // hidden activations     [L][3][n][H]
+    DevBuf<float> raw;             // spline parameters      [L][n][out]
+    DevBuf<float> draw, gf, dxa;
+    DevBuf<bf16> drawb, gb;
+    DevBuf<double> gx;
+};
+
+constexpr int64_t kInferChunk = int64_t(1) << 20;   // points per inference pass
+
+struct FlowEngine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int d = 0, L = 0, H = 64, nflows = 0;
+    float shift = 0;   // softplus offset: slope 1 at zero input
+    size_t nparams = 0;
+    std::vector<Layer> layers;
+    std::vector<Flow> flows;
+    Workspace ws;
+    DevBuf<float> part, part2;
+    DevBuf<double> lqb, dlqb, lpart;
+
+    int ws_lda() const { int m = 0; for (auto& l : layers) m = m > l.ldA ? m : l.ldA; return m; }
+    int ws_out() const { int m = 0; for (auto& l : layers) m = m > l.out ? m : l.out; return m; }
+
+    void build_layers(uint64_t seed);
+    void init_params(uint64_t seed);
+    void refresh_bf16(int f);
+    void ensure_ws(int64_t nb, bool keep);
+    void mlp_forward(Flow& F, Layer& ly, int64_t nb, const double* x, bf16* xa, bf16* h, float* raw);
+    void gemm_fwd(int64_t M, int N, int K, const bf16* A, int lda, const bf16* Bt, int ldb, const float* bias, int relu, float* C,
+                  int ldc, bf16* Cb, int ldcb);
+    void gemm_wgrad(int64_t n, int in, int out, const bf16* A, int lda, const bf16* G, int ldg, float* dW);
+    void colsum(int64_t n, int m, const float* g, float* db);
+    void density(int f, int64_t nb, const double* r, double* lq, double* zout, bool keep);
+    void backward(int f, int64_t nb, const double* dlq);
+    void sample(int f, int64_t nb, double* xy, double* lq);
+    void density_chunked(int f, int64_t n, const double* r, double* lq);
+    void sample_chunked(int f, int64_t n, double* zr, double* lq);
+    double loss_grad(int64_t nb, const double* r, const double* beta, const double* w2);
+    void adam_step(double lr);
+};
+
+// Runs a flow engine's work on another stream (same device) for a scope.
+struct StreamSwap {
+    FlowEngine& e;
+    hipStream_t old;
+    StreamSwap(FlowEngine& eng, hipStream_t s) : e(eng), old(eng.stream) { e.stream = s; }
+    ~StreamSwap() { e.stream = old; }
+};
+
+}  // namespace chili_nf
+
+struct chili_flow {
+    chili_nf::FlowEngine eng;
+};

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "flow.hpp"
 #include "mc_kernels.hpp"
 
 using namespace chili_dev;
@@ -172,10 +173,10 @@ __device__ __forceinline__ bool locate_tile(int C, const int* __restrict__ seg_s
 // k_forward: VEGAS forward map, FSMapper::GeneratePoint and the cut for one
 // channel per workgroup.  Momenta of external legs go to mom[(i*4+k)*nmax + e].
 // ---------------------------------------------------------------------------
-template <int NT>
+template <int NT, bool ZS>
 __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restrict__ seg_start, const int* __restrict__ tile_start,
                                                    const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,
-                                                   uint8_t* __restrict__ acc, PointRec R) {
+                                                   uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {
     constexpr int D = 3 * (NT - 2) - 2;
     int ch, e0, e1;
     if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;
@@ -183,7 +184,16 @@ __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restri
     if (e >= e1) return;
     const uint32_t a = sorted[e];
     double r[D + 1];
-    load_uniforms<D>(A, A.a0 + a, r);
+    if (ZS) {
+        // hypercube point drawn by the channel's learned proposal ([e][D])
+#pragma unroll
+        for (int i = 0; i < D; ++i) r[i] = zsrc[(int64_t)e * D + i];
+        if (R.x)
+#pragma unroll
+            for (int i = 0; i < D; ++i) R.x[(int64_t)i * R.n + a] = r[i];
+    } else {
+        load_uniforms<D>(A, A.a0 + a, r);
+    }
     // VEGAS forward map on the picked channel's grid (AdaptiveMap.cc:40-56);
     // the multichannel path drops its Jacobian (Integrand.hh:107-110).
     const double* __restrict__ edges = A.edges + (int64_t)ch * D * (A.B + 1);
@@ -191,7 +201,7 @@ __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restri
 #pragma unroll
     for (int i = 0; i < D; ++i) {
         double pos = r[i] * Bd;
-        int idx = (int)pos;
+        int idx = min((int)pos, A.B - 1);   // r = 1 (reachable from a proposal) stays in the last bin
         double loc = pos - (double)idx;
         const double* ed = edges + i * (A.B + 1);
         double left = ed[idx];
@@ -578,6 +588,133 @@ __global__ __launch_bounds__(256) void k_reduce_final(int C, int DB, const int* 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Learned-proposal path.  Per channel c a flow q_c on the VEGAS hypercube:
+// the attempt's d uniforms z are mapped to r = T_c^{-1}(z) before the VEGAS
+// forward map, and the point's density becomes
+//   g(p) = sum_j alpha_j q_j(r_j(p)) / (ps_j(p) vegas_j(x_j(p))),
+// r_j the VEGAS inverse of channel j's inverse point x_j.  W = 1 / g.
+// ---------------------------------------------------------------------------
+// z[e][k] = uniform k of attempt sorted[e]
+template <int D>
+__global__ __launch_bounds__(kTile) void k_zgen(MCArgs A, int64_t n, const uint32_t* __restrict__ sorted, double* __restrict__ z) {
+    const int64_t e = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (e >= n) return;
+    double r[D + 1];
+    load_uniforms<D>(A, A.a0 + sorted[e], r);
+#pragma unroll
+    for (int k = 0; k < D; ++k) z[e * D + k] = r[k];
+}
+
+// All-channel inverse of every kept point: f, a_j = alpha_j / ps_j / vegas_j
+// ([j][stride]) and the hypercube points r_j ([j][stride][D]).
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_geom(MCArgs A, int64_t kept, const uint32_t* __restrict__ comp,
+                                                const double* __restrict__ mom, int64_t nmax, double* __restrict__ rall,
+                                                double* __restrict__ abuf, double* __restrict__ fbuf, int64_t stride) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (k >= kept) return;
+    const uint32_t sp = comp[k];
+    MomRF<2 * NT> P;
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
+        P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
+        P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
+        P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+    }
+    fbuf[k] = integrand<NT>(A.f, P);
+    for (int j = 0; j < A.C; ++j) {
+        double vw;
+        const double ps = generate_weight<NT>(A, A.progs[j], j, P, &vw, nullptr, 0, nullptr, nullptr, 0,
+                                              rall + ((int64_t)j * stride + k) * D);
+        abuf[(int64_t)j * stride + k] = A.alpha[j] / ps / vw;
+    }
+}
+
+// Training records of the kept points (optional outputs of k_combine):
+// r [C][cap][d], beta [C][cap] = alpha_j h_j / g, w2 [cap] = value^2.
+struct FlowData {
+    double* r;
+    double* beta;
+    double* w2;
+    int64_t off, cap;
+};
+
+// W = 1 / sum_j a_j q_j, value = W == 0 ? 0 : f W (NaN -> 0, counted);
+// per-workgroup StatsData partials [n, min, max, sum, sum2, nf, nnan].
+__global__ __launch_bounds__(kTile) void k_combine(int C, int d, int64_t kept, int64_t stride, const double* __restrict__ abuf,
+                                                   const double* __restrict__ lq, const double* __restrict__ fbuf,
+                                                   const double* __restrict__ rall, double* __restrict__ part,
+                                                   double* __restrict__ vals_out, int64_t vals_off, int64_t vals_cap, FlowData FD,
+                                                   const uint32_t* __restrict__ comp, const uint32_t* __restrict__ sorted, PointRec R) {
+    __shared__ double s_part[kWaves][kStatW];
+    const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double val = 0, W = 0;
+    bool valid = k < kept, isnanv = false;
+    if (valid) {
+        double wsum = 0;
+        for (int j = 0; j < C; ++j) {
+            const double q = lq ? exp(lq[(int64_t)j * stride + k]) : 1.0;
+            wsum += abuf[(int64_t)j * stride + k] * q;
+        }
+        W = 1.0 / wsum;
+        const double fv = fbuf[k];
+        val = (W == 0) ? 0.0 : fv * W;
+        isnanv = isnan(val);
+        if (isnanv) { val = 0; W = 0; }
+        if (vals_out && vals_off + k < vals_cap) vals_out[vals_off + k] = val;
+        if (FD.r && FD.off + k < FD.cap) {
+            const int64_t o = FD.off + k;
+            for (int j = 0; j < C; ++j) {
+                FD.beta[(int64_t)j * FD.cap + o] = isnanv ? 0.0 : abuf[(int64_t)j * stride + k] / wsum;
+                for (int i = 0; i < d; ++i) FD.r[((int64_t)j * FD.cap + o) * d + i] = rall[((int64_t)j * stride + k) * d + i];
+            }
+            FD.w2[o] = val * val;
+        }
+        if (R.status) {
+            const uint32_t att = sorted[comp[k]];
+            R.status[att] = isnanv ? 2 : 1;
+            R.w[att] = W;
+            R.f[att] = fv;
+            R.val[att] = val;
+            if (R.ps)
+                for (int j = 0; j < C; ++j) R.ps[(int64_t)j * R.n + att] = lq ? lq[(int64_t)j * stride + k] : 0.0;
+        }
+    }
+    double n = valid ? 1.0 : 0.0, s1 = val, s2 = val * val, nf = (valid && val != 0) ? 1.0 : 0.0, nn = isnanv ? 1.0 : 0.0;
+    double mn = valid ? val : std::numeric_limits<double>::max(), mx = valid ? val : std::numeric_limits<double>::min();
+    n = wave_sum(n); s1 = wave_sum(s1); s2 = wave_sum(s2); nf = wave_sum(nf); nn = wave_sum(nn);
+    mn = wave_min(mn); mx = wave_max(mx);
+    if (lane == 0) {
+        s_part[w][0] = n; s_part[w][1] = mn; s_part[w][2] = mx; s_part[w][3] = s1; s_part[w][4] = s2; s_part[w][5] = nf; s_part[w][6] = nn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a[kStatW] = {0, std::numeric_limits<double>::max(), std::numeric_limits<double>::min(), 0, 0, 0, 0};
+        for (int ww = 0; ww < kWaves; ++ww) stat_merge(a, s_part[ww]);
+        for (int q = 0; q < kStatW; ++q) part[(int64_t)blockIdx.x * kStatW + q] = a[q];
+    }
+}
+
+// tot[0..6] += partials 0..np-1 in index order (one workgroup, fixed tree).
+__global__ __launch_bounds__(256) void k_stats_final(int64_t np, const double* __restrict__ part, double* __restrict__ tot) {
+    __shared__ double s[256][kStatW];
+    double a[kStatW] = {0, std::numeric_limits<double>::max(), std::numeric_limits<double>::min(), 0, 0, 0, 0};
+    const int64_t per = (np + 255) / 256;
+    const int64_t b0 = threadIdx.x * per, b1 = std::min<int64_t>(np, b0 + per);
+    for (int64_t b = b0; b < b1; ++b) stat_merge(a, part + b * kStatW);
+    for (int q = 0; q < kStatW; ++q) s[threadIdx.x][q] = a[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) stat_merge(s[threadIdx.x], s[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) stat_merge(tot, s[0]);
+}
+
 // ===========================================================================
 // host context
 // ===========================================================================
@@ -599,6 +736,10 @@ struct chili_mc {
     DevBuf<uint8_t> d_acc;
     DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff, d_grp2;
     DevBuf<int64_t> d_total, d_bsum;
+    // learned-proposal path: z/r [fmax][d], r_j [C][fmax][d], a_j and log q_j
+    // [C][fmax], f [fmax], sampling log-density [fmax], StatsData partials
+    DevBuf<double> d_z, d_rall, d_abuf, d_lq, d_fbuf, d_lqs, d_fpart;
+    int64_t fmax = 0;
     int64_t nmax = 0;            // attempts per sub-batch
     double acc_rate = 0.25;      // running estimate of the cut acceptance
     // k_main timing with HIP events on the context's stream
@@ -701,9 +842,13 @@ MCArgs make_args(chili_mc* m, const double* u, int64_t ustride, uint64_t seed, u
 }
 
 template <int NT>
-void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R) {
-    hipLaunchKernelGGL(k_forward<NT>, dim3(grid), dim3(kTile), 0, m->stream, A, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr,
-                       m->d_mom.ptr, m->nmax, m->d_acc.ptr, R);
+void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, const double* z = nullptr) {
+    if (z)
+        hipLaunchKernelGGL((k_forward<NT, true>), dim3(grid), dim3(kTile), 0, m->stream, A, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr,
+                           m->d_mom.ptr, m->nmax, m->d_acc.ptr, R, z);
+    else
+        hipLaunchKernelGGL((k_forward<NT, false>), dim3(grid), dim3(kTile), 0, m->stream, A, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr,
+                           m->d_mom.ptr, m->nmax, m->d_acc.ptr, R, z);
 }
 template <int NT>
 void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals, int64_t vals_off, int64_t vals_cap,
@@ -767,34 +912,11 @@ void harvest_events(chili_mc* m) {
     m->fev_live.clear();
 }
 
-// One sub-batch of attempts [a0, a0 + n): returns kept (accepted below the
-// cutoff), the cutoff (attempts consumed) and NaN count, accumulating into d_tot.
-void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, uint64_t seed, uint32_t stream, uint64_t a0,
-                  int64_t n, int64_t need, int train, double* vals, int64_t vals_off, int64_t vals_cap, int64_t& kept, int64_t& used,
-                  const PointRec& R = PointRec{}) {
-    ensure_capacity(m, n);
+// Cutoff (the attempt at which `need` accepted attempts are reached; host
+// sync) and stable compaction of the kept entries of the forward tiles.
+void cutoff_and_compact(chili_mc* m, int64_t n, int nblk, int tgrid, int64_t need, int64_t& kept, int64_t& used) {
     const int C = m->C;
-    const int nblk = (int)((n + kTile - 1) / kTile);
-    MCArgs A = make_args(m, u, ustride, seed, stream, a0);
-    A.ua0 = ua0;
-    hipLaunchKernelGGL(k_pick, dim3(nblk), dim3(kTile), sizeof(int) * C, m->stream, A, n, m->d_pick.ptr, m->d_counts.ptr, nblk);
-    scan_int(m, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * C);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kTile), sizeof(int) * kWaves * C, m->stream, C, n, m->d_pick.ptr, m->d_offs.ptr, nblk,
-                       m->d_sorted.ptr);
-    hipLaunchKernelGGL(k_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_offs.ptr, nblk, n, m->d_seg.ptr, m->d_tiles.ptr);
-    const int tgrid = nblk + C;
-    std::pair<hipEvent_t, hipEvent_t> fev{};
-    if (m->timing) {
-        fev = take_events(m);
-        HIP_CHECK(hipEventRecord(fev.first, m->stream));
-    }
-    CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R));
-    if (m->timing) {
-        HIP_CHECK(hipEventRecord(fev.second, m->stream));
-        m->fev_live.push_back(fev);
-    }
     hipLaunchKernelGGL(k_acc_count, dim3(nblk), dim3(kTile), 0, m->stream, m->d_acc.ptr, n, m->d_blkacc.ptr);
-    // cutoff: the attempt at which `need` accepted attempts are reached
     std::vector<int> blkacc(nblk);
     m->d_blkacc.download(blkacc.data(), nblk, m->stream);
     HIP_CHECK(hipStreamSynchronize(m->stream));
@@ -825,6 +947,35 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
                        (uint32_t)cut, m->d_keptoff.ptr, m->d_comp.ptr);
     hipLaunchKernelGGL(k_keep_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_tiles.ptr, m->d_keptoff.ptr, kept, m->d_seg2.ptr,
                        m->d_tiles2.ptr, kMainTile);
+}
+
+// One sub-batch of attempts [a0, a0 + n): returns kept (accepted below the
+// cutoff), the cutoff (attempts consumed) and NaN count, accumulating into d_tot.
+void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, uint64_t seed, uint32_t stream, uint64_t a0,
+                  int64_t n, int64_t need, int train, double* vals, int64_t vals_off, int64_t vals_cap, int64_t& kept, int64_t& used,
+                  const PointRec& R = PointRec{}) {
+    ensure_capacity(m, n);
+    const int C = m->C;
+    const int nblk = (int)((n + kTile - 1) / kTile);
+    MCArgs A = make_args(m, u, ustride, seed, stream, a0);
+    A.ua0 = ua0;
+    hipLaunchKernelGGL(k_pick, dim3(nblk), dim3(kTile), sizeof(int) * C, m->stream, A, n, m->d_pick.ptr, m->d_counts.ptr, nblk);
+    scan_int(m, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * C);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kTile), sizeof(int) * kWaves * C, m->stream, C, n, m->d_pick.ptr, m->d_offs.ptr, nblk,
+                       m->d_sorted.ptr);
+    hipLaunchKernelGGL(k_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_offs.ptr, nblk, n, m->d_seg.ptr, m->d_tiles.ptr);
+    const int tgrid = nblk + C;
+    std::pair<hipEvent_t, hipEvent_t> fev{};
+    if (m->timing) {
+        fev = take_events(m);
+        HIP_CHECK(hipEventRecord(fev.first, m->stream));
+    }
+    CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R));
+    if (m->timing) {
+        HIP_CHECK(hipEventRecord(fev.second, m->stream));
+        m->fev_live.push_back(fev);
+    }
+    cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
     if (kept > 0) {
         const int mgrid = (int)((kept + kMainTile - 1) / kMainTile) + C;
         std::pair<hipEvent_t, hipEvent_t> ev{};
@@ -855,6 +1006,148 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
         hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, m->stream, C, m->d * m->B,
                            m->d_tiles2.ptr, m->d_part.ptr, rec, m->d_gsum.ptr, m->d_tot.ptr, train, m->d_grp2.ptr, m->d_hsum.ptr);
     }
+}
+
+
+// Host uniforms of a call on the device ([d+1][stride]); null for Philox.
+const double* stage_uniforms(chili_mc* m, const chili_uniforms* uni, int64_t& ustride) {
+    ustride = 0;
+    if (!uni->u) return nullptr;
+    if (uni->stride <= 0) throw chili_error(CHILI_EINVAL, "uniform stride must be positive");
+    const size_t cnt = (size_t)(m->d + 1) * uni->stride;
+    check_uniforms(uni->u, cnt);
+    m->d_u.resize(cnt);
+    HIP_CHECK(hipMemcpyAsync(m->d_u.ptr, uni->u, cnt * sizeof(double), hipMemcpyHostToDevice, m->stream));
+    ustride = uni->stride;
+    return m->d_u.ptr;
+}
+
+// The retry loop of one iteration (MultiChannel.hh:214-248): sub-batches of
+// attempts from a0 until ncalls points pass the cut with a finite value.
+// sub(u, ustride, first attempt, n, need, kept so far, kept, used) runs one
+// sub-batch and accumulates into d_tot; tot receives the totals and stats
+// the iteration's StatsData with a zero per rejected attempt.
+template <class F>
+void iterate_loop(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, int64_t cap_max, std::vector<double>& tot,
+                  int64_t& att, chili_stats* stats, F&& sub) {
+    int64_t ustride = 0;
+    const double* ud = stage_uniforms(m, uni, ustride);
+    const size_t ntot = kStatW + m->C + (size_t)m->C * m->d * m->B;
+    tot.assign(ntot, 0.0);
+    tot[1] = std::numeric_limits<double>::max();
+    tot[2] = std::numeric_limits<double>::min();
+    m->d_tot.upload(tot, m->stream);
+    int64_t need = ncalls, rejected = 0, got = 0;
+    att = 0;
+    double prev_nan = 0;
+    const int64_t cap = std::max<int64_t>(int64_t(1) << 16, std::min<int64_t>(cap_max, 8 * ncalls + 4096));
+    // the reference loops until ncalls points pass; stop instead of spinning
+    // when the cut (or NaN values) leave essentially nothing
+    const int64_t max_attempts = 100000 * ncalls + (int64_t(1) << 24);
+    while (need > 0) {
+        if (att > max_attempts) throw chili_error(CHILI_ERANGE, "no attempts pass the cut: retry limit reached");
+        int64_t n = (int64_t)((double)need / std::max(m->acc_rate, 0.02) * 1.08) + 2048;
+        n = std::min(n, cap);
+        if (ud) {
+            n = std::min<int64_t>(n, ustride - att);
+            if (n <= 0) throw chili_error(CHILI_ERANGE, "host uniforms exhausted before ncalls points passed");
+        }
+        int64_t kept = 0, used = 0;
+        sub(ud, ustride, a0 + (uint64_t)att, n, need, got, kept, used);
+        double nanc = 0;
+        HIP_CHECK(hipMemcpyAsync(&nanc, m->d_tot.ptr + 6, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+        const int64_t nan_new = (int64_t)(nanc - prev_nan);
+        prev_nan = nanc;
+        rejected += used - kept;
+        need -= kept - nan_new;
+        got += kept;
+        att += used;
+        if (used > 0) m->acc_rate = 0.5 * m->acc_rate + 0.5 * std::max(0.005, (double)kept / (double)used);
+    }
+    HIP_CHECK(hipMemcpyAsync(tot.data(), m->d_tot.ptr, ntot * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+    if (m->timing) harvest_events(m);
+    // rejected attempts add a zero each (MultiChannel.hh:219-223)
+    double n = tot[0] + (double)rejected;
+    double mn = tot[1], mx = tot[2];
+    if (rejected > 0) { mn = std::min(mn, 0.0); mx = std::max(mx, 0.0); }
+    stats->n = n; stats->min = mn; stats->max = mx; stats->sum = tot[3]; stats->sum2 = tot[4]; stats->n_finite = tot[5];
+}
+
+void ensure_flow_capacity(chili_mc* m, int64_t n) {
+    if (n <= m->fmax) return;
+    m->d_z.resize((size_t)n * m->d);
+    m->d_rall.resize((size_t)m->C * n * m->d);
+    m->d_abuf.resize((size_t)m->C * n);
+    m->d_lq.resize((size_t)m->C * n);
+    m->d_fbuf.resize(n);
+    m->d_lqs.resize(n);
+    m->d_fpart.resize((size_t)((n + kTile - 1) / kTile) * kStatW);
+    m->fmax = n;
+}
+
+template <int NT>
+void launch_zgen(chili_mc* m, const MCArgs& A, int64_t n) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    hipLaunchKernelGGL(k_zgen<D>, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, n, m->d_sorted.ptr,
+                       m->d_z.ptr);
+}
+template <int NT>
+void launch_geom(chili_mc* m, const MCArgs& A, int64_t kept) {
+    hipLaunchKernelGGL(k_geom<NT>, dim3((unsigned)((kept + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, kept, m->d_comp.ptr,
+                       m->d_mom.ptr, m->nmax, m->d_rall.ptr, m->d_abuf.ptr, m->d_fbuf.ptr, m->fmax);
+}
+
+// One sub-batch of the learned-proposal path (fl == null: every q_j = 1,
+// the plain multichannel weight through the same kernels).
+void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ustride, uint64_t ua0, uint64_t seed, uint32_t stream,
+                       uint64_t a0, int64_t n, int64_t need, double* vals, int64_t vals_off, int64_t vals_cap, FlowData FD,
+                       int64_t& kept, int64_t& used, const PointRec& R = PointRec{}) {
+    ensure_capacity(m, n);
+    ensure_flow_capacity(m, n);
+    const int C = m->C, d = m->d;
+    const int nblk = (int)((n + kTile - 1) / kTile);
+    MCArgs A = make_args(m, u, ustride, seed, stream, a0);
+    A.ua0 = ua0;
+    hipLaunchKernelGGL(k_pick, dim3(nblk), dim3(kTile), sizeof(int) * C, m->stream, A, n, m->d_pick.ptr, m->d_counts.ptr, nblk);
+    scan_int(m, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * C);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kTile), sizeof(int) * kWaves * C, m->stream, C, n, m->d_pick.ptr, m->d_offs.ptr, nblk,
+                       m->d_sorted.ptr);
+    hipLaunchKernelGGL(k_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_offs.ptr, nblk, n, m->d_seg.ptr, m->d_tiles.ptr);
+    const int tgrid = nblk + C;
+    if (fl) {
+        // z -> r through each channel's flow, on the channel-sorted segments
+        CHILI_DISPATCH_NT(m->NT, launch_zgen<NT>(m, A, n));
+        std::vector<int> seg(C + 1);
+        m->d_seg.download(seg.data(), C + 1, m->stream);
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+        chili_nf::StreamSwap sw(fl->eng, m->stream);
+        for (int c = 0; c < C; ++c) {
+            const int64_t cnt = (int64_t)seg[c + 1] - seg[c];
+            if (cnt > 0) fl->eng.sample_chunked(c, cnt, m->d_z.ptr + (size_t)seg[c] * d, m->d_lqs.ptr + seg[c]);
+        }
+    }
+    CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr));
+    cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
+    if (kept == 0) return;
+    CHILI_DISPATCH_NT(m->NT, launch_geom<NT>(m, A, kept));
+    if (fl) {
+        chili_nf::StreamSwap sw(fl->eng, m->stream);
+        for (int j = 0; j < C; ++j)
+            fl->eng.density_chunked(j, kept, m->d_rall.ptr + (size_t)j * m->fmax * d, m->d_lq.ptr + (size_t)j * m->fmax);
+    }
+    const int cg = (int)((kept + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_combine, dim3(cg), dim3(kTile), 0, m->stream, C, d, kept, m->fmax, m->d_abuf.ptr, fl ? m->d_lq.ptr : nullptr,
+                       m->d_fbuf.ptr, m->d_rall.ptr, m->d_fpart.ptr, vals, vals_off, vals_cap, FD, m->d_comp.ptr, m->d_sorted.ptr, R);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, m->stream, (int64_t)cg, m->d_fpart.ptr, m->d_tot.ptr);
+}
+
+void check_flow(const chili_mc* m, const chili_flow* fl) {
+    if (!fl) return;
+    if (fl->eng.device != m->device) throw chili_error(CHILI_EINVAL, "flow and context live on different devices");
+    if (fl->eng.d != m->d || fl->eng.nflows != m->C)
+        throw chili_error(CHILI_EINVAL, "flow dimensions / count do not match the channels");
 }
 
 }  // namespace
@@ -1008,58 +1301,14 @@ int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_
         if (!m || !uni || !stats) throw chili_error(CHILI_EINVAL, "null argument");
         if (ncalls <= 0) throw chili_error(CHILI_EINVAL, "ncalls must be positive");
         select_device(m->device);
-        const double* ud = nullptr;
-        int64_t ustride = 0;
-        if (uni->u) {
-            if (uni->stride <= 0) throw chili_error(CHILI_EINVAL, "uniform stride must be positive");
-            const size_t cnt = (size_t)(m->d + 1) * uni->stride;
-            check_uniforms(uni->u, cnt);
-            m->d_u.resize(cnt);
-            HIP_CHECK(hipMemcpyAsync(m->d_u.ptr, uni->u, cnt * sizeof(double), hipMemcpyHostToDevice, m->stream));
-            ud = m->d_u.ptr;
-            ustride = uni->stride;
-        }
-        const size_t ntot = kStatW + m->C + (size_t)m->C * m->d * m->B;
-        std::vector<double> tot(ntot, 0.0);
-        tot[1] = std::numeric_limits<double>::max();
-        tot[2] = std::numeric_limits<double>::min();
-        m->d_tot.upload(tot, m->stream);
-        int64_t need = ncalls, att = 0, rejected = 0, got = 0;
-        double prev_nan = 0;
-        const int64_t cap = std::max<int64_t>(int64_t(1) << 16, std::min<int64_t>(int64_t(1) << 23, 8 * ncalls + 4096));
-        // the reference loops until ncalls points pass; stop instead of spinning
-        // when the cut (or NaN values) leave essentially nothing
-        const int64_t max_attempts = 100000 * ncalls + (int64_t(1) << 24);
-        while (need > 0) {
-            if (att > max_attempts) throw chili_error(CHILI_ERANGE, "no attempts pass the cut: retry limit reached");
-            int64_t n = (int64_t)((double)need / std::max(m->acc_rate, 0.02) * 1.08) + 2048;
-            n = std::min(n, cap);
-            if (ud) {
-                n = std::min<int64_t>(n, ustride - att);
-                if (n <= 0) throw chili_error(CHILI_ERANGE, "host uniforms exhausted before ncalls points passed");
-            }
-            int64_t kept = 0, used = 0;
-            run_subbatch(m, ud, ustride, a0, uni->seed, uni->stream, a0 + (uint64_t)att, n, need, train, vals_dev, got, vals_cap,
-                         kept, used);
-            double nanc = 0;
-            HIP_CHECK(hipMemcpyAsync(&nanc, m->d_tot.ptr + 6, sizeof(double), hipMemcpyDeviceToHost, m->stream));
-            HIP_CHECK(hipStreamSynchronize(m->stream));
-            const int64_t nan_new = (int64_t)(nanc - prev_nan);
-            prev_nan = nanc;
-            rejected += used - kept;
-            need -= kept - nan_new;
-            got += kept;
-            att += used;
-            if (used > 0) m->acc_rate = 0.5 * m->acc_rate + 0.5 * std::max(0.005, (double)kept / (double)used);
-        }
-        HIP_CHECK(hipMemcpyAsync(tot.data(), m->d_tot.ptr, ntot * sizeof(double), hipMemcpyDeviceToHost, m->stream));
-        HIP_CHECK(hipStreamSynchronize(m->stream));
-        if (m->timing) harvest_events(m);
-        // rejected attempts add a zero each (MultiChannel.hh:219-223)
-        double n = tot[0] + (double)rejected;
-        double mn = tot[1], mx = tot[2];
-        if (rejected > 0) { mn = std::min(mn, 0.0); mx = std::max(mx, 0.0); }
-        stats->n = n; stats->min = mn; stats->max = mx; stats->sum = tot[3]; stats->sum2 = tot[4]; stats->n_finite = tot[5];
+        std::vector<double> tot;
+        int64_t att = 0;
+        iterate_loop(m, uni, a0, ncalls, int64_t(1) << 23, tot, att, stats,
+                     [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
+                         int64_t& used) {
+                         run_subbatch(m, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, train, vals_dev, got, vals_cap, kept,
+                                      used);
+                     });
         if (T && train) std::copy(tot.begin() + kStatW, tot.begin() + kStatW + m->C, T);
         if (hist && train) std::copy(tot.begin() + kStatW + m->C, tot.end(), hist);
         if (n_attempts) *n_attempts = att;
@@ -1135,6 +1384,98 @@ int chili_mc_points(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t
         if (x) xv.download(x, (size_t)m->d * n, m->stream);
         if (bins) bn.download(bins, (size_t)m->d * n, m->stream);
         if (mom) mv.download(mom, (size_t)m->NT * 4 * n, m->stream);
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Learned proposal (no reference counterpart).  fl (chili_flow_create with
+// nflows = channels, ndims = dimensions) may be NULL: every q_j = 1, which is
+// the plain multichannel weight evaluated through the same kernels.
+// ---------------------------------------------------------------------------
+// One production iteration: attempts from a0 until ncalls points pass (the
+// retry semantics of chili_mc_iterate); grids, alpha and flows stay fixed.
+int chili_mc_flow_iterate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, chili_stats* stats,
+                          int64_t* n_attempts, double* vals_dev, int64_t vals_cap) {
+    return guarded([&] {
+        if (!m || !uni || !stats) throw chili_error(CHILI_EINVAL, "null argument");
+        if (ncalls <= 0) throw chili_error(CHILI_EINVAL, "ncalls must be positive");
+        check_flow(m, fl);
+        select_device(m->device);
+        std::vector<double> tot;
+        int64_t att = 0;
+        iterate_loop(m, uni, a0, ncalls, int64_t(1) << 21, tot, att, stats,
+                     [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
+                         int64_t& used) {
+                         run_subbatch_flow(m, fl, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, vals_dev, got, vals_cap,
+                                           FlowData{nullptr, nullptr, nullptr, 0, 0}, kept, used);
+                     });
+        if (n_attempts) *n_attempts = att;
+    });
+}
+
+// Training records of one iteration drawn from the current proposal: for the
+// first min(ncalls, cap) kept points r_j (device [C][cap][d]), beta_j = alpha_j
+// h_j / g (device [C][cap]) and value^2 (device [cap]); n_kept receives the
+// number written.
+int chili_mc_flow_data(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, double* r_dev,
+                       double* beta_dev, double* w2_dev, int64_t cap, chili_stats* stats, int64_t* n_kept, int64_t* n_attempts) {
+    return guarded([&] {
+        if (!m || !uni || !stats || !r_dev || !beta_dev || !w2_dev) throw chili_error(CHILI_EINVAL, "null argument");
+        if (ncalls <= 0 || cap <= 0) throw chili_error(CHILI_EINVAL, "ncalls and cap must be positive");
+        check_flow(m, fl);
+        select_device(m->device);
+        std::vector<double> tot;
+        int64_t att = 0, total_kept = 0;
+        iterate_loop(m, uni, a0, ncalls, int64_t(1) << 21, tot, att, stats,
+                     [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
+                         int64_t& used) {
+                         run_subbatch_flow(m, fl, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, nullptr, 0, 0,
+                                           FlowData{r_dev, beta_dev, w2_dev, got, cap}, kept, used);
+                         total_kept = got + kept;
+                     });
+        if (n_kept) *n_kept = std::min(total_kept, cap);
+        if (n_attempts) *n_attempts = att;
+    });
+}
+
+// Per-attempt records of attempts [a0, a0+n) (no retry loop): status, pick,
+// w, f, val as chili_mc_points; lq[C][n] = log q_j at the point (0 without
+// flows); r[d][n] = the hypercube point the picked channel's flow drew.
+int chili_mc_flow_points(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t n, int32_t* status,
+                         int32_t* pick, double* w, double* f, double* val, double* lq, double* r) {
+    return guarded([&] {
+        if (!m || !uni) throw chili_error(CHILI_EINVAL, "null argument");
+        if (n <= 0 || n > (int64_t(1) << 22)) throw chili_error(CHILI_ERANGE, "n must be in [1, 2^22]");
+        check_flow(m, fl);
+        select_device(m->device);
+        int64_t ustride = 0;
+        const double* ud = stage_uniforms(m, uni, ustride);
+        if (ud && ustride < n) throw chili_error(CHILI_ERANGE, "host uniforms shorter than n");
+        DevBuf<int32_t> st, pk;
+        DevBuf<double> wv, fv, vv, lv, rv, mv;
+        st.resize(n); pk.resize(n); wv.resize(n); fv.resize(n); vv.resize(n);
+        lv.resize((size_t)m->C * n); rv.resize((size_t)m->d * n); mv.resize((size_t)m->NT * 4 * n);
+        HIP_CHECK(hipMemsetAsync(wv.ptr, 0, n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(fv.ptr, 0, n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(vv.ptr, 0, n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(lv.ptr, 0, (size_t)m->C * n * sizeof(double), m->stream));
+        HIP_CHECK(hipMemsetAsync(rv.ptr, 0, (size_t)m->d * n * sizeof(double), m->stream));
+        PointRec R{st.ptr, pk.ptr, wv.ptr, fv.ptr, vv.ptr, lv.ptr, fl ? rv.ptr : nullptr, nullptr, mv.ptr, n};
+        const size_t ntot = kStatW + m->C + (size_t)m->C * m->d * m->B;
+        std::vector<double> tot(ntot, 0.0);
+        m->d_tot.upload(tot, m->stream);
+        int64_t kept = 0, used = 0;
+        run_subbatch_flow(m, fl, ud, ustride, a0, uni->seed, uni->stream, a0, n, n, nullptr, 0, 0, FlowData{nullptr, nullptr, nullptr, 0, 0},
+                          kept, used, R);
+        HIP_CHECK(hipGetLastError());
+        if (status) st.download(status, n, m->stream);
+        if (pick) pk.download(pick, n, m->stream);
+        if (w) wv.download(w, n, m->stream);
+        if (f) fv.download(f, n, m->stream);
+        if (val) vv.download(val, n, m->stream);
+        if (lq) lv.download(lq, (size_t)m->C * n, m->stream);
+        if (r) rv.download(r, (size_t)m->d * n, m->stream);
         HIP_CHECK(hipStreamSynchronize(m->stream));
     });
 }

@@ -214,7 +214,7 @@ CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* 
 template <int NT>
 CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, MomRF<2 * NT>& P, double* vw_out,
                                  short* bins_out, int bstride, double* x_out = nullptr, int32_t* bin_rec = nullptr,
-                                 int64_t xstride = 0) {
+                                 int64_t xstride = 0, double* r_out = nullptr) {
     constexpr int ND = NT - 3;
     constexpr int NTT = NT - 2;
     const int nd = p.nd, nt = p.nt;
@@ -229,6 +229,12 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, Mom
         vw *= (e[b + 1] - e[b]) * Bd;
         if (bins_out) bins_out[slot * bstride] = (short)b;
         if (x_out) { x_out[slot * xstride] = x; bin_rec[slot * xstride] = b; }
+        if (r_out) {
+            // VEGAS inverse: the hypercube point the forward map sends to x
+            const double wb = e[b + 1] - e[b];
+            double r = (wb > 0) ? ((double)b + (x - e[b]) / wb) / Bd : (double)b / Bd;
+            r_out[slot] = fmin(fmax(r, 0.0), 1.0);
+        }
         ++slot;
     };
     // composite momenta, ascending decays (Channel.cc:47-50)

@@ -21,6 +21,10 @@ EXPORTS = [
     "chili_mc_set_alpha", "chili_mc_get_alpha", "chili_mc_iterate", "chili_mc_points", "chili_mc_timing",
     "chili_vegas_create", "chili_vegas_destroy", "chili_vegas_set_grid", "chili_vegas_get_grid", "chili_vegas_iterate",
     "chili_grid_adapt", "chili_grid_split", "chili_alpha_adapt", "chili_max_difference",
+    "chili_mc_flow_iterate", "chili_mc_flow_data", "chili_mc_flow_points",
+    "chili_flow_create", "chili_flow_destroy", "chili_flow_info", "chili_flow_mask", "chili_flow_get_params",
+    "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
+    "chili_flow_train", "chili_gemm_bf16_test",
 ]
 
 
@@ -66,6 +70,24 @@ def load():
     L.chili_alpha_adapt.argtypes = [C.c_int, D, D, C.c_double, C.c_double]
     L.chili_max_difference.restype = C.c_double
     L.chili_max_difference.argtypes = [C.c_int, D]
+    VP, F = C.c_void_p, C.POINTER(C.c_float)
+    L.chili_mc_flow_iterate.argtypes = [P, P, C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), C.POINTER(C.c_int64), VP, I64]
+    L.chili_mc_flow_data.argtypes = [P, P, C.POINTER(abi.Uniforms), U64, I64, VP, VP, VP, I64, C.POINTER(abi.Stats),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.chili_mc_flow_points.argtypes = [P, P, C.POINTER(abi.Uniforms), U64, I64, I32, I32, D, D, D, D, D]
+    L.chili_flow_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, U64, C.c_int, C.POINTER(P)]
+    L.chili_flow_destroy.argtypes = [P]
+    L.chili_flow_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.chili_flow_mask.argtypes = [P, C.c_int, I32, C.POINTER(C.c_int)]
+    L.chili_flow_get_params.argtypes = [P, C.c_int, F]
+    L.chili_flow_set_params.argtypes = [P, C.c_int, F]
+    L.chili_flow_get_grads.argtypes = [P, C.c_int, F]
+    L.chili_flow_log_density.argtypes = [P, C.c_int, I64, VP, VP, VP]
+    L.chili_flow_sample.argtypes = [P, C.c_int, I64, VP, VP]
+    L.chili_flow_loss_grad.argtypes = [P, I64, VP, VP, VP, D]
+    L.chili_flow_train.argtypes = [P, I64, VP, VP, VP, I64, C.c_int, C.c_double, U64, D]
+    L.chili_gemm_bf16_test.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), F, C.c_int,
+                                       C.c_int]
     _lib = L
     return L
 
@@ -178,6 +200,101 @@ class MultiChannelGPU:
         out["bins"] = out["bins"].T.copy()
         out["mom"] = out["mom"].T.reshape(n, nt, 4).copy()
         return out
+
+    # -- learned proposal ------------------------------------------------------
+    def flow_iterate(self, flow, uni, ncalls, a0=0, vals_ptr=None, vals_cap=0):
+        st = abi.Stats()
+        na = C.c_int64()
+        check(load().chili_mc_flow_iterate(self.h, flow.h if flow else None, C.byref(uni), a0, ncalls, C.byref(st), C.byref(na),
+                                           vals_ptr, vals_cap))
+        return st, na.value
+
+    def flow_data(self, flow, uni, ncalls, r_ptr, beta_ptr, w2_ptr, cap, a0=0):
+        st = abi.Stats()
+        nk, na = C.c_int64(), C.c_int64()
+        check(load().chili_mc_flow_data(self.h, flow.h if flow else None, C.byref(uni), a0, ncalls, r_ptr, beta_ptr, w2_ptr, cap,
+                                        C.byref(st), C.byref(nk), C.byref(na)))
+        return st, nk.value, na.value
+
+    def flow_points(self, flow, uni, a0, n):
+        d, c = self.ndims, self.nchannels
+        out = dict(status=np.zeros(n, np.int32), pick=np.zeros(n, np.int32), w=np.zeros(n), f=np.zeros(n), val=np.zeros(n),
+                   lq=np.zeros((c, n)), r=np.zeros((d, n)))
+        check(load().chili_mc_flow_points(self.h, flow.h if flow else None, C.byref(uni), a0, n, _i(out["status"]),
+                                          _i(out["pick"]), _d(out["w"]), _d(out["f"]), _d(out["val"]), _d(out["lq"]),
+                                          _d(out["r"])))
+        out["lq"] = out["lq"].T.copy()
+        out["r"] = out["r"].T.copy()
+        return out
+
+
+class FlowGPU:
+    """Per-channel coupling-layer spline flows (chili_flow_*); device buffers
+    are passed as raw pointers (e.g. torch tensors' data_ptr())."""
+
+    def __init__(self, ndims, nlayers, nflows, seed=1, hidden=64, device=0):
+        L = load()
+        h = C.c_void_p()
+        check(L.chili_flow_create(ndims, nlayers, hidden, nflows, seed, device, C.byref(h)))
+        self.h = h
+        npar, nd, nl, nf = C.c_int64(), C.c_int(), C.c_int(), C.c_int()
+        check(L.chili_flow_info(h, C.byref(npar), C.byref(nd), C.byref(nl), C.byref(nf)))
+        self.nparams, self.ndims, self.nlayers, self.nflows = npar.value, nd.value, nl.value, nf.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().chili_flow_destroy(self.h)
+            self.h = None
+
+    def masks(self):
+        out = []
+        for l in range(self.nlayers):
+            b = np.zeros(self.ndims, np.int32)
+            nb = C.c_int()
+            check(load().chili_flow_mask(self.h, l, _i(b), C.byref(nb)))
+            out.append(b[:nb.value].tolist())
+        return out
+
+    def params(self, flow):
+        p = np.zeros(self.nparams, np.float32)
+        check(load().chili_flow_get_params(self.h, flow, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, flow, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(load().chili_flow_set_params(self.h, flow, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def grads(self, flow):
+        g = np.zeros(self.nparams, np.float32)
+        check(load().chili_flow_get_grads(self.h, flow, g.ctypes.data_as(C.POINTER(C.c_float))))
+        return g
+
+    def log_density(self, flow, n, r_ptr, lq_ptr, z_ptr=None):
+        check(load().chili_flow_log_density(self.h, flow, n, r_ptr, lq_ptr, z_ptr))
+
+    def sample(self, flow, n, zr_ptr, lq_ptr):
+        check(load().chili_flow_sample(self.h, flow, n, zr_ptr, lq_ptr))
+
+    def loss_grad(self, n, r_ptr, beta_ptr, w2_ptr):
+        L = C.c_double()
+        check(load().chili_flow_loss_grad(self.h, n, r_ptr, beta_ptr, w2_ptr, C.byref(L)))
+        return L.value
+
+    def train(self, n, r_ptr, beta_ptr, w2_ptr, batch, epochs, lr, seed=7):
+        hist = np.zeros(max(epochs, 1))
+        check(load().chili_flow_train(self.h, n, r_ptr, beta_ptr, w2_ptr, batch, epochs, lr, seed, _d(hist)))
+        return hist[:epochs]
+
+
+def gemm_bf16_test(A_bits, B_bits, M, N, K, transA=False, transB=True):
+    """MFMA GEMM probe: A [M][K] (or [K][M] if transA) and B [N][K] (or
+    [K][N]) as bf16 bit patterns (uint16); returns C [M][N] fp32."""
+    a = np.ascontiguousarray(A_bits, dtype=np.uint16)
+    b = np.ascontiguousarray(B_bits, dtype=np.uint16)
+    out = np.zeros((M, N), np.float32)
+    check(load().chili_gemm_bf16_test(M, N, K, a.ctypes.data_as(C.POINTER(C.c_uint16)), b.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                      out.ctypes.data_as(C.POINTER(C.c_float)), 1 if transA else 0, 1 if transB else 0))
+    return out
 
 
 class VegasGPU:

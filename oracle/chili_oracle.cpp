@@ -345,7 +345,7 @@ static Real grid_forward(const Grid& g, Real* r) {                              
     Real jac = 1.0;
     for (size_t i = 0; i < g.dims; ++i) {
         Real pos = r[i] * (double)g.bins;
-        size_t idx = (size_t)D(pos);
+        size_t idx = std::min((size_t)D(pos), g.bins - 1);   // r = 1 (only from a learned proposal) stays in the last bin
         Real loc = pos - (double)idx;
         double left = g.lower(i, idx);
         double size = g.width(i, idx);
@@ -1283,6 +1283,22 @@ double oracle_grid_forward(int dims, int bins, const double* edges, double* r) {
     Real j = grid_forward(g, rr.data());
     for (int i = 0; i < dims; ++i) r[i] = D(rr[i]);
     return D(j);
+}
+
+// AdaptiveMap::GenerateWeight (AdaptiveMap.cc:58-65) of x, and the
+// hypercube point r that the forward map (AdaptiveMap.cc:40-56) sends to x:
+// r_i = (k + (x_i - e_k) / w_k) / B with k = FindBin(x_i).
+double oracle_grid_inverse(int dims, int bins, const double* edges, const double* x, double* r) {
+    Grid g; g.dims = (size_t)dims; g.bins = (size_t)bins; g.h.assign(edges, edges + dims * (bins + 1));
+    double jac = 1.0;
+    for (size_t i = 0; i < g.dims; ++i) {
+        const size_t k = grid_find_bin(g, i, x[i]);
+        const double w = g.width(i, k);
+        jac *= w * (double)g.bins;
+        double ri = (w > 0) ? ((double)k + (x[i] - g.lower(i, k)) / w) / (double)g.bins : (double)k / (double)g.bins;
+        r[i] = std::min(std::max(ri, 0.0), 1.0);
+    }
+    return jac;
 }
 
 // Plain VEGAS iteration, Vegas::operator() (src/Integrator/Vegas.cc:18-52),

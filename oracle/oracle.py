@@ -43,6 +43,8 @@ def lib(count=False):
         L.oracle_grid_adapt.argtypes = [C.c_int, C.c_int, D, C.c_double, D]
         L.oracle_grid_split.argtypes = [C.c_int, C.c_int, D, C.c_int, D]
         L.oracle_grid_find_bin.argtypes = [C.c_int, C.c_int, D, C.c_int, C.c_double]
+        L.oracle_grid_inverse.restype = C.c_double
+        L.oracle_grid_inverse.argtypes = [C.c_int, C.c_int, D, D, D]
         L.oracle_grid_forward.restype = C.c_double
         L.oracle_grid_forward.argtypes = [C.c_int, C.c_int, D, D]
         L.oracle_vegas_iterate.argtypes = [C.c_int, C.c_int, D, C.POINTER(abi.Integrand), C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), D, D]
@@ -210,6 +212,15 @@ def grid_forward(edges, r):
     rr = np.array(r, dtype=np.float64)
     jac = lib().oracle_grid_forward(e.shape[0], e.shape[1] - 1, _d(e), _d(rr))
     return rr, jac
+
+
+def grid_inverse(edges, x):
+    """(r, vegas weight) of x: the hypercube point the forward map sends to x."""
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    r = np.zeros(e.shape[0])
+    jac = lib().oracle_grid_inverse(e.shape[0], e.shape[1] - 1, _d(e), _d(xx), _d(r))
+    return r, jac
 
 
 def uniform_grid(dims, bins):

@@ -1,0 +1,297 @@
+"""GPU tests of the learned proposal: the MFMA GEMM's fragment layout, the
+flow's density / sampling / gradients against the CPU restatement
+(oracle/flow_oracle.py), and the multichannel path with flows against the
+plain path and the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from chili_amd import abi, lib
+from chili_amd import processes as P
+from oracle import flow_oracle as FO
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0xC0FFEE03
+
+
+def dev(a):
+    # callers keep the tensor alive while the library reads its pointer
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def bf16_bits(x):
+    t = torch.as_tensor(np.asarray(x, dtype=np.float32)).to(torch.bfloat16)
+    return t.view(torch.int16).numpy().view(np.uint16), t.to(torch.float64).numpy()
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (96, 70, 80), (200, 235, 64), (5, 64, 8)])
+def test_gemm_mfma_layout(shape):
+    M, N, K = shape
+    rng = np.random.default_rng(1)
+    a_bits, a = bf16_bits(rng.normal(size=(M, K)))
+    b_bits, b = bf16_bits(rng.normal(size=(N, K)))
+    c = lib.gemm_bf16_test(a_bits, b_bits, M, N, K, transA=False, transB=True)
+    ref = a @ b.T
+    err = np.abs(c - ref) / (np.abs(a) @ np.abs(b).T)
+    assert err.max() < 1e-6, f"A B^T: max scaled error {err.max():.2e}"
+    # transposed-A / plain-B form (weight gradients): A stored [K][M], B [K][N]
+    at_bits, at = bf16_bits(rng.normal(size=(K, M)))
+    bn_bits, bn = bf16_bits(rng.normal(size=(K, N)))
+    c2 = lib.gemm_bf16_test(at_bits, bn_bits, M, N, K, transA=True, transB=False)
+    ref2 = at.T @ bn
+    err2 = np.abs(c2 - ref2) / (np.abs(at.T) @ np.abs(bn))
+    assert err2.max() < 1e-6, f"A^T B: max scaled error {err2.max():.2e}"
+
+
+def make_flow(d, L, nflows, scale, seed=3):
+    fl = lib.FlowGPU(d, L, nflows, seed=seed)
+    rng = np.random.default_rng(seed)
+    ps = []
+    for j in range(nflows):
+        p = fl.params(j)
+        if scale > 0:
+            p = (p + rng.normal(0, scale, p.shape)).astype(np.float32)
+            fl.set_params(j, p)
+        ps.append(p)
+    return fl, ps
+
+
+def test_flow_identity_at_init():
+    fl, _ = make_flow(10, 6, 2, 0.0)
+    n = 4096
+    r = torch.rand(n, 10, dtype=torch.float64, device="cuda")
+    lq = torch.empty(n, dtype=torch.float64, device="cuda")
+    z = torch.empty_like(r)
+    fl.log_density(1, n, r.data_ptr(), lq.data_ptr(), z.data_ptr())
+    torch.cuda.synchronize()
+    assert float(lq.abs().max()) < 1e-6
+    assert float((z - r).abs().max()) < 1e-7
+
+
+@pytest.mark.parametrize("d,L", [(10, 6), (16, 8), (4, 2)])
+def test_flow_density_vs_oracle(d, L):
+    fl, ps = make_flow(d, L, 1, 0.05, seed=d)
+    o = FO.FlowOracle(d, fl.masks(), mode="gpu")
+    assert o.nparams == fl.nparams
+    n = 4096
+    rng = np.random.default_rng(5)
+    r = rng.random((n, d))
+    lq = torch.empty(n, dtype=torch.float64, device="cuda")
+    z = torch.empty(n, d, dtype=torch.float64, device="cuda")
+    rd = dev(r)
+    fl.log_density(0, n, rd.data_ptr(), lq.data_ptr(), z.data_ptr())
+    torch.cuda.synchronize()
+    with torch.no_grad():
+        lo, zo = o.density(torch.tensor(ps[0], dtype=torch.float64), torch.tensor(r))
+    dl = np.abs(lq.cpu().numpy() - lo.numpy())
+    dz = np.abs(z.cpu().numpy() - zo.numpy())
+    spread = float(lo.std())
+    print(f"d={d} L={L}: log q spread {spread:.3f}; |dlq| median {np.median(dl):.1e} p99 {np.quantile(dl, .99):.1e} "
+          f"max {dl.max():.1e}; |dz| max {dz.max():.1e}")
+    assert spread > 0.05
+    # same bf16 operands; FP32 accumulation order differs, which can flip the
+    # bf16 rounding of an activation now and then
+    assert np.median(dl) < 1e-5
+    assert np.quantile(dl, 0.99) < 1e-3
+    assert dl.max() < 0.02 * spread
+
+
+def test_flow_sample_inverts_density():
+    d = 10
+    fl, _ = make_flow(d, 6, 1, 0.05, seed=4)
+    n = 8192
+    z = torch.rand(n, d, dtype=torch.float64, device="cuda")
+    r = z.clone()
+    lqs = torch.empty(n, dtype=torch.float64, device="cuda")
+    fl.sample(0, n, r.data_ptr(), lqs.data_ptr())
+    lqd = torch.empty_like(lqs)
+    z2 = torch.empty_like(z)
+    fl.log_density(0, n, r.data_ptr(), lqd.data_ptr(), z2.data_ptr())
+    torch.cuda.synchronize()
+    assert float((r - z).abs().max()) > 1e-3          # not the identity
+    assert float((z2 - z).abs().max()) < 1e-10
+    assert float((lqs - lqd).abs().max()) < 1e-8
+
+
+def test_flow_normalised_2d():
+    # the density integrates to one whatever the network computes
+    fl, _ = make_flow(2, 4, 1, 0.1, seed=6)
+    n = 600
+    c = (np.arange(n) + 0.5) / n
+    r = np.stack(np.meshgrid(c, c, indexing="ij"), -1).reshape(-1, 2)
+    lq = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    rd = dev(r)
+    fl.log_density(0, n * n, rd.data_ptr(), lq.data_ptr())
+    q = torch.exp(lq).cpu().numpy()
+    assert q.max() / q.min() > 5
+    assert abs(q.mean() - 1) < 5e-5
+
+
+def test_flow_gradients_vs_oracle():
+    d, L, C = 6, 4, 2
+    fl, ps = make_flow(d, L, C, 0.05, seed=8)
+    o = [FO.FlowOracle(d, fl.masks(), mode="gpu") for _ in range(C)]
+    rng = np.random.default_rng(11)
+    n = 2048
+    r = rng.random((C, n, d))
+    beta = rng.random((C, n)) + 0.05
+    beta /= beta.sum(0)
+    w2 = rng.gamma(1.0, 1.0, n)
+    rd, bd, wd = dev(r), dev(beta), dev(w2)
+    loss = fl.loss_grad(n, rd.data_ptr(), bd.data_ptr(), wd.data_ptr())
+    Lo, go = FO.loss_and_grads(o, ps, r, beta, w2)
+    assert abs(loss - Lo) <= 1e-4 * abs(Lo)
+    for j in range(C):
+        g = fl.grads(j).astype(np.float64)
+        rel = np.linalg.norm(g - go[j]) / np.linalg.norm(go[j])
+        cos = g @ go[j] / (np.linalg.norm(g) * np.linalg.norm(go[j]))
+        print(f"flow {j}: |g - g_oracle| / |g_oracle| = {rel:.2e}, cos = {cos:.6f}")
+        # the backward pass rounds gradients to bf16 at each product
+        assert rel < 3e-2 and cos > 0.999
+
+
+def test_flow_training_reduces_loss():
+    d, L, C = 4, 4, 1
+    fl, _ = make_flow(d, L, C, 0.0, seed=9)
+    # target density on [0,1]^4: a product of peaked profiles; samples drawn
+    # uniformly, so beta = 1 and w2 = f^2 (the variance loss of q against f)
+    rng = np.random.default_rng(12)
+    n = 1 << 16
+    r = rng.random((C, n, d))
+    f = np.prod(1.0 + 4.0 * np.exp(-((r[0] - 0.3) ** 2) / 0.02), axis=1)
+    f /= f.mean()
+    beta = np.ones((C, n))
+    w2 = f ** 2
+    rd, bd, wd = dev(r), dev(beta), dev(w2)
+    hist = fl.train(n, rd.data_ptr(), bd.data_ptr(), wd.data_ptr(), 4096, 12, 2e-3, seed=1)
+    print("loss by epoch:", np.array2string(hist, precision=4))
+    # loss = E[f^2 / q] = 1 + var(f/q) for normalised f: starts at E[f^2]
+    assert hist[0] > hist[-1]
+    assert hist[-1] < 0.5 * float(np.mean(w2))
+
+
+# ---------------------------------------------------------------------------
+# multichannel path with flows
+# ---------------------------------------------------------------------------
+def adapted_context(case="gg_3g", B=64, iters=3, seed=SEED):
+    proc, f = (P.gluon_process(3), P.integrand_propprod()) if case == "gg_3g" else (P.gluon_process(4, smax=3), P.integrand_propprod())
+    ch = lib.Channels(proc)
+    gpu = lib.MultiChannelGPU(ch, B, f)
+    uni = abi.philox_uniforms(seed, stream=1)
+    a0 = 0
+    for it in range(iters):
+        st, T, hist, na = gpu.iterate(uni, 20000, a0=a0, train=True)
+        a0 += na
+        gpu.set_alpha(lib.alpha_adapt(gpu.alpha(), T))
+        g = gpu.grids()
+        for c in range(gpu.nchannels):
+            if hist[c].sum() > 0:
+                g[c] = lib.grid_adapt(g[c], hist[c])
+        gpu.set_grids(g)
+    return proc, f, ch, gpu
+
+
+def test_flow_path_without_flows_is_plain_path():
+    proc, f, ch, gpu = adapted_context()
+    uni = abi.philox_uniforms(SEED + 9, stream=2)
+    n = 20000
+    a = gpu.points(uni, 5, n)
+    b = gpu.flow_points(None, uni, 5, n)
+    assert np.array_equal(a["status"], b["status"])
+    assert np.array_equal(a["pick"], b["pick"])
+    ok = a["status"] == 1
+    assert np.array_equal(a["w"][ok], b["w"][ok])
+    assert np.array_equal(a["val"][ok], b["val"][ok])
+
+
+def test_flow_path_identity_flows():
+    proc, f, ch, gpu = adapted_context()
+    fl = lib.FlowGPU(gpu.ndims, 4, gpu.nchannels, seed=2)
+    uni = abi.philox_uniforms(SEED + 9, stream=2)
+    n = 20000
+    a = gpu.points(uni, 5, n)
+    b = gpu.flow_points(fl, uni, 5, n)
+    assert np.array_equal(a["pick"], b["pick"])
+    ok = (a["status"] == 1) & (b["status"] == 1)
+    assert ok.sum() > 0.9 * (a["status"] == 1).sum()
+    d = np.abs(b["w"][ok] - a["w"][ok]) / np.abs(a["w"][ok])
+    print(f"identity flows: rel dW median {np.median(d):.1e} max {d.max():.1e}; |lq| max {np.abs(b['lq'][ok]).max():.1e}")
+    assert np.median(d) < 1e-6 and np.quantile(d, 0.99) < 1e-4
+
+
+def test_flow_points_vs_oracle():
+    """End to end: the oracle recomputes the proposal's weight of every
+    accepted point from the hypercube point the GPU's flow drew."""
+    proc, f, ch, gpu = adapted_context()
+    fl, ps = make_flow(gpu.ndims, 4, gpu.nchannels, 0.03, seed=13)
+    ora = O.Oracle(proc, gpu.nbins, f)
+    ora.set_grids(gpu.grids())
+    ora.set_alpha(gpu.alpha())
+    alpha = gpu.alpha()
+    grids = gpu.grids()
+    fos = FO.FlowOracle(gpu.ndims, fl.masks(), mode="gpu")
+    uni = abi.philox_uniforms(SEED + 4, stream=5)
+    n = 1500
+    g = gpu.flow_points(fl, uni, 11, n)
+    # the flow's sample of each attempt vs the oracle's inverse flow
+    z = np.array([[O.philox_uniform(uni.seed, uni.stream, 11 + i, k) for k in range(gpu.ndims)] for i in range(n)])
+    pick = np.array([O.pick_channel(alpha, O.philox_uniform(uni.seed, uni.stream, 11 + i, gpu.ndims)) for i in range(n)])
+    assert np.array_equal(pick, g["pick"])
+    for c in range(gpu.nchannels):
+        sel = pick == c
+        if not sel.any():
+            continue
+        with torch.no_grad():
+            ro, _ = fos.sample(torch.tensor(ps[c], dtype=torch.float64), torch.tensor(z[sel]))
+        assert np.abs(ro.numpy() - g["r"][sel]).max() < 1e-3
+    ok = np.nonzero(g["status"] == 1)[0]
+    assert len(ok) > 100
+    dW, dWst, dlq = [], [], []
+    for i in ok:
+        c = g["pick"][i]
+        r = g["r"][i].copy()
+        x, _ = O.grid_forward(grids[c], r)
+        mom = ora.generate_point(c, x)
+        fv, passed = O.integrand(f, mom, proc.sqrts)
+        assert passed
+        tot, tot_st = 0.0, 0.0
+        for j in range(gpu.nchannels):
+            psj, xj = ora.generate_weight(j, mom)
+            rj, vwj = O.grid_inverse(grids[j], xj)
+            with torch.no_grad():
+                lo, _ = fos.density(torch.tensor(ps[j], dtype=torch.float64), torch.tensor(rj[None, :]))
+            lo = float(lo[0])
+            dlq.append(abs(lo - g["lq"][i, j]))
+            tot += alpha[j] * np.exp(lo) / psj / vwj
+            tot_st += alpha[j] * np.exp(g["lq"][i, j]) / psj / vwj
+        dW.append(abs(1 / tot - g["w"][i]) / abs(1 / tot))
+        dWst.append(abs(1 / tot_st - g["w"][i]) / abs(1 / tot_st))
+    dW, dWst, dlq = map(np.array, (dW, dWst, dlq))
+    print(f"flow points: {len(ok)} accepted; staged rel dW median {np.median(dWst):.1e} max {dWst.max():.1e}; "
+          f"full rel dW median {np.median(dW):.1e} max {dW.max():.1e}; |dlq| median {np.median(dlq):.1e} max {dlq.max():.1e}")
+    # staged (the GPU's log q): the remaining weight algebra agrees as the plain path's end-to-end weights do
+    assert np.median(dWst) <= 1e-10 and dWst.max() <= 1e-6
+    # full: the oracle's own log q (bf16 MLP, FP32 accumulation order)
+    assert np.median(dW) < 1e-5 and dW.max() < 1e-2
+
+
+def test_flow_path_unbiased():
+    """Any proposal gives the same integral: random (untrained) flows against
+    the plain path, independent samples, within 4 combined sigma."""
+    proc, f, ch, gpu = adapted_context()
+    fl, _ = make_flow(gpu.ndims, 4, gpu.nchannels, 0.05, seed=21)
+    n = 400000
+    st1, _, _, _ = gpu.iterate(abi.philox_uniforms(SEED + 30, stream=1), n, train=False)
+    st2, _ = gpu.flow_iterate(fl, abi.philox_uniforms(SEED + 31, stream=1), n)
+
+    def mean_err(st):
+        m = st.sum / st.n
+        var = (st.sum2 / st.n - m * m) / (st.n - 1)
+        return m, np.sqrt(var)
+
+    m1, e1 = mean_err(st1)
+    m2, e2 = mean_err(st2)
+    print(f"plain {m1:.6e} +- {e1:.2e}; random flows {m2:.6e} +- {e2:.2e}")
+    assert abs(m1 - m2) < 4 * np.hypot(e1, e2)
