@@ -109,14 +109,15 @@ __global__ void k_scatter_add(int64_t n, int d, const float* __restrict__ dxa, i
     for (int c = 0; c < nA; ++c) gx[i * d + A[c]] += (double)dxa[i * ld + c];
 }
 
-// Column sums of g [n x m] in fixed order: partials over row chunks.
+// Column sums of g [n x m] in fixed order: block z sums rows
+// [z chunk, (z+1) chunk) of every column (coalesced over columns).
 __global__ void k_colsum_part(int64_t n, int m, const float* __restrict__ g, int64_t chunk, float* __restrict__ part) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m) return;
-    const int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = std::min<int64_t>(n, r0 + chunk);
-    float s = 0;
-    for (int64_t r = r0; r < r1; ++r) s += g[r * m + c];
-    part[(int64_t)blockIdx.y * m + c] = s;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk, r1 = std::min<int64_t>(n, r0 + chunk);
+    for (int c = threadIdx.x; c < m; c += blockDim.x) {
+        float s = 0;
+        for (int64_t r = r0; r < r1; ++r) s += g[r * m + c];
+        part[(int64_t)blockIdx.x * m + c] = s;
+    }
 }
 
 // dst[k] += sum_z part[z][k], z in order
@@ -171,14 +172,65 @@ __global__ void k_weights_bf16(int in, int out, const float* __restrict__ W, bf1
     Wt[c * in + r] = f2bf(W[i]);
 }
 
-__global__ void k_fill_f64(int64_t n, double v, double* __restrict__ x) {
+// Swizzle one flow's fp32 parameters (layer l: W1[nA][H] b1 W2[H][H] b2 W3 b3
+// W4[H][out] b4[out]) into the fused layout.  One thread per output element.
+__global__ void k_swizzle(const float* __restrict__ p, int64_t pW1, int64_t pb1, int64_t pW2, int64_t pb2, int64_t pW3, int64_t pb3,
+                          int64_t pW4, int64_t pb4, int nA, int nB, bf16* __restrict__ w, float* __restrict__ b) {
+    const FusedLayout FL = fused_layout(nB);
+    const int out = nB * kRaw;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
+    if (i < FL.wel) {
+        const int jj = (int)(i & 7), lane = (int)((i >> 3) & 63), h = lane >> 5;
+        const int64_t frag = i >> 9;            // 512 elements per fragment block
+        float v = 0.f;
+        if (i < FL.oW2) {                       // W1: [to][lane][8], natural k
+            const int to = (int)frag, rowo = 32 * to + (lane & 31), k = 8 * h + jj;
+            if (k < nA) v = p[pW1 + (int64_t)k * kHid + rowo];
+        } else if (i < FL.oW4) {                // W2 / W3: [to][s][lane][8]
+            const bool second = i >= FL.oW3;
+            const int64_t f = (i - (second ? FL.oW3 : FL.oW2)) >> 9;
+            const int to = (int)(f / 4), s = (int)(f % 4), rowo = 32 * to + (lane & 31), k = phi_k(s, h, jj);
+            v = p[(second ? pW3 : pW2) + (int64_t)k * kHid + rowo];
+        } else {                                // W4: [3 npair tiles][s][lane][8]
+            const int64_t f = (i - FL.oW4) >> 9;
+            const int T = (int)(f / 4), s = (int)(f % 4), P = T / 3, tt = T % 3;
+            const int rp = 32 * tt + (lane & 31), ds = rp / 48, q = rp % 48, dim = 2 * P + ds;
+            if (q < kRaw && dim < nB) v = p[pW4 + (int64_t)phi_k(s, h, jj) * out + dim * kRaw + q];
+        }
+        w[i] = f2bf(v);
+    }
+    if (i < FL.bel) {
+        float v = 0.f;
+        if (i < 64) v = p[pb1 + i];
+        else if (i < 128) v = p[pb2 + i - 64];
+        else if (i < 192) v = p[pb3 + i - 128];
+        else {
+            const int64_t k = i - 192;
+            const int P = (int)(k / kPairRows), rp = (int)(k % kPairRows), ds = rp / 48, q = rp % 48, dim = 2 * P + ds;
+            if (q < kRaw && dim < nB) v = p[pb4 + dim * kRaw + q];
+        }
+        b[i] = v;
+    }
 }
-__global__ void k_fill_f32(int64_t n, float v, float* __restrict__ x) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
+
+// Work-item table of a fused pass: flow j's rows start at start[j] (seg[j],
+// or j * stride) and number count[j] (seg[j+1] - seg[j], or ucount); items[]
+// prefix-sums ceil(count / chunk).
+__global__ void k_items(int nfl, const int* __restrict__ seg, int64_t ucount, int64_t stride, int64_t chunk, int64_t* start,
+                        int64_t* count, int* items) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int acc = 0;
+    for (int j = 0; j < nfl; ++j) {
+        const int64_t s0 = seg ? (int64_t)seg[j] : (int64_t)j * stride;
+        const int64_t c = seg ? (int64_t)seg[j + 1] - seg[j] : ucount;
+        start[j] = s0;
+        count[j] = c;
+        items[j] = acc;
+        acc += (int)((c + chunk - 1) / chunk);
+    }
+    items[nfl] = acc;
 }
+
 // out[i] = in[perm[i]] rows of width w (fp64)
 __global__ void k_gather_rows(int64_t n, int w, const double* __restrict__ in, const int64_t* __restrict__ perm, int64_t base,
                               double* __restrict__ out) {
@@ -243,6 +295,13 @@ void FlowEngine::build_layers(uint64_t seed) {
         ly.dA.upload(a, stream);
         ly.dB.upload(b, stream);
     }
+    fwstride = 0;
+    fbstride = 0;
+    for (auto& ly : layers) {
+        const FusedLayout FL = fused_layout(ly.nB);
+        fwstride = std::max<int64_t>(fwstride, ((FL.wel + 7) / 8) * 8);
+        fbstride = std::max<int64_t>(fbstride, ((FL.bel + 3) / 4) * 4);
+    }
 }
 
 void FlowEngine::init_params(uint64_t seed) {
@@ -272,6 +331,17 @@ void FlowEngine::init_params(uint64_t seed) {
 
 void FlowEngine::refresh_bf16(int f) {
     Flow& F = flows[f];
+    fusedW.resize((size_t)nflows * L * fwstride);
+    fusedB.resize((size_t)nflows * L * fbstride);
+    for (int l = 0; l < L; ++l) {
+        const Layer& ly = layers[l];
+        const FusedLayout FL = fused_layout(ly.nB);
+        const int64_t cnt = std::max(FL.wel, FL.bel);
+        const int64_t blk = (int64_t)f * L + l;
+        hipLaunchKernelGGL(k_swizzle, dim3(nblk(cnt)), dim3(kTh), 0, stream, F.params.ptr, (int64_t)ly.oW1, (int64_t)ly.ob1,
+                           (int64_t)ly.oW2, (int64_t)ly.ob2, (int64_t)ly.oW3, (int64_t)ly.ob3, (int64_t)ly.oW4, (int64_t)ly.ob4, ly.nA,
+                           ly.nB, fusedW.ptr + blk * fwstride, fusedB.ptr + blk * fbstride);
+    }
     size_t wt = 0;
     for (auto& ly : layers) wt += (size_t)ly.nA * H + 2 * (size_t)H * H + (size_t)H * ly.out;
     F.wb.resize(wt);
@@ -318,7 +388,7 @@ void FlowEngine::gemm_fwd(int64_t M, int N, int K, const bf16* A, int lda, const
 }
 // dW [in x out] += A^T G with A [n x in] (bf16), G [n x out] (bf16), split over n
 void FlowEngine::gemm_wgrad(int64_t n, int in, int out, const bf16* A, int lda, const bf16* G, int ldg, float* dW) {
-    const int chunk = 4096;
+    const int chunk = 256;     // points per split-K partial: enough workgroups to fill the chip
     const int nz = (int)((n + chunk - 1) / chunk);
     part.resize((size_t)nz * in * out);
     dim3 g((out + kGT - 1) / kGT, (in + kGT - 1) / kGT, nz);
@@ -327,10 +397,10 @@ void FlowEngine::gemm_wgrad(int64_t n, int in, int out, const bf16* A, int lda, 
     hipLaunchKernelGGL(k_sum_parts, dim3(nblk((int64_t)in * out)), dim3(kTh), 0, stream, (int64_t)in * out, nz, part.ptr, dW);
 }
 void FlowEngine::colsum(int64_t n, int m, const float* g, float* db) {
-    const int64_t chunk = 2048;
+    const int64_t chunk = 256;
     const int nz = (int)((n + chunk - 1) / chunk);
     part2.resize((size_t)nz * m);
-    hipLaunchKernelGGL(k_colsum_part, dim3((m + 63) / 64, nz), dim3(64), 0, stream, n, m, g, chunk, part2.ptr);
+    hipLaunchKernelGGL(k_colsum_part, dim3(nz), dim3(256), 0, stream, n, m, g, chunk, part2.ptr);
     hipLaunchKernelGGL(k_sum_parts, dim3(nblk(m)), dim3(kTh), 0, stream, (int64_t)m, nz, part2.ptr, db);
 }
 
@@ -340,7 +410,7 @@ void FlowEngine::density(int f, int64_t nb, const double* r, double* lq, double*
     ensure_ws(nb, keep);
     Flow& F = flows[f];
     HIP_CHECK(hipMemcpyAsync(ws.x.ptr, r, (size_t)nb * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
-    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, 0.0, lq);
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, 0.0, lq);
     const int lda = ws_lda(), ldo = ws_out();
     for (int l = 0; l < L; ++l) {
         Layer& ly = layers[l];
@@ -414,24 +484,89 @@ void FlowEngine::backward(int f, int64_t nb, const double* dlq) {
     }
 }
 
-// Sampling direction: y (fp64 [nb x d], in place) from z to r; lq = log q(r).
-void FlowEngine::sample(int f, int64_t nb, double* xy, double* lq) {
-    ensure_ws(nb, false);
-    Flow& F = flows[f];
-    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, 0.0, lq);
-    for (int l = L - 1; l >= 0; --l) {
-        Layer& ly = layers[l];
-        mlp_forward(F, ly, nb, xy, ws.xa.ptr, ws.h.ptr, ws.raw.ptr);
-        hipLaunchKernelGGL(k_spline_inv<kSplineBins>, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, d, xy, ws.raw.ptr, ly.out, ly.dB.ptr, ly.nB,
-                           shift, lq);
+template <bool INV, int NP>
+static void launch_couple_np(dim3 g, dim3 b, size_t lds, hipStream_t st, const CoupleArgs& a) {
+    HIP_CHECK(hipFuncSetAttribute((const void*)k_couple<kSplineBins, INV, NP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_couple<kSplineBins, INV, NP>), g, b, lds, st, a);
+}
+
+void FlowEngine::launch_couple(bool inverse, int npair, dim3 g, dim3 b, size_t lds, const CoupleArgs& a) {
+#define CHILI_COUPLE_CASE(NPV)                                                        \
+    case NPV:                                                                         \
+        if (inverse) launch_couple_np<true, NPV>(g, b, lds, stream, a);              \
+        else launch_couple_np<false, NPV>(g, b, lds, stream, a);                     \
+        break;
+    switch (npair) {
+        CHILI_COUPLE_CASE(1)
+        CHILI_COUPLE_CASE(2)
+        CHILI_COUPLE_CASE(3)
+        CHILI_COUPLE_CASE(4)
+        CHILI_COUPLE_CASE(5)
+        CHILI_COUPLE_CASE(6)
+        default: throw chili_error(CHILI_ERANGE, "too many transformed dimensions per coupling layer");
+    }
+#undef CHILI_COUPLE_CASE
+}
+
+void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, int64_t ucount, int64_t stride, int64_t rows_bound,
+                            double* x, double* lq) {
+    d_start.resize(nfl);
+    d_count.resize(nfl);
+    d_items.resize(nfl + 1);
+    hipLaunchKernelGGL(k_items, dim3(1), dim3(1), 0, stream, nfl, seg, ucount, stride, kChunk, d_start.ptr, d_count.ptr, d_items.ptr);
+    const unsigned grid = (unsigned)(rows_bound / kChunk + nfl + 1);
+    for (int q = 0; q < L; ++q) {
+        const int l = inverse ? L - 1 - q : q;
+        const Layer& ly = layers[l];
+        const FusedLayout FL = fused_layout(ly.nB);
+        const size_t wbytes = ((FL.wel * 2 + 15) / 16) * 16, bbytes = ((FL.bel + 3) / 4) * 16;
+        const size_t stage = (size_t)32 * kStagePitch * sizeof(float);
+        const int nw = (int)std::min<size_t>(8, (163840 - wbytes - bbytes) / stage);
+        if (nw < 1) throw chili_error(CHILI_ERANGE, "coupling layer does not fit the LDS of one workgroup");
+        const size_t lds = wbytes + bbytes + nw * stage;
+        CoupleArgs a{};
+        a.x = x;
+        a.lq = lq;
+        a.start = d_start.ptr;
+        a.count = d_count.ptr;
+        a.items = d_items.ptr;
+        a.nflows = nfl;
+        a.flow0 = flow0;
+        a.chunk = kChunk;
+        a.w = fusedW.ptr;
+        a.b = fusedB.ptr;
+        a.wstride = fwstride;
+        a.bstride = fbstride;
+        a.L = L;
+        a.layer = l;
+        a.d = d;
+        a.nA = ly.nA;
+        a.nB = ly.nB;
+        for (int i = 0; i < 16; ++i) {
+            a.A[i] = i < ly.nA ? ly.A[i] : 0;
+            a.B[i] = i < ly.nB ? ly.B[i] : 0;
+        }
+        a.shift = shift;
+        launch_couple(inverse, FL.npair, dim3(grid), dim3(64 * nw), lds, a);
     }
 }
 
-void FlowEngine::density_chunked(int f, int64_t n, const double* r, double* lq) {
-    for (int64_t o = 0; o < n; o += kInferChunk) density(f, std::min(kInferChunk, n - o), r + o * d, lq + o, nullptr, false);
+// log q_f(r) of n points (device [n][d]); z (optional) receives the image.
+void FlowEngine::log_density(int f, int64_t n, const double* r, double* lq, double* z) {
+    double* x = z;
+    if (!x) {
+        xwork.resize((size_t)n * d);
+        x = xwork.ptr;
+    }
+    HIP_CHECK(hipMemcpyAsync(x, r, (size_t)n * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(kTh), 0, stream, n, 0.0, lq);
+    fused_pass(false, 1, f, nullptr, n, 0, n, x, lq);
 }
-void FlowEngine::sample_chunked(int f, int64_t n, double* zr, double* lq) {
-    for (int64_t o = 0; o < n; o += kInferChunk) sample(f, std::min(kInferChunk, n - o), zr + o * d, lq + o);
+
+// z -> r in place for n points of flow f; lq = log q_f(r).
+void FlowEngine::sample(int f, int64_t n, double* zr, double* lq) {
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(kTh), 0, stream, n, 0.0, lq);
+    fused_pass(true, 1, f, nullptr, n, 0, n, zr, lq);
 }
 
 double FlowEngine::loss_grad(int64_t nb, const double* r, const double* beta, const double* w2) {
@@ -443,9 +578,13 @@ double FlowEngine::loss_grad(int64_t nb, const double* r, const double* beta, co
     for (int f = 0; f < nflows; ++f) {
         HIP_CHECK(hipMemsetAsync(flows[f].grads.ptr, 0, nparams * sizeof(float), stream));
     }
-    // density passes keep only one workspace, so each flow runs forward then
-    // backward before the next; the loss needs every flow's log-density first
-    for (int f = 0; f < nflows; ++f) density(f, nb, r + (size_t)f * nb * d, lqb.ptr + (size_t)f * nb, nullptr, false);
+    // the loss needs every flow's log-density first (fused pass over all
+    // flows); the backward pass then recomputes one flow at a time, keeping
+    // its activations
+    xwork.resize((size_t)nflows * nb * d);
+    HIP_CHECK(hipMemcpyAsync(xwork.ptr, r, (size_t)nflows * nb * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk((int64_t)nflows * nb)), dim3(kTh), 0, stream, (int64_t)nflows * nb, 0.0, lqb.ptr);
+    fused_pass(false, nflows, 0, nullptr, nb, nb, (int64_t)nflows * nb, xwork.ptr, lqb.ptr);
     hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, dlqb.ptr, lpart.ptr);
     std::vector<double> lp(nbk);
     lpart.download(lp.data(), nbk, stream);
@@ -567,12 +706,7 @@ int chili_flow_log_density(chili_flow* f, int flow, int64_t n, const double* r_d
         if (!f || !r_dev || !lq_dev || flow < 0 || flow >= f->eng.nflows) throw chili_error(CHILI_EINVAL, "bad argument");
         if (n <= 0 || n > (int64_t(1) << 24)) throw chili_error(CHILI_ERANGE, "n out of range");
         select_device(f->eng.device);
-        const int64_t chunk = kInferChunk;
-        const int d = f->eng.d;
-        for (int64_t o = 0; o < n; o += chunk) {
-            const int64_t m = std::min(chunk, n - o);
-            f->eng.density(flow, m, r_dev + o * d, lq_dev + o, z_dev ? z_dev + o * d : nullptr, false);
-        }
+        f->eng.log_density(flow, n, r_dev, lq_dev, z_dev);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(f->eng.stream));
     });
@@ -584,7 +718,7 @@ int chili_flow_sample(chili_flow* f, int flow, int64_t n, double* zr_dev, double
         if (!f || !zr_dev || !lq_dev || flow < 0 || flow >= f->eng.nflows) throw chili_error(CHILI_EINVAL, "bad argument");
         if (n <= 0 || n > (int64_t(1) << 24)) throw chili_error(CHILI_ERANGE, "n out of range");
         select_device(f->eng.device);
-        f->eng.sample_chunked(flow, n, zr_dev, lq_dev);
+        f->eng.sample(flow, n, zr_dev, lq_dev);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(f->eng.stream));
     });

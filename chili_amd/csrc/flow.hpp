@@ -37,8 +37,6 @@ struct Workspace {
     DevBuf<double> gx;
 };
 
-constexpr int64_t kInferChunk = int64_t(1) << 20;   // points per inference pass
-
 struct FlowEngine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -50,6 +48,14 @@ struct FlowEngine {
     Workspace ws;
     DevBuf<float> part, part2;
     DevBuf<double> lqb, dlqb, lpart;
+    // fused inference: swizzled weight blocks [flow][layer] and the work table
+    DevBuf<bf16> fusedW;
+    DevBuf<float> fusedB;
+    int64_t fwstride = 0, fbstride = 0;
+    DevBuf<int64_t> d_start, d_count;
+    DevBuf<int> d_items;
+    DevBuf<double> xwork;
+    static constexpr int64_t kChunk = 4096;   // rows per work item of a fused pass
 
     int ws_lda() const { int m = 0; for (auto& l : layers) m = m > l.ldA ? m : l.ldA; return m; }
     int ws_out() const { int m = 0; for (auto& l : layers) m = m > l.out ? m : l.out; return m; }
@@ -65,9 +71,14 @@ struct FlowEngine {
     void colsum(int64_t n, int m, const float* g, float* db);
     void density(int f, int64_t nb, const double* r, double* lq, double* zout, bool keep);
     void backward(int f, int64_t nb, const double* dlq);
-    void sample(int f, int64_t nb, double* xy, double* lq);
-    void density_chunked(int f, int64_t n, const double* r, double* lq);
-    void sample_chunked(int f, int64_t n, double* zr, double* lq);
+    void log_density(int f, int64_t n, const double* r, double* lq, double* z);
+    void sample(int f, int64_t n, double* zr, double* lq);
+    // Fused passes over nfl flows (flow0 ...): rows of flow j at seg[j] ..
+    // seg[j+1] (seg device ints) or j * stride .. + ucount; x in place, lq
+    // accumulated; rows_bound >= total rows.
+    void launch_couple(bool inverse, int npair, dim3 g, dim3 b, size_t lds, const CoupleArgs& a);
+    void fused_pass(bool inverse, int nfl, int flow0, const int* seg, int64_t ucount, int64_t stride, int64_t rows_bound, double* x,
+                    double* lq);
     double loss_grad(int64_t nb, const double* r, const double* beta, const double* w2);
     void adam_step(double lr);
 };

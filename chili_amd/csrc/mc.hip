@@ -738,7 +738,7 @@ struct chili_mc {
     DevBuf<int64_t> d_total, d_bsum;
     // learned-proposal path: z/r [fmax][d], r_j [C][fmax][d], a_j and log q_j
     // [C][fmax], f [fmax], sampling log-density [fmax], StatsData partials
-    DevBuf<double> d_z, d_rall, d_abuf, d_lq, d_fbuf, d_lqs, d_fpart;
+    DevBuf<double> d_z, d_rall, d_abuf, d_lq, d_fbuf, d_lqs, d_fpart, d_xw;
     int64_t fmax = 0;
     int64_t nmax = 0;            // attempts per sub-batch
     double acc_rate = 0.25;      // running estimate of the cut acceptance
@@ -1119,23 +1119,25 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
     if (fl) {
         // z -> r through each channel's flow, on the channel-sorted segments
         CHILI_DISPATCH_NT(m->NT, launch_zgen<NT>(m, A, n));
-        std::vector<int> seg(C + 1);
-        m->d_seg.download(seg.data(), C + 1, m->stream);
-        HIP_CHECK(hipStreamSynchronize(m->stream));
         chili_nf::StreamSwap sw(fl->eng, m->stream);
-        for (int c = 0; c < C; ++c) {
-            const int64_t cnt = (int64_t)seg[c + 1] - seg[c];
-            if (cnt > 0) fl->eng.sample_chunked(c, cnt, m->d_z.ptr + (size_t)seg[c] * d, m->d_lqs.ptr + seg[c]);
-        }
+        hipLaunchKernelGGL(chili_nf::k_fill<double>, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, n, 0.0,
+                           m->d_lqs.ptr);
+        fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr);
     }
     CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr));
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
     if (kept == 0) return;
     CHILI_DISPATCH_NT(m->NT, launch_geom<NT>(m, A, kept));
     if (fl) {
+        // log q_j(r_j) of every kept point and channel; the flows move a copy
+        // of r_j (the training records keep the originals)
         chili_nf::StreamSwap sw(fl->eng, m->stream);
-        for (int j = 0; j < C; ++j)
-            fl->eng.density_chunked(j, kept, m->d_rall.ptr + (size_t)j * m->fmax * d, m->d_lq.ptr + (size_t)j * m->fmax);
+        const int64_t rows = (int64_t)C * m->fmax;
+        m->d_xw.resize((size_t)rows * d);
+        HIP_CHECK(hipMemcpyAsync(m->d_xw.ptr, m->d_rall.ptr, (size_t)rows * d * sizeof(double), hipMemcpyDeviceToDevice, m->stream));
+        hipLaunchKernelGGL(chili_nf::k_fill<double>, dim3((unsigned)((rows + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, rows, 0.0,
+                           m->d_lq.ptr);
+        fl->eng.fused_pass(false, C, 0, nullptr, kept, m->fmax, (int64_t)C * kept, m->d_xw.ptr, m->d_lq.ptr);
     }
     const int cg = (int)((kept + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_combine, dim3(cg), dim3(kTile), 0, m->stream, C, d, kept, m->fmax, m->d_abuf.ptr, fl ? m->d_lq.ptr : nullptr,

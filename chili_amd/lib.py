@@ -131,7 +131,10 @@ class Channels:
 
     def __del__(self):
         if getattr(self, "h", None):
-            load().chili_channels_free(self.h)
+            try:
+                load().chili_channels_free(self.h)
+            except TypeError:       # interpreter shutdown: module globals already cleared
+                pass
             self.h = None
 
 
@@ -149,7 +152,10 @@ class MultiChannelGPU:
 
     def __del__(self):
         if getattr(self, "h", None):
-            load().chili_mc_destroy(self.h)
+            try:
+                load().chili_mc_destroy(self.h)
+            except TypeError:       # interpreter shutdown: module globals already cleared
+                pass
             self.h = None
 
     def set_grids(self, grids):
@@ -243,7 +249,10 @@ class FlowGPU:
 
     def __del__(self):
         if getattr(self, "h", None):
-            load().chili_flow_destroy(self.h)
+            try:
+                load().chili_flow_destroy(self.h)
+            except TypeError:       # interpreter shutdown: module globals already cleared
+                pass
             self.h = None
 
     def masks(self):
@@ -308,7 +317,10 @@ class VegasGPU:
 
     def __del__(self):
         if getattr(self, "h", None):
-            load().chili_vegas_destroy(self.h)
+            try:
+                load().chili_vegas_destroy(self.h)
+            except TypeError:       # interpreter shutdown: module globals already cleared
+                pass
             self.h = None
 
     def set_grid(self, edges):

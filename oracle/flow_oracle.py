@@ -15,8 +15,9 @@ Conventions restated here (the product computes the same):
     K width logits, K height logits, K-1 interior derivative inputs;
   * conditioner: xa = x[A]; h = relu(xa W1 + b1), two more ReLU layers,
     raw = h W4 + b4;
-  * knots: FP32 softmax; w_i = minW + (1-K minW) sm_i (FP64), X_0 = 0,
-    X_{i+1} = X_i + w_i, X_K = 1; derivatives d_0 = d_K = 1,
+  * knots (FP32): sm = softmax, w_i = minW + (1-K minW) sm_i, X_0 = 0,
+    X_{i+1} = X_i + w_i (sequential FP32 sums), X_K = 1, bin search in FP32;
+    transform and log-Jacobian in FP64 from those knots; derivatives d_0 = d_K = 1,
     d_i = minD + softplus_f32(u_i + shift), shift = log(expm1(1 - minD)) so
     that the zero network is the identity;
   * log q(r) = sum over layers and transformed dims of log dy/dx.
@@ -107,18 +108,33 @@ class FlowOracle:
 
     # -- spline --------------------------------------------------------------
     def knots(self, u):
-        """u: [..., 3K-1] -> X, Y [..., K+1], d [..., K+1] (FP64)."""
+        """u: [..., 3K-1] -> X, Y [..., K+1], d [..., K+1] (FP64 tensors).
+        gpu mode: knots are FP32 sums (sequential, as the device adds them)."""
         K = self.K
         if self.mode == "gpu":
             u32 = u.to(torch.float32)
-            sw = torch.softmax(u32[..., :K], dim=-1).to(torch.float64)
-            sh = torch.softmax(u32[..., K:2 * K], dim=-1).to(torch.float64)
+            ew = torch.exp(u32[..., :K] - u32[..., :K].max(-1, keepdim=True).values.detach())
+            eh = torch.exp(u32[..., K:2 * K] - u32[..., K:2 * K].max(-1, keepdim=True).values.detach())
+            sw = ew * (1.0 / ew.sum(-1, keepdim=True))
+            sh = eh * (1.0 / eh.sum(-1, keepdim=True))
+            cw = torch.tensor(1.0 - K * MIN_W, dtype=torch.float32)
+            w = torch.tensor(MIN_W, dtype=torch.float32) + cw * sw
+            h = torch.tensor(MIN_H, dtype=torch.float32) + cw * sh
+            X, Y = [torch.zeros_like(w[..., 0])], [torch.zeros_like(h[..., 0])]
+            for i in range(K - 1):
+                X.append(X[-1] + w[..., i])
+                Y.append(Y[-1] + h[..., i])
+            X.append(torch.ones_like(w[..., 0]))
+            Y.append(torch.ones_like(h[..., 0]))
+            X = torch.stack(X, -1).to(torch.float64)
+            Y = torch.stack(Y, -1).to(torch.float64)
             dz = torch.nn.functional.softplus(u32[..., 2 * K:] + SHIFT32).to(torch.float64)
-        else:
-            shift = math.log(math.expm1(1.0 - MIN_D))
-            sw = torch.softmax(u[..., :K], dim=-1)
-            sh = torch.softmax(u[..., K:2 * K], dim=-1)
-            dz = torch.nn.functional.softplus(u[..., 2 * K:] + shift)
+            one = torch.ones_like(X[..., :1])
+            return X, Y, torch.cat([one, MIN_D + dz, one], -1)
+        shift = math.log(math.expm1(1.0 - MIN_D))
+        sw = torch.softmax(u[..., :K], dim=-1)
+        sh = torch.softmax(u[..., K:2 * K], dim=-1)
+        dz = torch.nn.functional.softplus(u[..., 2 * K:] + shift)
         w = MIN_W + (1 - K * MIN_W) * sw
         h = MIN_H + (1 - K * MIN_H) * sh
         zero = torch.zeros_like(w[..., :1])
@@ -135,7 +151,11 @@ class FlowOracle:
         X, Y, dd = self.knots(u)
         v = x.clamp(0.0, 1.0)
         edges = Y if inverse else X
-        k = (v.detach().unsqueeze(-1) >= edges[..., 1:K].detach()).sum(-1)
+        if self.mode == "gpu":      # the device compares in FP32
+            vs = v.detach().to(torch.float32).to(torch.float64)
+        else:
+            vs = v.detach()
+        k = (vs.unsqueeze(-1) >= edges[..., 1:K].detach()).sum(-1)
         k = k.unsqueeze(-1)
         X0 = X.gather(-1, k).squeeze(-1)
         Y0 = Y.gather(-1, k).squeeze(-1)
