@@ -138,6 +138,7 @@ typedef struct chili_uniforms {
 typedef struct chili_channels chili_channels;
 typedef struct chili_mc chili_mc;
 typedef struct chili_vegas chili_vegas;
+typedef struct chili_flow chili_flow;
 
 const char* chili_last_error(void);
 int chili_device_count(void);
@@ -151,6 +152,95 @@ int  chili_channels_count(const chili_channels* ch);
 int  chili_channels_ndims(const chili_channels* ch);
 int  chili_channels_get(const chili_channels* ch, int i, chili_channel_desc* out);
 void chili_channels_free(chili_channels* ch);
+
+/* ---- multichannel context: MultiChannel + Integrand<FourVector> of FSMapper
+ *      channels (include/Channel/MultiChannel.hh:51-139, Integrand.hh:9-133) ---- */
+/* nbins: VEGAS bins per dimension, 2..256 (AdaptiveMap(dims, bins),
+ * include/Integrator/AdaptiveMap.hh:29-40); grids start uniform, alpha = 1/C
+ * (MultiChannel ctor, src/Channel/MultiChannel.cc:62-67). */
+int chili_mc_create(const chili_channels* ch, int nbins, const chili_integrand* f, int device, chili_mc** out);
+int chili_mc_destroy(chili_mc* m);
+int chili_mc_info(const chili_mc* m, int* nchannels, int* ndims, int* nbins, int* ntot);
+/* grids [C][d][nbins+1] (edges, non-decreasing; AdaptiveMap storage
+ * AdaptiveMap.hh:71-72), alpha [C] (MultiChannel::channel_weights). */
+int chili_mc_set_grids(chili_mc* m, const double* grids, int nbins);
+int chili_mc_get_grids(const chili_mc* m, double* grids);
+int chili_mc_set_alpha(chili_mc* m, const double* alpha);
+int chili_mc_get_alpha(const chili_mc* m, double* alpha);
+/* One iteration of MultiChannel::operator() (MultiChannel.hh:197-251): attempts
+ * from a0 until ncalls pass the cut with a finite value; rejected attempts add
+ * a zero to stats.  train != 0: T [C] (alpha training sums, MultiChannel.hh:
+ * 168-180) and hist [C][d][nbins] (picked channel's f^2 histograms,
+ * Integrand.hh:89-96) are returned for MultiChannel::Train (.hh:182-194).
+ * vals_dev (device, optional): value of every kept point, at most vals_cap. */
+int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, int train, chili_stats* stats,
+                     double* T, double* hist, int64_t* n_attempts, double* vals_dev, int64_t vals_cap);
+/* Per-attempt records of attempts [a0, a0+n) (no retry loop): the batch FFI
+ * view (include/Interfaces/nanobind.hh:23-50) of MultiChannel::GeneratePoint
+ * / GenerateWeight (MultiChannel.hh:141-165).  Host outputs, each optional:
+ * status[n] (0 cut, 1 accepted, 2 NaN), pick[n], w[n], f[n], val[n],
+ * ps[C][n], x[d][n] (picked channel's inverse point), bins[d][n], mom[ntot*4][n]. */
+int chili_mc_points(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n, int32_t* status, int32_t* pick, double* w,
+                    double* f, double* val, double* ps, double* x, int32_t* bins, double* mom);
+/* Kernel timing on the context's stream (HIP events around the weight and
+ * forward kernels); no reference counterpart (measurement only). */
+int chili_mc_timing(chili_mc* m, int enable, int reset, double* main_ms, int64_t* launches, int64_t* points, double* fwd_ms);
+
+/* ---- plain VEGAS (chili::Vegas, include/Integrator/Vegas.hh:55-92,
+ *      src/Integrator/Vegas.cc:18-69); device integrand CHILI_F_EEMUMU ---- */
+int chili_vegas_create(int ndims, int nbins, const chili_integrand* f, int device, chili_vegas** out);
+int chili_vegas_destroy(chili_vegas* v);
+int chili_vegas_set_grid(chili_vegas* v, const double* edges, int nbins);
+int chili_vegas_get_grid(const chili_vegas* v, double* edges, int* nbins);
+/* One Vegas::operator() pass without its final Adapt (Vegas.cc:18-52):
+ * stats and the f^2 histogram hist [d][nbins] for AdaptiveMap::Adapt. */
+int chili_vegas_iterate(chili_vegas* v, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, chili_stats* stats, double* hist,
+                        double* vals_dev, int64_t vals_cap);
+
+/* ---- host-side adaptation (small, serial; the reference's own steps) ---- */
+/* AdaptiveMap::Adapt (src/Integrator/AdaptiveMap.cc:67-125), in place on edges [dims][bins+1]. */
+int    chili_grid_adapt(int dims, int bins, double* edges, double alpha, const double* data);
+/* AdaptiveMap::Split (AdaptiveMap.cc:127-160): out [dims][nsplit*bins+1]. */
+int    chili_grid_split(int dims, int bins, const double* edges, int nsplit, double* out);
+/* MultiChannel::Adapt (src/Channel/MultiChannel.cc:109-136), in place on alpha [n]. */
+int    chili_alpha_adapt(int n, double* alpha, const double* train, double beta, double min_alpha);
+/* MultiChannel::MaxDifference (src/Channel/MultiChannel.cc:138-154). */
+double chili_max_difference(int n, const double* train);
+
+/* ---- learned proposal: per-channel coupling-layer spline flows on the
+ *      VEGAS hypercube.  No counterpart in the reference; the sampler it
+ *      feeds replaces MultiChannel::GeneratePoint's uniforms
+ *      (MultiChannel.hh:141-150) and its density joins the channel
+ *      densities of Integrand::GenerateWeight (Integrand.hh:112-123). ---- */
+/* nflows flows of ndims dims, nlayers coupling layers, hidden == 64; the
+ * last layer starts at zero (every flow is the identity). */
+int chili_flow_create(int ndims, int nlayers, int hidden, int nflows, uint64_t seed, int device, chili_flow** out);
+int chili_flow_destroy(chili_flow* f);
+int chili_flow_info(const chili_flow* f, int64_t* nparams, int* ndims, int* nlayers, int* nflows);
+int chili_flow_mask(const chili_flow* f, int layer, int32_t* B, int* nB);
+int chili_flow_get_params(chili_flow* f, int flow, float* out);
+int chili_flow_set_params(chili_flow* f, int flow, const float* in);
+int chili_flow_get_grads(chili_flow* f, int flow, float* out);
+/* device pointers: r, z [n][ndims], lq [n] = log q(r) */
+int chili_flow_log_density(chili_flow* f, int flow, int64_t n, const double* r_dev, double* lq_dev, double* z_dev);
+int chili_flow_sample(chili_flow* f, int flow, int64_t n, double* zr_dev, double* lq_dev);
+/* weighted-variance loss mean(w2 / sum_j beta_j q_j(r_j)) and its gradients
+ * (device: r [C][n][d], beta [C][n], w2 [n]) */
+int chili_flow_loss_grad(chili_flow* f, int64_t n, const double* r_dev, const double* beta_dev, const double* w2_dev, double* loss);
+int chili_flow_train(chili_flow* f, int64_t n, const double* r_dev, const double* beta_dev, const double* w2_dev, int64_t batch,
+                     int epochs, double lr, uint64_t seed, double* loss_hist);
+/* multichannel iteration with the flows' proposals (fl may be NULL: q = 1) */
+int chili_mc_flow_iterate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, chili_stats* stats,
+                          int64_t* n_attempts, double* vals_dev, int64_t vals_cap);
+/* training records drawn from the current proposal (device outputs) */
+int chili_mc_flow_data(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, double* r_dev,
+                       double* beta_dev, double* w2_dev, int64_t cap, chili_stats* stats, int64_t* n_kept, int64_t* n_attempts);
+/* per-attempt records; lq [C][n], r [d][n] host outputs */
+int chili_mc_flow_points(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t n, int32_t* status,
+                         int32_t* pick, double* w, double* f, double* val, double* lq, double* r);
+/* MFMA GEMM probe (test of the BF16 fragment layout; host buffers):
+ * C [M][N] = A B^T (A [M][K], B [N][K]) or A^T B (A [K][M], B [K][N]). */
+int chili_gemm_bf16_test(int M, int N, int K, const uint16_t* A, const uint16_t* B, float* C, int transA, int transB);
 
 #ifdef __cplusplus
 }
