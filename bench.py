@@ -99,6 +99,18 @@ def adapt(gpu, ora_lib, W, uni_seed):
                 time_to_1pct_s=t_1pct, rel_err_history=[float(x) for x in hist_rel])
 
 
+def measured_traffic(workload, pts_launch):
+    """HBM bytes per k_main launch at this launch size, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), scaled by the
+    counters' reading of a copy kernel of known size; None if absent."""
+    path = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    return round(t["bytes_per_point"] * pts_launch, 1)
+
+
 def _cpu_worker(job):
     """One host core: the oracle's MultiChannel loop over a share of the sample."""
     proc_name, grids, alpha, seed, stream, ncalls = job
@@ -156,6 +168,9 @@ def main():
     ap.add_argument("--flow-epochs", type=int, default=30, help="training epochs (the full schedule is 200)")
     ap.add_argument("--flow-batch", type=int, default=1 << 16)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: adapt, run two production steps and a 1 GiB device copy (the "
+                         "calibration kernel of known traffic), print nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (each GPU would run independent "
@@ -176,6 +191,15 @@ def main():
     log(f"workload {args.workload}: {gpu.nchannels} channels, {gpu.ndims} dims, {gpu.nbins} bins")
     seed = SEED + W["cfg"]
     ad = adapt(gpu, None, W, seed)
+    if args.traffic_probe:
+        for s in range(2):
+            gpu.iterate(abi.philox_uniforms(seed, stream=2000 + s), W["batch"], train=False)
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"traffic probe done: {gpu.nchannels} channels, copy of {a.numel() * 8} bytes")
+        return
 
     # ---- production with frozen state ----
     n = W["batch"]
@@ -266,7 +290,7 @@ def main():
     pts_launch = tm["points"] / max(tm["launches"], 1)
     achieved = f_main * pts_launch / (avg_ms * 1e-3) / 1e12
     roof = dict(bound="mfma", achieved=round(achieved, 4), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / FP64_PEAK_TFLOPS, 5), traffic=None,
+                frac=round(achieved / FP64_PEAK_TFLOPS, 5), traffic=measured_traffic(args.workload, pts_launch),
                 kernel="k_main (all-channel inverse maps + VEGAS densities + multichannel weight)",
                 compute="FP64 VALU; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s",
                 flop_eq_per_point=round(f_main, 1), points_per_launch=round(pts_launch, 1),

@@ -43,53 +43,49 @@ __global__ void k_gather_bf16(int64_t n, int d, const double* __restrict__ x, co
     for (int c = 0; c < ld; ++c) xa[i * ld + c] = f2bf(c < nA ? (float)x[i * d + A[c]] : 0.f);
 }
 
-// Forward (density direction) spline over the B dims of one layer.
+// Forward (density direction) spline of one layer, one thread per (point,
+// transformed dim); A dims are copied by the b == 0 thread, log-Jacobians go
+// to ldt[i][b] (summed per point in fixed order by k_add_rows).
 template <int K>
 __global__ void k_spline_fwd(int64_t n, int d, const double* __restrict__ x, const float* __restrict__ raw, int ldr,
-                             const int* __restrict__ Bd, int nB, float shift, double* __restrict__ y, double* __restrict__ lq) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (int c = 0; c < d; ++c) y[i * d + c] = x[i * d + c];
-    double ld = 0;
-    for (int b = 0; b < nB; ++b) {
-        double l;
-        const int c = Bd[b];
-        y[i * d + c] = rqs_forward<K>(raw + i * ldr + b * (3 * K - 1), shift, x[i * d + c], l);
-        ld += l;
-    }
-    lq[i] += ld;
+                             const int* __restrict__ Bd, int nB, const int* __restrict__ Ad, int nA, float shift,
+                             double* __restrict__ y, double* __restrict__ ldt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nB) return;
+    const int64_t i = t / nB;
+    const int b = (int)(t - i * nB);
+    if (b == 0)
+        for (int c = 0; c < nA; ++c) y[i * d + Ad[c]] = x[i * d + Ad[c]];
+    double l;
+    const int c = Bd[b];
+    y[i * d + c] = rqs_forward<K>(raw + i * ldr + b * (3 * K - 1), shift, x[i * d + c], l);
+    ldt[t] = l;
 }
 
-// Sampling-direction spline: x[B] = inverse(y[B]); lq accumulates log dy/dx.
-template <int K>
-__global__ void k_spline_inv(int64_t n, int d, double* __restrict__ xy, const float* __restrict__ raw, int ldr,
-                             const int* __restrict__ Bd, int nB, float shift, double* __restrict__ lq) {
+// lq[i] += sum_b ldt[i][b] (b in order)
+__global__ void k_add_rows(int64_t n, int nB, const double* __restrict__ ldt, double* __restrict__ lq) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double ld = 0;
-    for (int b = 0; b < nB; ++b) {
-        double l;
-        const int c = Bd[b];
-        xy[i * d + c] = rqs_inverse<K>(raw + i * ldr + b * (3 * K - 1), shift, xy[i * d + c], l);
-        ld += l;
-    }
-    lq[i] += ld;
+    double s = 0;
+    for (int b = 0; b < nB; ++b) s += ldt[i * nB + b];
+    lq[i] += s;
 }
 
-// Spline backward: gx holds dL/dy on entry and dL/dx (B dims) on exit; draw
-// receives dL/draw (bf16 for the next GEMM and fp32 for the bias sums).
+// Spline backward, one thread per (point, transformed dim): gx holds dL/dy on
+// entry and dL/dx on exit for the B dims; draw / drawb receive dL/draw (fp32
+// for the bias sums, bf16 for the next product).
 template <int K>
 __global__ void k_spline_bwd(int64_t n, int d, const double* __restrict__ x, const float* __restrict__ raw, int ldr,
                              const int* __restrict__ Bd, int nB, float shift, const double* __restrict__ glq,
                              double* __restrict__ gx, float* __restrict__ draw, bf16* __restrict__ drawb) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (int b = 0; b < nB; ++b) {
-        const int c = Bd[b];
-        float* du = draw + i * ldr + b * (3 * K - 1);
-        gx[i * d + c] = rqs_backward<K>(raw + i * ldr + b * (3 * K - 1), shift, x[i * d + c], gx[i * d + c], glq[i], du);
-        for (int q = 0; q < 3 * K - 1; ++q) drawb[i * ldr + b * (3 * K - 1) + q] = f2bf(du[q]);
-    }
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nB) return;
+    const int64_t i = t / nB;
+    const int b = (int)(t - i * nB);
+    const int c = Bd[b];
+    float* du = draw + i * ldr + b * (3 * K - 1);
+    gx[i * d + c] = rqs_backward<K>(raw + i * ldr + b * (3 * K - 1), shift, x[i * d + c], gx[i * d + c], glq[i], du);
+    for (int q = 0; q < 3 * K - 1; ++q) drawb[i * ldr + b * (3 * K - 1) + q] = f2bf(du[q]);
 }
 
 // gh (bf16) = relu'(h) * g (fp32)
@@ -109,15 +105,20 @@ __global__ void k_scatter_add(int64_t n, int d, const float* __restrict__ dxa, i
     for (int c = 0; c < nA; ++c) gx[i * d + A[c]] += (double)dxa[i * ld + c];
 }
 
-// Column sums of g [n x m] in fixed order: block z sums rows
-// [z chunk, (z+1) chunk) of every column (coalesced over columns).
-__global__ void k_colsum_part(int64_t n, int m, const float* __restrict__ g, int64_t chunk, float* __restrict__ part) {
+// Column sums of g [n x m] in fixed order: block (z, cb) sums rows
+// [z chunk, (z+1) chunk) of columns cb*64 .. cb*64+63, four row phases per
+// column combined in order.
+__global__ __launch_bounds__(256) void k_colsum_part(int64_t n, int m, const float* __restrict__ g, int64_t chunk,
+                                                     float* __restrict__ part) {
+    __shared__ float s[4][64];
+    const int c = blockIdx.y * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * chunk, r1 = std::min<int64_t>(n, r0 + chunk);
-    for (int c = threadIdx.x; c < m; c += blockDim.x) {
-        float s = 0;
-        for (int64_t r = r0; r < r1; ++r) s += g[r * m + c];
-        part[(int64_t)blockIdx.x * m + c] = s;
-    }
+    float acc = 0;
+    if (c < m)
+        for (int64_t r = r0 + ph; r < r1; r += 4) acc += g[r * m + c];
+    s[ph][threadIdx.x & 63] = acc;
+    __syncthreads();
+    if (ph == 0 && c < m) part[(int64_t)blockIdx.x * m + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) + s[2][threadIdx.x]) + s[3][threadIdx.x];
 }
 
 // dst[k] += sum_z part[z][k], z in order
@@ -388,7 +389,7 @@ void FlowEngine::gemm_fwd(int64_t M, int N, int K, const bf16* A, int lda, const
 }
 // dW [in x out] += A^T G with A [n x in] (bf16), G [n x out] (bf16), split over n
 void FlowEngine::gemm_wgrad(int64_t n, int in, int out, const bf16* A, int lda, const bf16* G, int ldg, float* dW) {
-    const int chunk = 256;     // points per split-K partial: enough workgroups to fill the chip
+    const int chunk = 512;     // points per split-K partial (a multiple of the 32-point K step)
     const int nz = (int)((n + chunk - 1) / chunk);
     part.resize((size_t)nz * in * out);
     dim3 g((out + kGT - 1) / kGT, (in + kGT - 1) / kGT, nz);
@@ -400,7 +401,7 @@ void FlowEngine::colsum(int64_t n, int m, const float* g, float* db) {
     const int64_t chunk = 256;
     const int nz = (int)((n + chunk - 1) / chunk);
     part2.resize((size_t)nz * m);
-    hipLaunchKernelGGL(k_colsum_part, dim3(nz), dim3(256), 0, stream, n, m, g, chunk, part2.ptr);
+    hipLaunchKernelGGL(k_colsum_part, dim3(nz, (m + 63) / 64), dim3(256), 0, stream, n, m, g, chunk, part2.ptr);
     hipLaunchKernelGGL(k_sum_parts, dim3(nblk(m)), dim3(kTh), 0, stream, (int64_t)m, nz, part2.ptr, db);
 }
 
@@ -421,8 +422,10 @@ void FlowEngine::density(int f, int64_t nb, const double* r, double* lq, double*
         bf16* h = ws.h.ptr + sl * 3 * nb * H;
         float* raw = ws.raw.ptr + sl * nb * ldo;
         mlp_forward(F, ly, nb, xl, xa, h, raw);
-        hipLaunchKernelGGL(k_spline_fwd<kSplineBins>, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, d, xl, raw, ly.out, ly.dB.ptr, ly.nB,
-                           shift, xn, lq);
+        ws.ldt.resize((size_t)nb * ly.nB);
+        hipLaunchKernelGGL(k_spline_fwd<kSplineBins>, dim3(nblk(nb * ly.nB)), dim3(kTh), 0, stream, nb, d, xl, raw, ly.out, ly.dB.ptr,
+                           ly.nB, ly.dA.ptr, ly.nA, shift, xn, ws.ldt.ptr);
+        hipLaunchKernelGGL(k_add_rows, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, ly.nB, ws.ldt.ptr, lq);
     }
     const double* zl = ws.x.ptr + (keep ? (size_t)L : (size_t)(L & 1)) * nb * d;
     if (zout) HIP_CHECK(hipMemcpyAsync(zout, zl, (size_t)nb * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
@@ -448,7 +451,7 @@ void FlowEngine::backward(int f, int64_t nb, const double* dlq) {
         bf16* xa = ws.xa.ptr + (size_t)l * nb * ws_lda();
         bf16* h = ws.h.ptr + (size_t)l * 3 * nb * H;
         float* raw = ws.raw.ptr + (size_t)l * nb * ws_out();
-        hipLaunchKernelGGL(k_spline_bwd<kSplineBins>, dim3(nblk(nb)), dim3(kTh), 0, stream, nb, d, xl, raw, ly.out, ly.dB.ptr, ly.nB,
+        hipLaunchKernelGGL(k_spline_bwd<kSplineBins>, dim3(nblk(nb * ly.nB)), dim3(kTh), 0, stream, nb, d, xl, raw, ly.out, ly.dB.ptr, ly.nB,
                            shift, dlq, ws.gx.ptr, ws.draw.ptr, ws.drawb.ptr);
         // last linear layer
         gemm_wgrad(nb, H, ly.out, h + 2 * (size_t)nb * H, H, ws.drawb.ptr, ly.out, F.grads.ptr + ly.oW4);
@@ -513,8 +516,12 @@ void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, in
     d_start.resize(nfl);
     d_count.resize(nfl);
     d_items.resize(nfl + 1);
-    hipLaunchKernelGGL(k_items, dim3(1), dim3(1), 0, stream, nfl, seg, ucount, stride, kChunk, d_start.ptr, d_count.ptr, d_items.ptr);
-    const unsigned grid = (unsigned)(rows_bound / kChunk + nfl + 1);
+    // rows per work item: large enough to amortise the workgroup's weight
+    // load, small enough to give every CU work on small passes
+    int64_t chunk = ((rows_bound / 512 + 255) / 256) * 256;
+    chunk = std::max<int64_t>(256, std::min<int64_t>(kChunk, chunk));
+    hipLaunchKernelGGL(k_items, dim3(1), dim3(1), 0, stream, nfl, seg, ucount, stride, chunk, d_start.ptr, d_count.ptr, d_items.ptr);
+    const unsigned grid = (unsigned)(rows_bound / chunk + nfl + 1);
     for (int q = 0; q < L; ++q) {
         const int l = inverse ? L - 1 - q : q;
         const Layer& ly = layers[l];
@@ -532,7 +539,7 @@ void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, in
         a.items = d_items.ptr;
         a.nflows = nfl;
         a.flow0 = flow0;
-        a.chunk = kChunk;
+        a.chunk = chunk;
         a.w = fusedW.ptr;
         a.b = fusedB.ptr;
         a.wstride = fwstride;

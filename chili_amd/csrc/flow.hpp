@@ -35,6 +35,7 @@ struct Workspace {
     DevBuf<float> draw, gf, dxa;
     DevBuf<bf16> drawb, gb;
     DevBuf<double> gx;
+    DevBuf<double> ldt;            // per (point, transformed dim) log-Jacobians
 };
 
 struct FlowEngine {
@@ -55,7 +56,7 @@ struct FlowEngine {
     DevBuf<int64_t> d_start, d_count;
     DevBuf<int> d_items;
     DevBuf<double> xwork;
-    static constexpr int64_t kChunk = 4096;   // rows per work item of a fused pass
+    static constexpr int64_t kChunk = 4096;   // largest rows per work item of a fused pass
 
     int ws_lda() const { int m = 0; for (auto& l : layers) m = m > l.ldA ? m : l.ldA; return m; }
     int ws_out() const { int m = 0; for (auto& l : layers) m = m > l.out ? m : l.out; return m; }

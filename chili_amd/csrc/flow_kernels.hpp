@@ -47,26 +47,58 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(int M, int N, int K, const b
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int kbeg = blockIdx.z * kchunk;
     const int kend = min(K, kbeg + kchunk);
+    // 16-byte loads need 8-element aligned rows and base pointers
+    const bool vecA = (lda % 8 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+    const bool vecB = (ldb % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
     for (int k0 = kbeg; k0 < kend; k0 += kGK) {
-        // stage A tile [64 m][32 k] and B tile [64 n][32 k] (zero outside)
-        for (int i = tid; i < kGT * kGK; i += kGThreads) {
-            int r, c;
-            if (!A_T) { r = i / kGK; c = i % kGK; } else { c = i / kGT; r = i % kGT; }
-            const int m = m0 + r, k = k0 + c;
-            bf16 v = f2bf(0.f);
-            if (m < M && k < kend) v = A_T ? A[(int64_t)k * lda + m] : A[(int64_t)m * lda + k];
-            sA[r][c] = v;
+        // stage A tile [64 m][32 k] and B tile [64 n][32 k] (zero outside);
+        // the k-major layouts (A_T, !B_T) load 8 consecutive m / n at once
+        if (A_T && vecA) {
+            const int r8 = (tid & 7) * 8, c = tid >> 3;        // 32 k rows x 8 groups of 8 m
+            const int m = m0 + r8, k = k0 + c;
+            if (k < kend && m + 8 <= M) {
+                const uint4 q = *reinterpret_cast<const uint4*>(A + (int64_t)k * lda + m);
+                const bf16* e = reinterpret_cast<const bf16*>(&q);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sA[r8 + j][c] = e[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sA[r8 + j][c] = (k < kend && m + j < M) ? A[(int64_t)k * lda + m + j] : f2bf(0.f);
+            }
+        } else {
+            for (int i = tid; i < kGT * kGK; i += kGThreads) {
+                int r, c;
+                if (!A_T) { r = i / kGK; c = i % kGK; } else { c = i / kGT; r = i % kGT; }
+                const int m = m0 + r, k = k0 + c;
+                bf16 v = f2bf(0.f);
+                if (m < M && k < kend) v = A_T ? A[(int64_t)k * lda + m] : A[(int64_t)m * lda + k];
+                sA[r][c] = v;
+            }
         }
-        for (int i = tid; i < kGT * kGK; i += kGThreads) {
-            int r, c;
-            if (B_T) { r = i / kGK; c = i % kGK; } else { c = i / kGT; r = i % kGT; }
-            const int n = n0 + r, k = k0 + c;
-            bf16 v = f2bf(0.f);
-            if (n < N && k < kend) v = B_T ? B[(int64_t)n * ldb + k] : B[(int64_t)k * ldb + n];
-            sB[r][c] = v;
+        if (!B_T && vecB) {
+            const int r8 = (tid & 7) * 8, c = tid >> 3;
+            const int n = n0 + r8, k = k0 + c;
+            if (k < kend && n + 8 <= N) {
+                const uint4 q = *reinterpret_cast<const uint4*>(B + (int64_t)k * ldb + n);
+                const bf16* e = reinterpret_cast<const bf16*>(&q);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sB[r8 + j][c] = e[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sB[r8 + j][c] = (k < kend && n + j < N) ? B[(int64_t)k * ldb + n + j] : f2bf(0.f);
+            }
+        } else {
+            for (int i = tid; i < kGT * kGK; i += kGThreads) {
+                int r, c;
+                if (B_T) { r = i / kGK; c = i % kGK; } else { c = i / kGT; r = i % kGT; }
+                const int n = n0 + r, k = k0 + c;
+                bf16 v = f2bf(0.f);
+                if (n < N && k < kend) v = B_T ? B[(int64_t)n * ldb + k] : B[(int64_t)k * ldb + n];
+                sB[r][c] = v;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -109,9 +141,7 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(int M, int N, int K, const b
 //   * transform and log-Jacobian in FP64 from those knots (exact in FP64).
 // The map is a monotone bijection of [0,1] whatever the FP32 rounding of the
 // knots, and its log-derivative is evaluated from the same knots, so the
-// density stays exactly normalised.  The bin is found in FP32; a point
-// within FP32 rounding of a knot may use the neighbouring piece, which
-// agrees with the other to first order there (the spline is C1).
+// density stays exactly normalised.
 // ---------------------------------------------------------------------------
 constexpr double kMinW = 1e-3, kMinH = 1e-3, kMinD = 1e-3;
 
@@ -153,14 +183,15 @@ __device__ __forceinline__ SplineBin spline_bin(const float* __restrict__ u, flo
     rw = 1.f / sw;
     rh = 1.f / sh;
     const float cw = (float)(1.0 - K * kMinW), ch = (float)(1.0 - K * kMinH);
-    const float vf = (float)v;
     float X = 0.f, Y = 0.f, X0 = 0.f, Y0 = 0.f, X1 = 1.f, Y1 = 1.f;
     int k = 0;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         const float Xn = (i == K - 1) ? 1.f : X + ((float)kMinW + cw * (ew[i] * rw));
         const float Yn = (i == K - 1) ? 1.f : Y + ((float)kMinH + ch * (eh[i] * rh));
-        const bool in = (i == 0) || (by_y ? (vf >= Y) : (vf >= X));
+        // compared in FP64 (the knots are exact there), so that the bin of v
+        // agrees with the FP64 transform: forward and inverse round-trip
+        const bool in = (i == 0) || (by_y ? (v >= (double)Y) : (v >= (double)X));
         k = in ? i : k;                  // selects, not branches
         X0 = in ? X : X0;
         Y0 = in ? Y : Y0;

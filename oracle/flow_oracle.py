@@ -16,7 +16,7 @@ Conventions restated here (the product computes the same):
   * conditioner: xa = x[A]; h = relu(xa W1 + b1), two more ReLU layers,
     raw = h W4 + b4;
   * knots (FP32): sm = softmax, w_i = minW + (1-K minW) sm_i, X_0 = 0,
-    X_{i+1} = X_i + w_i (sequential FP32 sums), X_K = 1, bin search in FP32;
+    X_{i+1} = X_i + w_i (sequential FP32 sums), X_K = 1, bin search in FP64;
     transform and log-Jacobian in FP64 from those knots; derivatives d_0 = d_K = 1,
     d_i = minD + softplus_f32(u_i + shift), shift = log(expm1(1 - minD)) so
     that the zero network is the identity;
@@ -151,11 +151,7 @@ class FlowOracle:
         X, Y, dd = self.knots(u)
         v = x.clamp(0.0, 1.0)
         edges = Y if inverse else X
-        if self.mode == "gpu":      # the device compares in FP32
-            vs = v.detach().to(torch.float32).to(torch.float64)
-        else:
-            vs = v.detach()
-        k = (vs.unsqueeze(-1) >= edges[..., 1:K].detach()).sum(-1)
+        k = (v.detach().unsqueeze(-1) >= edges[..., 1:K].detach()).sum(-1)
         k = k.unsqueeze(-1)
         X0 = X.gather(-1, k).squeeze(-1)
         Y0 = Y.gather(-1, k).squeeze(-1)
