@@ -173,16 +173,15 @@ __device__ __forceinline__ bool locate_tile(int C, const int* __restrict__ seg_s
 // k_forward: VEGAS forward map, FSMapper::GeneratePoint and the cut for one
 // channel per workgroup.  Momenta of external legs go to mom[(i*4+k)*nmax + e].
 // ---------------------------------------------------------------------------
+// One attempt: uniforms (or the proposal's hypercube point), VEGAS forward
+// map, FSMapper::GeneratePoint and the cut.  e: sorted position (momentum
+// column), a: attempt slot of acc / per-attempt records, att: the attempt's
+// uniform counter.
 template <int NT, bool ZS>
-__global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restrict__ seg_start, const int* __restrict__ tile_start,
-                                                   const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,
-                                                   uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {
+__device__ __forceinline__ void forward_one(const MCArgs& A, int ch, int e, uint32_t a, uint64_t att, double* __restrict__ mom,
+                                            int64_t nmax, uint8_t* __restrict__ acc, const PointRec& R,
+                                            const double* __restrict__ zsrc) {
     constexpr int D = 3 * (NT - 2) - 2;
-    int ch, e0, e1;
-    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;
-    const int e = e0 + threadIdx.x;
-    if (e >= e1) return;
-    const uint32_t a = sorted[e];
     double r[D + 1];
     if (ZS) {
         // hypercube point drawn by the channel's learned proposal ([e][D])
@@ -192,7 +191,7 @@ __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restri
 #pragma unroll
             for (int i = 0; i < D; ++i) R.x[(int64_t)i * R.n + a] = r[i];
     } else {
-        load_uniforms<D>(A, A.a0 + a, r);
+        load_uniforms<D>(A, att, r);
     }
     // VEGAS forward map on the picked channel's grid (AdaptiveMap.cc:40-56);
     // the multichannel path drops its Jacobian (Integrand.hh:107-110).
@@ -234,6 +233,19 @@ __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restri
             mom[(int64_t)(i * 4 + 3) * nmax + e] = P.v[4 * i + 3];
         }
     }
+}
+
+// k_forward: forward_one for the attempts of one channel tile per workgroup.
+template <int NT, bool ZS>
+__global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restrict__ seg_start, const int* __restrict__ tile_start,
+                                                   const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,
+                                                   uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {
+    int ch, e0, e1;
+    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;
+    const int e = e0 + threadIdx.x;
+    if (e >= e1) return;
+    const uint32_t a = sorted[e];
+    forward_one<NT, ZS>(A, ch, e, a, A.a0 + a, mom, nmax, acc, R, zsrc);
 }
 
 // Accepted attempts per block of kTile attempts (attempt order).
@@ -345,30 +357,31 @@ __device__ __forceinline__ double wave_max(double v) {
 constexpr int kRounds = 1;                 // sub-tiles of kTile points per k_main workgroup
 constexpr int kMainTile = kTile * kRounds;
 
-template <int NT>
-__global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict__ seg2, const int* __restrict__ tile2,
-                                                const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
-                                                double* __restrict__ psbuf, int64_t psstride, int train,
-                                                double* __restrict__ part, int64_t rec, double* __restrict__ vals_out, int64_t vals_off,
-                                                int64_t vals_cap, const uint32_t* __restrict__ sorted, PointRec R) {
+// One kept tile [e0, e1) of channel ch (RND sub-tiles of kTile points):
+// psbuf column slot * kTile + thread, partial record `out`.  Every thread of
+// the workgroup calls it (it synchronises the workgroup).
+template <int NT, int RND>
+__device__ __forceinline__ void main_tile(const MCArgs& A, int ch, int e0, int e1, double* s_mem, int64_t slot,
+                                          double* __restrict__ psbuf, int64_t psstride, int train, const uint32_t* __restrict__ comp,
+                                          const double* __restrict__ mom, int64_t nmax, double* __restrict__ out,
+                                          double* __restrict__ vals_out, int64_t vals_off, int64_t vals_cap,
+                                          const uint32_t* __restrict__ sorted, const PointRec& R) {
     constexpr int D = 3 * (NT - 2) - 2;
-    extern __shared__ double s_mem[];
     // LDS: hist [D][B], per-wave sums [kWaves][8 + C], bins [D][kTile] shorts
     const int B = A.B, C = A.C;
     double* s_hist = s_mem;
     double* s_small = s_hist + D * B;
     short* s_bins = (short*)(s_small + kWaves * (8 + C));
-    int ch, e0, e1;
-    if (!locate_tile(C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t gt = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const int64_t gt = slot * kTile + threadIdx.x;
+    __syncthreads();                     // a previous tile's record may still be reading LDS
     for (int i = threadIdx.x; i < D * B; i += kTile) s_hist[i] = 0.0;
     for (int i = threadIdx.x; i < kWaves * (8 + C); i += kTile) s_small[i] = 0.0;
     // per-lane StatsData moments, in round order
     double st_n = 0, st_s1 = 0, st_s2 = 0, st_nf = 0, st_nn = 0;
     double st_mn = std::numeric_limits<double>::max(), st_mx = std::numeric_limits<double>::min();
     __syncthreads();
-    for (int rnd = 0; rnd < kRounds; ++rnd) {
+    for (int rnd = 0; rnd < RND; ++rnd) {
         const int e = e0 + rnd * kTile + threadIdx.x;
         if (e0 + rnd * kTile >= e1) break;           // uniform: no points left in this super-tile
         const bool valid = e < e1;
@@ -467,7 +480,6 @@ __global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict_
             q[0] = n; q[1] = mn; q[2] = mx; q[3] = s1; q[4] = s2; q[5] = nf; q[6] = nn;
         }
     }
-    double* out = part + (int64_t)blockIdx.x * rec;
     __syncthreads();
     if (threadIdx.x == 0) {
         double acc[7] = {0, std::numeric_limits<double>::max(), std::numeric_limits<double>::min(), 0, 0, 0, 0};
@@ -485,6 +497,19 @@ __global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict_
         out[kStatW + j] = sj;
     }
     for (int i = threadIdx.x; i < D * B; i += kTile) out[kStatW + C + i] = s_hist[i];
+}
+
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict__ seg2, const int* __restrict__ tile2,
+                                                const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
+                                                double* __restrict__ psbuf, int64_t psstride, int train,
+                                                double* __restrict__ part, int64_t rec, double* __restrict__ vals_out, int64_t vals_off,
+                                                int64_t vals_cap, const uint32_t* __restrict__ sorted, PointRec R) {
+    extern __shared__ double s_mem[];
+    int ch, e0, e1;
+    if (!locate_tile(A.C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;
+    main_tile<NT, kRounds>(A, ch, e0, e1, s_mem, blockIdx.x, psbuf, psstride, train, comp, mom, nmax, part + (int64_t)blockIdx.x * rec,
+                           vals_out, vals_off, vals_cap, sorted, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -1154,6 +1179,491 @@ void check_flow(const chili_mc* m, const chili_flow* fl) {
 
 }  // namespace
 
+namespace {
+
+// ===========================================================================
+// Batched jobs: many independent multichannel integrations (each its own
+// chili_mc: channels, grids, alpha, integrand) of one multiplicity class in
+// one pass.  Attempts of job j occupy the blocks [blk0, blk0 + nblk) of one
+// concatenated attempt space; segments are (job, channel) pairs flattened to
+// g = g0(job) + channel; the forward and weight kernels are persistent
+// (fixed grids looping over device-counted tiles) and the cutoff is found on
+// the device, so a whole pass is host-synchronisation free and is captured
+// once in a hipGraph and replayed.
+// ===========================================================================
+struct BJob {
+    MCArgs A;           // the job's context (A.a0: first attempt counter of the pass)
+    int64_t att0;       // first attempt slot (multiple of kTile)
+    int64_t n;          // attempts of this pass
+    int64_t need;       // accepted points wanted
+    int64_t cnt0;       // offset of the job's count matrix [C][nblk]
+    int blk0, nblk;     // attempt blocks
+    int g0;             // first flattened segment
+    int pad_;
+    double* tot;        // running totals [7 + C + C d B]
+};
+
+__device__ __forceinline__ int block_job(const int* __restrict__ blkjob, int b) { return blkjob[b]; }
+
+__global__ __launch_bounds__(kTile) void k_bpick(const BJob* __restrict__ J, const int* __restrict__ blkjob, uint16_t* __restrict__ pick,
+                                                 int* __restrict__ counts) {
+    extern __shared__ int s_cnt[];
+    const BJob& jb = J[block_job(blkjob, blockIdx.x)];
+    const int C = jb.A.C;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) s_cnt[c] = 0;
+    __syncthreads();
+    const int lb = blockIdx.x - jb.blk0;
+    const int64_t local = (int64_t)lb * kTile + threadIdx.x;
+    if (local < jb.n) {
+        const int c = pick_channel(jb.A, channel_uniform(jb.A, jb.A.a0 + (uint64_t)local));
+        pick[jb.att0 + local] = (uint16_t)c;
+        atomicAdd(&s_cnt[c], 1);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < C; c += blockDim.x) counts[jb.cnt0 + (int64_t)c * jb.nblk + lb] = s_cnt[c];
+}
+
+__global__ __launch_bounds__(kTile) void k_bscatter(const BJob* __restrict__ J, const int* __restrict__ blkjob,
+                                                    const uint16_t* __restrict__ pick, const int* __restrict__ offs,
+                                                    uint32_t* __restrict__ sorted) {
+    extern __shared__ int s_wc[];   // [kWaves][C]
+    const BJob& jb = J[block_job(blkjob, blockIdx.x)];
+    const int C = jb.A.C;
+    for (int i = threadIdx.x; i < kWaves * C; i += blockDim.x) s_wc[i] = 0;
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    const int lb = blockIdx.x - jb.blk0;
+    const int64_t local = (int64_t)lb * kTile + threadIdx.x;
+    const bool valid = local < jb.n;
+    const int c = valid ? pick[jb.att0 + local] : 0;
+    const int rank = wave_rank(c, valid, s_wc + w * C);
+    __syncthreads();
+    if (valid) {
+        int before = 0;
+        for (int ww = 0; ww < w; ++ww) before += s_wc[ww * C + c];
+        sorted[offs[jb.cnt0 + (int64_t)c * jb.nblk + lb] + before + rank] = (uint32_t)(jb.att0 + local);
+    }
+}
+
+// Segment table of the flattened (job, channel) list; total = attempts of the pass.
+__global__ void k_bsegments(int G, const BJob* __restrict__ J, const int* __restrict__ gjob, const int* __restrict__ offs,
+                            const int64_t* __restrict__ total, int* seg, int* tile) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int t = 0;
+    for (int g = 0; g < G; ++g) {
+        const BJob& jb = J[gjob[g]];
+        const int c = g - jb.g0;
+        const int s0 = offs[jb.cnt0 + (int64_t)c * jb.nblk];
+        int e;
+        if (g + 1 < G) {
+            const BJob& jn = J[gjob[g + 1]];
+            e = offs[jn.cnt0 + (int64_t)(g + 1 - jn.g0) * jn.nblk];
+        } else {
+            e = (int)*total;
+        }
+        seg[g] = s0;
+        tile[g] = t;
+        t += (e - s0 + kTile - 1) / kTile;
+    }
+    seg[G] = (int)*total;
+    tile[G] = t;
+}
+
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_bforward(int G, const BJob* __restrict__ J, const int* __restrict__ gjob,
+                                                    const int* __restrict__ seg, const int* __restrict__ tile,
+                                                    const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,
+                                                    uint8_t* __restrict__ acc) {
+    const int ntile = tile[G];
+    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+        int g, e0, e1;
+        if (!locate_tile(G, seg, tile, t, g, e0, e1)) break;
+        const BJob& jb = J[gjob[g]];
+        const int e = e0 + threadIdx.x;
+        if (e < e1) {
+            const uint32_t a = sorted[e];
+            forward_one<NT, false>(jb.A, g - jb.g0, e, a, jb.A.a0 + (uint64_t)(a - jb.att0), mom, nmax, acc, PointRec{}, nullptr);
+        }
+    }
+}
+
+// Per job: the attempt at which `need` accepted attempts are reached (local
+// index + 1, or n) and the kept count (MultiChannel.hh:214-248 on the device).
+__global__ void k_bcutoff(const BJob* __restrict__ J, const int* __restrict__ blkacc, const uint8_t* __restrict__ acc,
+                          int64_t* __restrict__ cut, int64_t* __restrict__ kept) {
+    if (threadIdx.x != 0) return;
+    const BJob& jb = J[blockIdx.x];
+    int64_t run = 0, c = jb.n;
+    for (int lb = 0; lb < jb.nblk; ++lb) {
+        const int bc = blkacc[jb.blk0 + lb];
+        if (run + bc >= jb.need) {
+            for (int i = 0; i < kTile; ++i) {
+                const int64_t local = (int64_t)lb * kTile + i;
+                if (local >= jb.n) break;
+                run += acc[jb.att0 + local];
+                if (run == jb.need) { c = local + 1; break; }
+            }
+            break;
+        }
+        run += bc;
+    }
+    cut[blockIdx.x] = c;
+    kept[blockIdx.x] = run < jb.need ? run : jb.need;
+}
+
+__device__ __forceinline__ bool bkeep(const BJob* __restrict__ J, const int* __restrict__ gjob, const int64_t* __restrict__ cut,
+                                      const uint8_t* __restrict__ acc, int g, uint32_t a) {
+    const int j = gjob[g];
+    return (int64_t)a - J[j].att0 < cut[j] && acc[a];
+}
+
+__global__ __launch_bounds__(kTile) void k_bkeep_count(int G, const BJob* __restrict__ J, const int* __restrict__ gjob,
+                                                       const int* __restrict__ seg, const int* __restrict__ tile,
+                                                       const uint32_t* __restrict__ sorted, const uint8_t* __restrict__ acc,
+                                                       const int64_t* __restrict__ cut, int* __restrict__ kept) {
+    __shared__ int s[kWaves];
+    int g, e0, e1;
+    if (!locate_tile(G, seg, tile, blockIdx.x, g, e0, e1)) {
+        if (threadIdx.x == 0) kept[blockIdx.x] = 0;
+        return;
+    }
+    const int e = e0 + threadIdx.x;
+    const bool k = e < e1 && bkeep(J, gjob, cut, acc, g, sorted[e]);
+    const uint64_t m = __ballot(k);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kWaves; ++w) t += s[w];
+        kept[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kTile) void k_bkeep_scatter(int G, const BJob* __restrict__ J, const int* __restrict__ gjob,
+                                                         const int* __restrict__ seg, const int* __restrict__ tile,
+                                                         const uint32_t* __restrict__ sorted, const uint8_t* __restrict__ acc,
+                                                         const int64_t* __restrict__ cut, const int* __restrict__ kept_off,
+                                                         uint32_t* __restrict__ comp) {
+    __shared__ int s[kWaves];
+    int g, e0, e1;
+    if (!locate_tile(G, seg, tile, blockIdx.x, g, e0, e1)) return;
+    const int e = e0 + threadIdx.x;
+    const bool k = e < e1 && bkeep(J, gjob, cut, acc, g, sorted[e]);
+    const uint64_t m = __ballot(k);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s[w] = __popcll(m);
+    __syncthreads();
+    if (k) {
+        int before = 0;
+        for (int ww = 0; ww < w; ++ww) before += s[ww];
+        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        comp[kept_off[blockIdx.x] + before + __popcll(m & lt)] = (uint32_t)e;
+    }
+}
+
+// Kept segments (tiles of tsize) from the per-forward-tile kept offsets; the
+// kept total is the scan's device total.
+__global__ void k_bkeep_segments(int G, const int* __restrict__ tile, const int* __restrict__ kept_off,
+                                 const int64_t* __restrict__ kept_total, int* seg2, int* tile2, int tsize) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int kt = (int)*kept_total;
+    int t = 0;
+    for (int g = 0; g < G; ++g) {
+        const int s0 = (tile[g] >= tile[G]) ? kt : kept_off[tile[g]];
+        const int e = (g + 1 < G && tile[g + 1] < tile[G]) ? kept_off[tile[g + 1]] : kt;
+        seg2[g] = s0;
+        tile2[g] = t;
+        t += (e - s0 + tsize - 1) / tsize;
+    }
+    seg2[G] = kt;
+    tile2[G] = t;
+}
+
+template <int NT, int RND>
+__global__ __launch_bounds__(kTile) void k_bmain(int G, const BJob* __restrict__ J, const int* __restrict__ gjob,
+                                                 const int* __restrict__ seg2, const int* __restrict__ tile2,
+                                                 const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
+                                                 double* __restrict__ psbuf, int64_t psstride, int train, double* __restrict__ part,
+                                                 int64_t rec) {
+    extern __shared__ double s_mem[];
+    const int ntile = tile2[G];
+    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+        int g, e0, e1;
+        if (!locate_tile(G, seg2, tile2, t, g, e0, e1, kTile * RND)) break;
+        const BJob& jb = J[gjob[g]];
+        main_tile<NT, RND>(jb.A, g - jb.g0, e0, e1, s_mem, blockIdx.x, psbuf, psstride, train, comp, mom, nmax,
+                           part + (int64_t)t * rec, nullptr, 0, 0, nullptr, PointRec{});
+    }
+}
+
+// StatsData start values of every job's totals (max starts at the smallest
+// positive normal, Statistics.hh:141)
+__global__ void k_btot_init(const BJob* __restrict__ J, int njobs, int DB) {
+    const int j = blockIdx.x;
+    if (j >= njobs) return;
+    const BJob& jb = J[j];
+    const int n = kStatW + jb.A.C + jb.A.C * DB;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        jb.tot[i] = (i == 1) ? std::numeric_limits<double>::max() : (i == 2) ? std::numeric_limits<double>::min() : 0.0;
+}
+
+// Per job (blockIdx.x) the fixed-order sums of its tiles' partial records
+// into its totals: stats, T_c, and each channel's histogram over that
+// channel's tiles.
+__global__ __launch_bounds__(256) void k_breduce(const BJob* __restrict__ J, const int* __restrict__ tile2, const double* __restrict__ part,
+                                                 int64_t rec, int DB, int train) {
+    const BJob& jb = J[blockIdx.x];
+    const int C = jb.A.C;
+    const int t0 = tile2[jb.g0], t1 = tile2[jb.g0 + C];
+    const int64_t nout = train ? 1 + (int64_t)C + (int64_t)C * DB : 1;
+    for (int64_t i = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; i < nout; i += (int64_t)gridDim.y * blockDim.x) {
+        if (i == 0) {
+            double a[7] = {jb.tot[0], jb.tot[1], jb.tot[2], jb.tot[3], jb.tot[4], jb.tot[5], jb.tot[6]};
+            for (int t = t0; t < t1; ++t) stat_merge(a, part + (int64_t)t * rec);
+            for (int k = 0; k < kStatW; ++k) jb.tot[k] = a[k];
+        } else if (i <= C) {
+            double s = jb.tot[kStatW + i - 1];
+            for (int t = t0; t < t1; ++t) s += part[(int64_t)t * rec + kStatW + i - 1];
+            jb.tot[kStatW + i - 1] = s;
+        } else {
+            const int64_t h = i - 1 - C;
+            const int c = (int)(h / DB), k = (int)(h % DB);
+            double s = jb.tot[kStatW + C + h];
+            for (int t = tile2[jb.g0 + c]; t < tile2[jb.g0 + c + 1]; ++t) s += part[(int64_t)t * rec + kStatW + C + k];
+            jb.tot[kStatW + C + h] = s;
+        }
+    }
+}
+
+}  // namespace
+
+namespace {
+
+// ===========================================================================
+// Batched jobs: host side
+// ===========================================================================
+struct BatchGroup {
+    int NT = 0, D = 0, G = 0, Cmax = 0;
+    std::vector<int> jobs;                 // indices into chili_batch::jobs
+    std::vector<int64_t> n;                // planned attempts per job (this group's order)
+    int64_t natt = 0, ncnt = 0;
+    int nblk = 0, tgrid = 0;
+    std::vector<int64_t> cnt0, att0;
+    std::vector<int> blk0, g0;
+    std::vector<int64_t> tot_off;
+    DevBuf<BJob> d_jobs;
+    DevBuf<int> d_blkjob, d_gjob, d_counts, d_offs, d_seg, d_tile, d_blkacc, d_kept, d_keptoff, d_seg2, d_tile2;
+    DevBuf<uint16_t> d_pick;
+    DevBuf<uint32_t> d_sorted, d_comp;
+    DevBuf<uint8_t> d_acc;
+    DevBuf<double> d_mom, d_ps, d_part, d_tot;
+    DevBuf<int64_t> d_total, d_total2, d_bsum, d_cut, d_keptj;
+    std::vector<BJob> h_jobs;
+    std::pair<hipEvent_t, hipEvent_t> ev_fwd{nullptr, nullptr}, ev_main{nullptr, nullptr};
+};
+
+}  // namespace
+
+struct chili_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ncu = 256;
+    std::vector<chili_mc*> jobs;
+    std::vector<int> job_group;
+    std::vector<BatchGroup> groups;
+    std::vector<double> acc_est;
+    int B = 0;
+    // launch shape (autotuned): workgroups per CU of the persistent kernels,
+    // kTile sub-tiles per weight-kernel tile
+    int fwd_wpc = 4, main_wpc = 2, main_rnd = 1;
+    // hipGraph of one pass of every group
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_valid = false;
+    int graph_train = -1;
+    int64_t captures = 0, replays = 0, topups = 0;
+    ~chili_batch() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        for (auto& g : groups) {
+            if (g.ev_fwd.first) { hipEventDestroy(g.ev_fwd.first); hipEventDestroy(g.ev_fwd.second); }
+            if (g.ev_main.first) { hipEventDestroy(g.ev_main.first); hipEventDestroy(g.ev_main.second); }
+        }
+    }
+};
+
+namespace {
+
+void bscan(hipStream_t st, const int* in, int* out, int64_t len, DevBuf<int64_t>& bsum, int64_t* total) {
+    const int64_t nb = (len + kScanBlock - 1) / kScanBlock;
+    bsum.resize(nb);
+    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, len, bsum.ptr);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, st, bsum.ptr, nb, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, out, len, bsum.ptr);
+}
+
+int batch_fgrid(const chili_batch* b, const BatchGroup& g) {
+    return std::max(1, std::min(g.tgrid, b->fwd_wpc * b->ncu));
+}
+int batch_mgrid(const chili_batch* b, const BatchGroup& g) {
+    const int mtiles = (int)(g.natt / (kTile * b->main_rnd)) + g.G + 1;
+    return std::max(1, std::min(mtiles, b->main_wpc * b->ncu));
+}
+
+// Attempt plan of a group: n_j from the acceptance estimate; buffers sized to
+// it.  Returns true when the plan (and so the captured graph) changed.
+bool batch_plan(chili_batch* b, BatchGroup& g, const int64_t* ncalls) {
+    std::vector<int64_t> n(g.jobs.size());
+    bool changed = g.n.size() != g.jobs.size();
+    for (size_t q = 0; q < g.jobs.size(); ++q) {
+        const int j = g.jobs[q];
+        const double acc = std::max(b->acc_est[j], 0.01);
+        int64_t want = (int64_t)((double)ncalls[j] / acc * 1.2) + 1024;
+        want = ((want + kTile - 1) / kTile) * kTile;
+        if (want > (int64_t(1) << 26)) want = int64_t(1) << 26;
+        // keep the previous plan (and graph) while it covers the need without
+        // more than 3x waste
+        if (!changed && g.n[q] >= (int64_t)((double)ncalls[j] / acc * 1.05) && g.n[q] <= 3 * want) n[q] = g.n[q];
+        else { n[q] = want; changed = true; }
+    }
+    if (!changed) return false;
+    g.n = n;
+    const int J = (int)g.jobs.size();
+    g.cnt0.assign(J, 0); g.att0.assign(J, 0); g.blk0.assign(J, 0); g.g0.assign(J, 0); g.tot_off.assign(J, 0);
+    int64_t att = 0, cnt = 0, toff = 0;
+    int blk = 0, gg = 0;
+    std::vector<int> blkjob, gjob;
+    for (int q = 0; q < J; ++q) {
+        chili_mc* m = b->jobs[g.jobs[q]];
+        const int nb = (int)(g.n[q] / kTile);
+        g.att0[q] = att; g.blk0[q] = blk; g.cnt0[q] = cnt; g.g0[q] = gg; g.tot_off[q] = toff;
+        for (int k = 0; k < nb; ++k) blkjob.push_back(q);
+        for (int c = 0; c < m->C; ++c) gjob.push_back(q);
+        att += g.n[q];
+        blk += nb;
+        cnt += (int64_t)m->C * nb;
+        gg += m->C;
+        toff += kStatW + m->C + (int64_t)m->C * m->d * m->B;
+    }
+    g.natt = att; g.nblk = blk; g.ncnt = cnt; g.G = gg;
+    g.tgrid = blk + gg;
+    if (att >= (int64_t(1) << 31)) throw chili_error(CHILI_ERANGE, "batched attempts exceed 2^31 per multiplicity class");
+    g.d_blkjob.upload(blkjob, b->stream);
+    g.d_gjob.upload(gjob, b->stream);
+    g.d_pick.resize(att);
+    g.d_sorted.resize(att);
+    g.d_comp.resize(att);
+    g.d_acc.resize(att);
+    g.d_mom.resize((size_t)att * g.NT * 4);
+    g.d_counts.resize(cnt);
+    g.d_offs.resize(cnt);
+    g.d_seg.resize(gg + 1);
+    g.d_tile.resize(gg + 1);
+    g.d_seg2.resize(gg + 1);
+    g.d_tile2.resize(gg + 1);
+    g.d_blkacc.resize(blk);
+    g.d_kept.resize(g.tgrid);
+    g.d_keptoff.resize(g.tgrid);
+    g.d_tot.resize(toff);
+    g.d_total.resize(1);
+    g.d_total2.resize(1);
+    g.d_cut.resize(J);
+    g.d_keptj.resize(J);
+    // both scans of a pass share the block-sum buffer: size it once (no
+    // allocation may happen while a pass is being captured)
+    g.d_bsum.resize((std::max<int64_t>(cnt, g.tgrid) + kScanBlock - 1) / kScanBlock + 1);
+    return true;
+}
+
+// Buffers that depend on the launch shape (persistent grids, rounds).
+void batch_shape_buffers(chili_batch* b, BatchGroup& g) {
+    const int mgrid = batch_mgrid(b, g);
+    const int64_t rec = kStatW + g.Cmax + (int64_t)g.D * b->B;
+    const int64_t mtiles = g.natt / kTile + g.G + 1;   // at rnd = 1 (the most tiles)
+    g.d_part.resize((size_t)mtiles * rec);
+    g.d_ps.resize((size_t)g.Cmax * mgrid * kTile);
+}
+
+template <int NT, int RND>
+void launch_bmain(chili_batch* b, BatchGroup& g, int train) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    const int mgrid = batch_mgrid(b, g);
+    const int64_t rec = kStatW + g.Cmax + (int64_t)D * b->B;
+    const size_t lds = sizeof(double) * (D * b->B + kWaves * (8 + g.Cmax)) + sizeof(short) * D * kTile;
+    if (lds > 160 * 1024) throw chili_error(CHILI_ERANGE, "channel count x bins exceed the LDS of one workgroup");
+    hipLaunchKernelGGL((k_bmain<NT, RND>), dim3(mgrid), dim3(kTile), lds, b->stream, g.G, g.d_jobs.ptr, g.d_gjob.ptr, g.d_seg2.ptr,
+                       g.d_tile2.ptr, g.d_comp.ptr, g.d_mom.ptr, g.natt, g.d_ps.ptr, (int64_t)mgrid * kTile, train, g.d_part.ptr, rec);
+}
+
+template <int NT>
+void enqueue_group(chili_batch* b, BatchGroup& g, int train, bool timed) {
+    hipStream_t st = b->stream;
+    const int J = (int)g.jobs.size();
+    const int DB = g.D * b->B;
+    hipLaunchKernelGGL(k_btot_init, dim3(J), dim3(256), 0, st, g.d_jobs.ptr, J, DB);
+    hipLaunchKernelGGL(chili_nf::k_fill<uint8_t>, dim3((unsigned)((g.natt + 255) / 256)), dim3(256), 0, st, g.natt, (uint8_t)0, g.d_acc.ptr);
+    hipLaunchKernelGGL(k_bpick, dim3(g.nblk), dim3(kTile), sizeof(int) * g.Cmax, st, g.d_jobs.ptr, g.d_blkjob.ptr, g.d_pick.ptr,
+                       g.d_counts.ptr);
+    bscan(st, g.d_counts.ptr, g.d_offs.ptr, g.ncnt, g.d_bsum, g.d_total.ptr);
+    hipLaunchKernelGGL(k_bscatter, dim3(g.nblk), dim3(kTile), sizeof(int) * kWaves * g.Cmax, st, g.d_jobs.ptr, g.d_blkjob.ptr,
+                       g.d_pick.ptr, g.d_offs.ptr, g.d_sorted.ptr);
+    hipLaunchKernelGGL(k_bsegments, dim3(1), dim3(1), 0, st, g.G, g.d_jobs.ptr, g.d_gjob.ptr, g.d_offs.ptr, g.d_total.ptr, g.d_seg.ptr,
+                       g.d_tile.ptr);
+    if (timed) HIP_CHECK(hipEventRecord(g.ev_fwd.first, st));
+    hipLaunchKernelGGL(k_bforward<NT>, dim3(batch_fgrid(b, g)), dim3(kTile), 0, st, g.G, g.d_jobs.ptr, g.d_gjob.ptr, g.d_seg.ptr,
+                       g.d_tile.ptr, g.d_sorted.ptr, g.d_mom.ptr, g.natt, g.d_acc.ptr);
+    if (timed) HIP_CHECK(hipEventRecord(g.ev_fwd.second, st));
+    hipLaunchKernelGGL(k_acc_count, dim3(g.nblk), dim3(kTile), 0, st, g.d_acc.ptr, g.natt, g.d_blkacc.ptr);
+    hipLaunchKernelGGL(k_bcutoff, dim3(J), dim3(64), 0, st, g.d_jobs.ptr, g.d_blkacc.ptr, g.d_acc.ptr, g.d_cut.ptr, g.d_keptj.ptr);
+    hipLaunchKernelGGL(k_bkeep_count, dim3(g.tgrid), dim3(kTile), 0, st, g.G, g.d_jobs.ptr, g.d_gjob.ptr, g.d_seg.ptr, g.d_tile.ptr,
+                       g.d_sorted.ptr, g.d_acc.ptr, g.d_cut.ptr, g.d_kept.ptr);
+    bscan(st, g.d_kept.ptr, g.d_keptoff.ptr, g.tgrid, g.d_bsum, g.d_total2.ptr);
+    hipLaunchKernelGGL(k_bkeep_scatter, dim3(g.tgrid), dim3(kTile), 0, st, g.G, g.d_jobs.ptr, g.d_gjob.ptr, g.d_seg.ptr, g.d_tile.ptr,
+                       g.d_sorted.ptr, g.d_acc.ptr, g.d_cut.ptr, g.d_keptoff.ptr, g.d_comp.ptr);
+    hipLaunchKernelGGL(k_bkeep_segments, dim3(1), dim3(1), 0, st, g.G, g.d_tile.ptr, g.d_keptoff.ptr, g.d_total2.ptr, g.d_seg2.ptr,
+                       g.d_tile2.ptr, kTile * b->main_rnd);
+    if (timed) HIP_CHECK(hipEventRecord(g.ev_main.first, st));
+    switch (b->main_rnd) {
+        case 1: launch_bmain<NT, 1>(b, g, train); break;
+        case 2: launch_bmain<NT, 2>(b, g, train); break;
+        case 4: launch_bmain<NT, 4>(b, g, train); break;
+        default: throw chili_error(CHILI_EINVAL, "points per thread must be 1, 2 or 4");
+    }
+    if (timed) HIP_CHECK(hipEventRecord(g.ev_main.second, st));
+    const int ny = train ? std::max(1, (int)std::min<int64_t>(64, (1 + g.Cmax + (int64_t)g.Cmax * DB + 255) / 256)) : 1;
+    hipLaunchKernelGGL(k_breduce, dim3(J, ny), dim3(256), 0, st, g.d_jobs.ptr, g.d_tile2.ptr, g.d_part.ptr,
+                       kStatW + g.Cmax + (int64_t)DB, DB, train);
+}
+
+void enqueue_all(chili_batch* b, int train, bool timed) {
+    for (auto& g : b->groups) CHILI_DISPATCH_NT(g.NT, enqueue_group<NT>(b, g, train, timed));
+}
+
+// Per-job device tables of this pass (Philox stream and seed per job, a0 = 0).
+void batch_upload_jobs(chili_batch* b, const uint64_t* seed, const uint32_t* streams, const int64_t* ncalls) {
+    for (auto& g : b->groups) {
+        const int J = (int)g.jobs.size();
+        g.h_jobs.assign(J, BJob{});
+        for (int q = 0; q < J; ++q) {
+            const int j = g.jobs[q];
+            chili_mc* m = b->jobs[j];
+            BJob& jb = g.h_jobs[q];
+            jb.A = make_args(m, nullptr, 0, seed[j], streams[j], 0);
+            jb.att0 = g.att0[q];
+            jb.n = g.n[q];
+            jb.need = ncalls[j];
+            jb.cnt0 = g.cnt0[q];
+            jb.blk0 = g.blk0[q];
+            jb.nblk = (int)(g.n[q] / kTile);
+            jb.g0 = g.g0[q];
+            jb.tot = g.d_tot.ptr + g.tot_off[q];
+        }
+        g.d_jobs.upload(g.h_jobs, b->stream);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int chili_construct_channels(const chili_process* proc, chili_channels** out) {
@@ -1479,6 +1989,232 @@ int chili_mc_flow_points(chili_mc* m, chili_flow* fl, const chili_uniforms* uni,
         if (lq) lv.download(lq, (size_t)m->C * n, m->stream);
         if (r) rv.download(r, (size_t)m->d * n, m->stream);
         HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Batched jobs (no single reference counterpart: many MultiChannel::operator()
+// iterations, MultiChannel.hh:196-251, of independent integrands at once).
+// ---------------------------------------------------------------------------
+int chili_batch_create(int njobs, chili_mc* const* jobs, chili_batch** out) {
+    return guarded([&] {
+        if (!jobs || !out || njobs < 1) throw chili_error(CHILI_EINVAL, "need at least one job");
+        auto b = std::make_unique<chili_batch>();
+        b->device = jobs[0]->device;
+        b->B = jobs[0]->B;
+        for (int j = 0; j < njobs; ++j) {
+            if (!jobs[j]) throw chili_error(CHILI_EINVAL, "null job");
+            if (jobs[j]->device != b->device) throw chili_error(CHILI_EINVAL, "jobs of a batch share one device");
+            if (jobs[j]->B != b->B) throw chili_error(CHILI_EINVAL, "jobs of a batch share one bin count");
+            b->jobs.push_back(jobs[j]);
+        }
+        select_device(b->device);
+        HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+        b->acc_est.assign(njobs, 0.3);
+        b->job_group.assign(njobs, -1);
+        for (int nt = 4; nt <= 8; ++nt) {
+            BatchGroup g;
+            g.NT = nt;
+            g.D = 3 * (nt - 2) - 2;
+            for (int j = 0; j < njobs; ++j)
+                if (b->jobs[j]->NT == nt) {
+                    b->job_group[j] = (int)b->groups.size();
+                    g.jobs.push_back(j);
+                    g.Cmax = std::max(g.Cmax, b->jobs[j]->C);
+                }
+            if (g.jobs.empty()) continue;
+            HIP_CHECK(hipEventCreate(&g.ev_fwd.first));
+            HIP_CHECK(hipEventCreate(&g.ev_fwd.second));
+            HIP_CHECK(hipEventCreate(&g.ev_main.first));
+            HIP_CHECK(hipEventCreate(&g.ev_main.second));
+            b->groups.push_back(std::move(g));
+        }
+        *out = b.release();
+    });
+}
+
+int chili_batch_destroy(chili_batch* b) {
+    return guarded([&] {
+        if (!b) return;
+        select_device(b->device);
+        hipStream_t s = b->stream;
+        if (s) hipStreamSynchronize(s);
+        delete b;
+        if (s) hipStreamDestroy(s);
+    });
+}
+
+// Launch shape: workgroups per CU of the persistent forward and weight
+// kernels, and kTile sub-tiles (points per thread) of a weight-kernel tile.
+int chili_batch_set_shape(chili_batch* b, int fwd_wpc, int main_wpc, int main_rnd) {
+    return guarded([&] {
+        if (!b) throw chili_error(CHILI_EINVAL, "null batch");
+        if (fwd_wpc < 1 || fwd_wpc > 16 || main_wpc < 1 || main_wpc > 8) throw chili_error(CHILI_ERANGE, "workgroups per CU out of range");
+        if (main_rnd != 1 && main_rnd != 2 && main_rnd != 4) throw chili_error(CHILI_ERANGE, "points per thread must be 1, 2 or 4");
+        b->fwd_wpc = fwd_wpc;
+        b->main_wpc = main_wpc;
+        b->main_rnd = main_rnd;
+        b->graph_valid = false;
+    });
+}
+
+int chili_batch_get_shape(const chili_batch* b, int* fwd_wpc, int* main_wpc, int* main_rnd, int64_t* captures, int64_t* replays,
+                          int64_t* topups) {
+    return guarded([&] {
+        if (!b) throw chili_error(CHILI_EINVAL, "null batch");
+        if (fwd_wpc) *fwd_wpc = b->fwd_wpc;
+        if (main_wpc) *main_wpc = b->main_wpc;
+        if (main_rnd) *main_rnd = b->main_rnd;
+        if (captures) *captures = b->captures;
+        if (replays) *replays = b->replays;
+        if (topups) *topups = b->topups;
+    });
+}
+
+// One iteration of every job: seed[j] / stream[j] select job j's Philox
+// stream (attempts from 0), ncalls[j] accepted points.  One device pass runs
+// every job (replayed from a hipGraph when use_graph != 0); a job whose pass
+// falls short (acceptance below the plan's estimate, or NaN values) continues
+// from its cutoff through the single-job path, so every job gets exactly the
+// reference's retry semantics.  Outputs per job: stats[j], attempts[j]; with
+// train != 0, T[j] (host [C_j]) and hist[j] (host [C_j][d_j][B]) when the
+// pointer arrays are given.
+int chili_batch_iterate(chili_batch* b, const uint64_t* seed, const uint32_t* streams, const int64_t* ncalls, int train, int use_graph,
+                        chili_stats* stats, int64_t* attempts, double* const* T, double* const* hist) {
+    return guarded([&] {
+        if (!b || !seed || !streams || !ncalls || !stats) throw chili_error(CHILI_EINVAL, "null argument");
+        const int NJ = (int)b->jobs.size();
+        for (int j = 0; j < NJ; ++j)
+            if (ncalls[j] <= 0) throw chili_error(CHILI_EINVAL, "ncalls must be positive");
+        select_device(b->device);
+        for (chili_mc* m : b->jobs) HIP_CHECK(hipStreamSynchronize(m->stream));   // state uploads
+        train = train ? 1 : 0;
+        for (auto& g : b->groups) {
+            if (batch_plan(b, g, ncalls)) b->graph_valid = false;
+            batch_shape_buffers(b, g);
+        }
+        batch_upload_jobs(b, seed, streams, ncalls);
+        if (use_graph) {
+            if (!b->graph_valid || b->graph_train != train) {
+                if (b->exec) { HIP_CHECK(hipGraphExecDestroy(b->exec)); b->exec = nullptr; }
+                if (b->graph) { HIP_CHECK(hipGraphDestroy(b->graph)); b->graph = nullptr; }
+                HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                enqueue_all(b, train, false);
+                HIP_CHECK(hipStreamEndCapture(b->stream, &b->graph));
+                HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+                b->graph_valid = true;
+                b->graph_train = train;
+                b->captures += 1;
+            }
+            HIP_CHECK(hipGraphLaunch(b->exec, b->stream));
+            b->replays += 1;
+        } else {
+            enqueue_all(b, train, false);
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(b->stream));
+        for (auto& g : b->groups) {
+            const int J = (int)g.jobs.size();
+            std::vector<int64_t> cut(J), kept(J);
+            std::vector<double> tot(g.d_tot.n);
+            g.d_cut.download(cut.data(), J, b->stream);
+            g.d_keptj.download(kept.data(), J, b->stream);
+            g.d_tot.download(tot.data(), tot.size(), b->stream);
+            HIP_CHECK(hipStreamSynchronize(b->stream));
+            for (int q = 0; q < J; ++q) {
+                const int j = g.jobs[q];
+                chili_mc* m = b->jobs[j];
+                const double* t = tot.data() + g.tot_off[q];
+                const size_t ntot = kStatW + m->C + (size_t)m->C * m->d * m->B;
+                std::vector<double> sum(t, t + ntot);
+                const int64_t rej = cut[q] - kept[q];
+                double n = t[0] + (double)rej, mn = t[1], mx = t[2];
+                if (rej > 0) { mn = std::min(mn, 0.0); mx = std::max(mx, 0.0); }
+                double s1 = t[3], s2 = t[4], nf = t[5];
+                int64_t att = cut[q];
+                const int64_t deficit = ncalls[j] - (kept[q] - (int64_t)t[6]);
+                if (deficit > 0) {
+                    // continue the job's attempt sequence from its cutoff
+                    chili_uniforms uni{nullptr, 0, seed[j], streams[j], 0};
+                    std::vector<double> tt;
+                    int64_t att2 = 0;
+                    chili_stats st2{};
+                    iterate_loop(m, &uni, (uint64_t)cut[q], deficit, int64_t(1) << 23, tt, att2, &st2,
+                                 [&](const double* ud, int64_t ustride, uint64_t ab, int64_t nn, int64_t need, int64_t got,
+                                     int64_t& kk, int64_t& used) {
+                                     run_subbatch(m, ud, ustride, (uint64_t)cut[q], seed[j], streams[j], ab, nn, need, train, nullptr,
+                                                  0, 0, kk, used);
+                                 });
+                    n += st2.n; mn = std::min(mn, st2.min); mx = std::max(mx, st2.max);
+                    s1 += st2.sum; s2 += st2.sum2; nf += st2.n_finite;
+                    for (size_t k = kStatW; k < ntot; ++k) sum[k] += tt[k];
+                    att += att2;
+                    b->topups += 1;
+                }
+                stats[j].n = n; stats[j].min = mn; stats[j].max = mx; stats[j].sum = s1; stats[j].sum2 = s2; stats[j].n_finite = nf;
+                if (attempts) attempts[j] = att;
+                if (train && T && T[j]) std::copy(sum.begin() + kStatW, sum.begin() + kStatW + m->C, T[j]);
+                if (train && hist && hist[j]) std::copy(sum.begin() + kStatW + m->C, sum.end(), hist[j]);
+                if (cut[q] > 0) b->acc_est[j] = std::max(0.01, (double)kept[q] / (double)cut[q]);
+            }
+        }
+    });
+}
+
+// Autotuner: times the persistent forward kernel at 1, 2, 4, 8 workgroups per
+// CU and the weight kernel at 1, 2 workgroups per CU x 1, 2, 4 points per
+// thread on one pass of this batch (no training), and keeps the fastest of
+// each.  best_ms (optional): [0] forward, [1] weight kernel time of the
+// chosen shapes.
+int chili_batch_tune(chili_batch* b, const uint64_t* seed, const uint32_t* streams, const int64_t* ncalls, double* best_ms) {
+    return guarded([&] {
+        if (!b || !seed || !streams || !ncalls) throw chili_error(CHILI_EINVAL, "null argument");
+        select_device(b->device);
+        for (chili_mc* m : b->jobs) HIP_CHECK(hipStreamSynchronize(m->stream));
+        auto pass_ms = [&](double& fwd, double& mainms) {
+            for (auto& g : b->groups) {
+                if (batch_plan(b, g, ncalls)) b->graph_valid = false;
+                batch_shape_buffers(b, g);
+            }
+            batch_upload_jobs(b, seed, streams, ncalls);
+            enqueue_all(b, 0, true);
+            HIP_CHECK(hipStreamSynchronize(b->stream));
+            fwd = 0;
+            mainms = 0;
+            for (auto& g : b->groups) {
+                float a = 0, c = 0;
+                HIP_CHECK(hipEventElapsedTime(&a, g.ev_fwd.first, g.ev_fwd.second));
+                HIP_CHECK(hipEventElapsedTime(&c, g.ev_main.first, g.ev_main.second));
+                fwd += a;
+                mainms += c;
+            }
+        };
+        double f0, m0;
+        pass_ms(f0, m0);                 // warm-up (plans and buffers)
+        int best_f = b->fwd_wpc;
+        double tf = 1e300;
+        for (int w : {1, 2, 4, 8}) {
+            b->fwd_wpc = w;
+            double f, mm;
+            pass_ms(f, mm);
+            if (f < tf) { tf = f; best_f = w; }
+        }
+        b->fwd_wpc = best_f;
+        int best_w = b->main_wpc, best_r = b->main_rnd;
+        double tm = 1e300;
+        for (int w : {1, 2})
+            for (int r : {1, 2, 4}) {
+                b->main_wpc = w;
+                b->main_rnd = r;
+                double f, mm;
+                pass_ms(f, mm);
+                if (mm < tm) { tm = mm; best_w = w; best_r = r; }
+            }
+        b->main_wpc = best_w;
+        b->main_rnd = best_r;
+        b->graph_valid = false;
+        if (best_ms) { best_ms[0] = tf; best_ms[1] = tm; }
     });
 }
 

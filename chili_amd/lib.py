@@ -25,6 +25,8 @@ EXPORTS = [
     "chili_flow_create", "chili_flow_destroy", "chili_flow_info", "chili_flow_mask", "chili_flow_get_params",
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
     "chili_flow_train", "chili_gemm_bf16_test",
+    "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
+    "chili_batch_tune",
 ]
 
 
@@ -88,6 +90,15 @@ def load():
     L.chili_flow_train.argtypes = [P, I64, VP, VP, VP, I64, C.c_int, C.c_double, U64, D]
     L.chili_gemm_bf16_test.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), F, C.c_int,
                                        C.c_int]
+    PP = C.POINTER(C.c_void_p)
+    L.chili_batch_create.argtypes = [C.c_int, PP, C.POINTER(P)]
+    L.chili_batch_destroy.argtypes = [P]
+    L.chili_batch_iterate.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_int64), C.c_int, C.c_int,
+                                      C.POINTER(abi.Stats), C.POINTER(C.c_int64), PP, PP]
+    L.chili_batch_set_shape.argtypes = [P, C.c_int, C.c_int, C.c_int]
+    L.chili_batch_get_shape.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.chili_batch_tune.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_int64), D]
     _lib = L
     return L
 
@@ -293,6 +304,65 @@ class FlowGPU:
         hist = np.zeros(max(epochs, 1))
         check(load().chili_flow_train(self.h, n, r_ptr, beta_ptr, w2_ptr, batch, epochs, lr, seed, _d(hist)))
         return hist[:epochs]
+
+
+class BatchGPU:
+    """Many independent multichannel jobs (MultiChannelGPU contexts) in one
+    device pass per multiplicity class (chili_batch_*)."""
+
+    def __init__(self, jobs):
+        L = load()
+        self.jobs = list(jobs)
+        arr = (C.c_void_p * len(self.jobs))(*[j.h.value for j in self.jobs])
+        h = C.c_void_p()
+        check(L.chili_batch_create(len(self.jobs), arr, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                load().chili_batch_destroy(self.h)
+            except TypeError:
+                pass
+            self.h = None
+
+    def _vecs(self, seeds, streams, ncalls):
+        n = len(self.jobs)
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        st = np.ascontiguousarray(streams, dtype=np.uint32)
+        nc = np.ascontiguousarray(ncalls, dtype=np.int64)
+        assert len(s) == len(st) == len(nc) == n
+        return s, st, nc
+
+    def iterate(self, seeds, streams, ncalls, train=True, graph=True):
+        s, st, nc = self._vecs(seeds, streams, ncalls)
+        n = len(self.jobs)
+        stats = (abi.Stats * n)()
+        att = np.zeros(n, np.int64)
+        Ts = [np.zeros(j.nchannels) for j in self.jobs]
+        hs = [np.zeros((j.nchannels, j.ndims, j.nbins)) for j in self.jobs]
+        Tp = (C.c_void_p * n)(*[t.ctypes.data for t in Ts])
+        hp = (C.c_void_p * n)(*[h.ctypes.data for h in hs])
+        check(load().chili_batch_iterate(self.h, s.ctypes.data_as(C.POINTER(C.c_uint64)), st.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         nc.ctypes.data_as(C.POINTER(C.c_int64)), 1 if train else 0, 1 if graph else 0, stats,
+                                         att.ctypes.data_as(C.POINTER(C.c_int64)), Tp, hp))
+        return [stats[i] for i in range(n)], att, Ts, hs
+
+    def set_shape(self, fwd_wpc, main_wpc, main_rnd):
+        check(load().chili_batch_set_shape(self.h, fwd_wpc, main_wpc, main_rnd))
+
+    def shape(self):
+        a, b_, c = C.c_int(), C.c_int(), C.c_int()
+        cap, rep, top = C.c_int64(), C.c_int64(), C.c_int64()
+        check(load().chili_batch_get_shape(self.h, C.byref(a), C.byref(b_), C.byref(c), C.byref(cap), C.byref(rep), C.byref(top)))
+        return dict(fwd_wpc=a.value, main_wpc=b_.value, main_rnd=c.value, captures=cap.value, replays=rep.value, topups=top.value)
+
+    def tune(self, seeds, streams, ncalls):
+        s, st, nc = self._vecs(seeds, streams, ncalls)
+        best = np.zeros(2)
+        check(load().chili_batch_tune(self.h, s.ctypes.data_as(C.POINTER(C.c_uint64)), st.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                      nc.ctypes.data_as(C.POINTER(C.c_int64)), _d(best)))
+        return dict(self.shape(), forward_ms=best[0], weight_ms=best[1])
 
 
 def gemm_bf16_test(A_bits, B_bits, M, N, K, transA=False, transB=True):

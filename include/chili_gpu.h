@@ -139,6 +139,7 @@ typedef struct chili_channels chili_channels;
 typedef struct chili_mc chili_mc;
 typedef struct chili_vegas chili_vegas;
 typedef struct chili_flow chili_flow;
+typedef struct chili_batch chili_batch;
 
 const char* chili_last_error(void);
 int chili_device_count(void);
@@ -185,6 +186,26 @@ int chili_mc_points(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t
 /* Kernel timing on the context's stream (HIP events around the weight and
  * forward kernels); no reference counterpart (measurement only). */
 int chili_mc_timing(chili_mc* m, int enable, int reset, double* main_ms, int64_t* launches, int64_t* points, double* fwd_ms);
+
+/* ---- batched jobs: many independent MultiChannel::operator() iterations
+ *      (include/Channel/MultiChannel.hh:196-251), each job a chili_mc context
+ *      (same device and bin count), in one device pass per multiplicity
+ *      class with persistent kernels, captured once in a hipGraph ---- */
+int chili_batch_create(int njobs, chili_mc* const* jobs, chili_batch** out);
+int chili_batch_destroy(chili_batch* b);
+/* seed[j], stream[j]: job j's Philox stream (attempts from 0); ncalls[j]
+ * accepted points; use_graph: replay the captured pass.  Outputs stats[j],
+ * attempts[j] and, when train != 0, T[j] (host [C_j]), hist[j] (host
+ * [C_j][d_j][nbins]); the pointer arrays may be NULL. */
+int chili_batch_iterate(chili_batch* b, const uint64_t* seed, const uint32_t* stream, const int64_t* ncalls, int train, int use_graph,
+                        chili_stats* stats, int64_t* attempts, double* const* T, double* const* hist);
+/* launch shape: workgroups per CU of the persistent forward / weight kernels
+ * and points per thread (1, 2, 4) of the weight kernel; the autotuner times
+ * the candidates on one pass and keeps the fastest (best_ms: forward, weight). */
+int chili_batch_set_shape(chili_batch* b, int fwd_wpc, int main_wpc, int main_rnd);
+int chili_batch_get_shape(const chili_batch* b, int* fwd_wpc, int* main_wpc, int* main_rnd, int64_t* captures, int64_t* replays,
+                          int64_t* topups);
+int chili_batch_tune(chili_batch* b, const uint64_t* seed, const uint32_t* stream, const int64_t* ncalls, double* best_ms);
 
 /* ---- plain VEGAS (chili::Vegas, include/Integrator/Vegas.hh:55-92,
  *      src/Integrator/Vegas.cc:18-69); device integrand CHILI_F_EEMUMU ---- */
