@@ -99,11 +99,11 @@ def adapt(gpu, ora_lib, W, uni_seed):
                 time_to_1pct_s=t_1pct, rel_err_history=[float(x) for x in hist_rel])
 
 
-def measured_traffic(workload, pts_launch):
+def measured_traffic(workload, variant, pts_launch):
     """HBM bytes per k_main launch at this launch size, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), scaled by the
     counters' reading of a copy kernel of known size; None if absent."""
-    path = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}.json")
+    path = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}_{variant}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -168,6 +168,9 @@ def main():
     ap.add_argument("--flow-epochs", type=int, default=30, help="training epochs (the full schedule is 200)")
     ap.add_argument("--flow-batch", type=int, default=1 << 16)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
+    ap.add_argument("--no-specialize", action="store_true",
+                    help="keep the generic kernels (default: the autotuner compiles kernels with the channel programs as "
+                         "constants and keeps them when a production step is faster)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: adapt, run two production steps and a 1 GiB device copy (the "
                          "calibration kernel of known traffic), print nothing else")
@@ -192,6 +195,8 @@ def main():
     seed = SEED + W["cfg"]
     ad = adapt(gpu, None, W, seed)
     if args.traffic_probe:
+        if not args.no_specialize:
+            gpu.specialize(True)
         for s in range(2):
             gpu.iterate(abi.philox_uniforms(seed, stream=2000 + s), W["batch"], train=False)
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
@@ -231,6 +236,29 @@ def main():
     def plain_step(uni, vp, cap):
         st, _, _, na = gpu.iterate(uni, n, train=False, vals_ptr=vp, vals_cap=cap)
         return na
+
+    # kernel variant: generic (programs read from HBM) or specialised at run
+    # time (programs as constants); the faster production step wins
+    variant, compile_s, t_variant = "generic", None, {}
+    if not args.no_specialize:
+        def one_step(stream):
+            plain_step(abi.philox_uniforms(seed, stream=stream), None, 0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            plain_step(abi.philox_uniforms(seed, stream=stream + 1), None, 0)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t
+        t_variant["generic"] = one_step(900)
+        try:
+            compile_s = gpu.specialize(True)
+            t_variant["specialized"] = one_step(902)
+            if t_variant["specialized"] < t_variant["generic"]:
+                variant = "specialized"
+            else:
+                gpu.specialize(False)
+        except lib.ChiliError as e:
+            log(f"  specialisation unavailable: {e}")
+        log(f"  kernel variant: {variant} (step s: {t_variant}, compile {compile_s} s)")
 
     gpu.timing(enable=True, reset=True)
     plain = production(plain_step, "plain")
@@ -290,7 +318,7 @@ def main():
     pts_launch = tm["points"] / max(tm["launches"], 1)
     achieved = f_main * pts_launch / (avg_ms * 1e-3) / 1e12
     roof = dict(bound="mfma", achieved=round(achieved, 4), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / FP64_PEAK_TFLOPS, 5), traffic=measured_traffic(args.workload, pts_launch),
+                frac=round(achieved / FP64_PEAK_TFLOPS, 5), traffic=measured_traffic(args.workload, variant, pts_launch),
                 kernel="k_main (all-channel inverse maps + VEGAS densities + multichannel weight)",
                 compute="FP64 VALU; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s",
                 flop_eq_per_point=round(f_main, 1), points_per_launch=round(pts_launch, 1),
@@ -312,7 +340,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (Philox4x32-10 uniforms)",
         "config": {"workload": args.workload, "description": W["desc"], "channels": gpu.nchannels, "dims": gpu.ndims,
-                   "bins": gpu.nbins, "points_per_step": n, "selected_leg": selected},
+                   "bins": gpu.nbins, "points_per_step": n, "selected_leg": selected, "kernel_variant": variant,
+                   "specialize_compile_s": compile_s},
         "points_per_s": round(pts_s, 1), "eps_unw": eps, "eps_unw_q999": eps999, "w_max": wmax, "w_mean": wmean,
         "attempts_per_step": atts / args.steps,
         "adaptation": ad, "roofline": roof, "cpu_baseline": cpu, "plain": plain, "flow": flow,

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/chili_gpu.h"
+#include "chanprog.hpp"
 
 namespace chili_host {
 
@@ -68,24 +69,6 @@ struct ProcessModel {
 
 ProcessModel model_from_abi(const chili_process& p, CutTable& cuts);
 std::vector<Topology> construct_topologies(const ProcessModel& m, CutTable& cuts);
-
-// ---------------------------------------------------------------------------
-// Compiled channel: the FSMapper op lists as fixed-size tables (device form).
-// Momentum slots: external particle i -> slot i; the k-th decaying current of
-// the channel (ascending order) -> slot ntot + k.
-// ---------------------------------------------------------------------------
-constexpr int kMaxExt = CHILI_MAX_FLAV;
-constexpr int kMaxDec = CHILI_MAX_FLAV - 3;
-constexpr int kMaxT = CHILI_MAX_FLAV - 2;
-
-struct ChanProg {
-    int nd, nt, pad0, pad1;
-    int dec_par[kMaxDec], dec_c1[kMaxDec], dec_c2[kMaxDec], dec_massive[kMaxDec];
-    double dec_mass[kMaxDec], dec_width[kMaxDec], dec_smin2[kMaxDec], dec_smincut[kMaxDec], dec_smax[kMaxDec];
-    int t_slot[kMaxT];
-    double t_ptmin[kMaxT], t_etacut[kMaxT];
-    int s_par[kMaxDec], s_a[kMaxDec], s_b[kMaxDec], pad2;
-};
 
 struct ChannelSet {
     ProcessModel model;

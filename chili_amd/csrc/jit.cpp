@@ -1,0 +1,145 @@
+// jit.cpp -- run-time specialisation of the multichannel kernels.
+//
+// The generic kernels read each channel's program (slot indices, masses,
+// cut constants) from device memory, so the per-point momentum slots are
+// indexed at run time and live in private memory.  For one context the
+// programs are fixed, so this file generates a kernel source in which every
+// program is a compile-time constant (one constexpr function per channel),
+// compiles it with hipRTC for gfx950 against the same device headers the
+// library is built from, and loads the code object.  The arithmetic is the
+// generic kernels' statement for statement (same -ffp-contract=off build);
+// the compiler can then keep the slots in registers and fold the constants.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <type_traits>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "chanprog.hpp"
+#include "common.hpp"
+#include "jit.hpp"
+
+// Header texts embedded at build time (chili_amd/build.py writes
+// jit_headers.inc): { include name, text } pairs.
+#include "jit_headers.inc"
+
+namespace chili_jit {
+
+using chili_host::ChanProg;
+using chili_host::kMaxDec;
+using chili_host::kMaxT;
+
+namespace {
+
+std::string lit(double v) {
+    if (std::isnan(v)) return "__builtin_nan(\"\")";
+    if (std::isinf(v)) return v > 0 ? "__builtin_inf()" : "(-__builtin_inf())";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%a", v);
+    return buf;
+}
+template <class T, size_t N>
+std::string arr(const T (&a)[N]) {
+    std::ostringstream o;
+    o << "{";
+    for (size_t i = 0; i < N; ++i) {
+        if (i) o << ", ";
+        if constexpr (std::is_floating_point<T>::value) o << lit(a[i]);
+        else o << a[i];
+    }
+    o << "}";
+    return o.str();
+}
+
+}  // namespace
+
+std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
+    std::ostringstream s;
+    s << "// generated: channel programs as compile-time constants\n"
+      << "#include \"mc_device.hpp\"\n"
+      << "using namespace chili_dev;\n"
+      << "using chili_host::ChanProg;\n"
+      << "namespace {\n"
+      << "constexpr int NT = " << NT << ";\n";
+    for (size_t j = 0; j < progs.size(); ++j) {
+        const ChanProg& p = progs[j];
+        s << "__host__ __device__ constexpr ChanProg chili_prog_" << j << "() { return ChanProg{" << p.nd << ", " << p.nt << ", 0, 0, "
+          << arr(p.dec_par) << ", " << arr(p.dec_c1) << ", " << arr(p.dec_c2) << ", " << arr(p.dec_massive) << ", "
+          << arr(p.dec_mass) << ", " << arr(p.dec_width) << ", " << arr(p.dec_smin2) << ", " << arr(p.dec_smincut) << ", "
+          << arr(p.dec_smax) << ", " << arr(p.t_slot) << ", " << arr(p.t_ptmin) << ", " << arr(p.t_etacut) << ", "
+          << arr(p.s_par) << ", " << arr(p.s_a) << ", " << arr(p.s_b) << ", 0}; }\n";
+    }
+    s << "struct JitLoop {\n"
+      << "    template <class F>\n"
+      << "    __device__ __forceinline__ static void each(const MCArgs& A, F&& f) {\n";
+    for (size_t j = 0; j < progs.size(); ++j) s << "        f(" << j << ", chili_prog_" << j << "());\n";
+    s << "    }\n};\n}  // namespace\n\n";
+    s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_main(MCArgs A, const int* __restrict__ seg2,\n"
+         "        const int* __restrict__ tile2, const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,\n"
+         "        double* __restrict__ psbuf, int64_t psstride, int train, double* __restrict__ part, int64_t rec,\n"
+         "        double* __restrict__ vals_out, int64_t vals_off, int64_t vals_cap, const uint32_t* __restrict__ sorted, PointRec R) {\n"
+         "    extern __shared__ double s_mem[];\n"
+         "    int ch, e0, e1;\n"
+         "    if (!locate_tile(A.C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;\n"
+         "    main_tile<NT, kRounds, JitLoop>(A, ch, e0, e1, s_mem, blockIdx.x, psbuf, psstride, train, comp, mom, nmax,\n"
+         "                                    part + (int64_t)blockIdx.x * rec, vals_out, vals_off, vals_cap, sorted, R);\n"
+         "}\n\n";
+    s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward(MCArgs A, const int* __restrict__ seg_start,\n"
+         "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"
+         "        uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {\n"
+         "    int ch, e0, e1;\n"
+         "    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;\n"
+         "    const int e = e0 + threadIdx.x;\n"
+         "    if (e >= e1) return;\n"
+         "    const uint32_t a = sorted[e];\n"
+         "    switch (ch) {\n";
+    for (size_t j = 0; j < progs.size(); ++j)
+        s << "        case " << j << ": forward_one<NT, false>(A, chili_prog_" << j << "(), " << j
+          << ", e, a, A.a0 + a, mom, nmax, acc, R, zsrc); break;\n";
+    s << "        default: break;\n    }\n}\n";
+    return s.str();
+}
+
+Module::~Module() {
+    if (mod) (void)hipModuleUnload(mod);
+}
+
+void compile(const std::string& src, Module& out, double* seconds, bool load) {
+    std::vector<const char*> hdr, names;
+    for (int i = 0; i < kJitHeaderCount; ++i) {
+        names.push_back(kJitHeaders[i].name);
+        hdr.push_back(kJitHeaders[i].text);
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "chili_specialised.hip", (int)hdr.size(), hdr.data(), names.data()) != HIPRTC_SUCCESS)
+        throw chili_error(CHILI_EHIP, "hiprtcCreateProgram failed");
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const auto t0 = std::chrono::steady_clock::now();
+    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        throw chili_error(CHILI_EHIP, "run-time compilation failed: " + log.substr(0, 2000));
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    std::vector<char> code(n);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    if (!load) return;
+    HIP_CHECK(hipModuleLoadData(&out.mod, code.data()));
+    HIP_CHECK(hipModuleGetFunction(&out.main, out.mod, "chili_k_main"));
+    HIP_CHECK(hipModuleGetFunction(&out.forward, out.mod, "chili_k_forward"));
+}
+
+}  // namespace chili_jit

@@ -15,11 +15,13 @@
 //                 histogram partials per workgroup in a fixed order
 //   k_reduce      fixed-order sum of the workgroup partials
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include "../../include/chili_gpu.h"
-#include "channels.hpp"
+#include "chanprog.hpp"
 #include "phys.hpp"
 
 namespace chili_dev {

@@ -5,7 +5,9 @@
 // (the kernels are compiled with -ffp-contract=off).  Citations are file:line
 // in the jxi24/Chili tree.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #define CHILI_DEV __device__ __forceinline__

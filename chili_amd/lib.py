@@ -26,7 +26,7 @@ EXPORTS = [
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
     "chili_flow_train", "chili_gemm_bf16_test",
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
-    "chili_batch_tune",
+    "chili_batch_tune", "chili_mc_specialize", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
 
 
@@ -90,6 +90,9 @@ def load():
     L.chili_flow_train.argtypes = [P, I64, VP, VP, VP, I64, C.c_int, C.c_double, U64, D]
     L.chili_gemm_bf16_test.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), F, C.c_int,
                                        C.c_int]
+    L.chili_mc_specialize.argtypes = [P, C.c_int, D]
+    L.chili_channels_kernel_source.argtypes = [P, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)]
+    L.chili_channels_compile_check.argtypes = [P, D]
     PP = C.POINTER(C.c_void_p)
     L.chili_batch_create.argtypes = [C.c_int, PP, C.POINTER(P)]
     L.chili_batch_destroy.argtypes = [P]
@@ -131,6 +134,21 @@ class Channels:
         self.count = L.chili_channels_count(h)
         self.ndims = L.chili_channels_ndims(h)
         self.ntot = proc.nflav
+
+    def kernel_source(self):
+        """The specialised kernel source generated for these channels."""
+        n = C.c_int64()
+        check(load().chili_channels_kernel_source(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(load().chili_channels_kernel_source(self.h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
+    def compile_check(self):
+        """Compile the specialised kernels at run time without loading them
+        (no GPU needed); returns the compile time in seconds."""
+        t = C.c_double()
+        check(load().chili_channels_compile_check(self.h, C.byref(t)))
+        return t.value
 
     def desc(self, i):
         d = abi.ChannelDesc()
@@ -196,6 +214,13 @@ class MultiChannelGPU:
         check(load().chili_mc_iterate(self.h, C.byref(uni), a0, ncalls, 1 if train else 0, C.byref(st), _d(T), _d(hist),
                                       C.byref(na), vals_ptr, vals_cap))
         return st, T, hist, na.value
+
+    def specialize(self, enable=True):
+        """Switch to kernels compiled at run time with this context's channel
+        programs as constants (chili_mc_specialize); returns the compile time."""
+        t = C.c_double()
+        check(load().chili_mc_specialize(self.h, 1 if enable else 0, C.byref(t)))
+        return t.value
 
     def timing(self, enable=True, reset=False):
         ms, fms = C.c_double(), C.c_double()

@@ -28,7 +28,9 @@
 #ifndef CHILI_GPU_H
 #define CHILI_GPU_H
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
+#endif
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -183,6 +185,15 @@ int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_
  * ps[C][n], x[d][n] (picked channel's inverse point), bins[d][n], mom[ntot*4][n]. */
 int chili_mc_points(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n, int32_t* status, int32_t* pick, double* w,
                     double* f, double* val, double* ps, double* x, int32_t* bins, double* mom);
+/* Run-time specialisation (no reference counterpart): compile the forward
+ * and weight kernels with this context's channel programs as compile-time
+ * constants (hipRTC, gfx950) and use them (enable != 0), or return to the
+ * generic kernels; compile_s receives the compile time.  Same results. */
+int chili_mc_specialize(chili_mc* m, int enable, double* compile_s);
+/* The generated specialised kernel source of a channel set (diagnostics),
+ * and its run-time compilation without loading it (no GPU needed). */
+int chili_channels_kernel_source(const chili_channels* ch, char* buf, int64_t cap, int64_t* len);
+int chili_channels_compile_check(const chili_channels* ch, double* compile_s);
 /* Kernel timing on the context's stream (HIP events around the weight and
  * forward kernels); no reference counterpart (measurement only). */
 int chili_mc_timing(chili_mc* m, int enable, int reset, double* main_ms, int64_t* launches, int64_t* points, double* fwd_ms);

@@ -48,3 +48,13 @@ def test_errors_cross_the_boundary_as_codes():
     rc = L.chili_grid_adapt(0, 0, None, 1.0, None)
     assert rc != 0
     assert len(L.chili_last_error()) > 0
+
+
+def test_specialized_kernel_source_compiles():
+    # the run-time specialisation compiles for gfx950 (hipRTC, no GPU needed)
+    from chili_amd import processes as P
+    ch = lib.Channels(P.gluon_process(3))
+    src = ch.kernel_source()
+    assert "chili_prog_0" in src and "chili_k_main" in src
+    secs = ch.compile_check()
+    assert secs > 0

@@ -116,6 +116,42 @@ def test_points_parity(case, grids):
     assert np.median(d) <= 1e-10 and d.max() <= 1e-6
 
 
+@pytest.mark.parametrize("case", ["gg_2g", "gg_3g", "tt_6j"])
+def test_specialized_kernels_parity(case):
+    """Kernels compiled at run time with the channel programs as constants
+    give the generic kernels' results (and so the oracle's)."""
+    proc, f = CASES[case]()
+    ch = lib.Channels(proc)
+    B = 50
+    gpu = lib.MultiChannelGPU(ch, B, f)
+    ora = O.Oracle(proc, B, f)
+    rng = np.random.default_rng(11)
+    g = random_grids(rng, gpu.nchannels, gpu.ndims, B)
+    a = random_alpha(rng, gpu.nchannels)
+    gpu.set_grids(g); ora.set_grids(g)
+    gpu.set_alpha(a); ora.set_alpha(a)
+    n = 3000
+    uni = abi.philox_uniforms(SEED + 5, stream=4)
+    generic = gpu.points(uni, 3, n)
+    secs = gpu.specialize(True)
+    spec = gpu.points(uni, 3, n)
+    print(f"{case}: compiled in {secs:.1f} s")
+    assert np.array_equal(generic["status"], spec["status"]) and np.array_equal(generic["pick"], spec["pick"])
+    ok = generic["status"] == 1
+    assert np.array_equal(generic["bins"][ok], spec["bins"][ok])
+    assert_rel(spec["mom"], generic["mom"], 1e-12, scale=np.nanmax(np.abs(generic["mom"])), what=f"{case} momenta")
+    assert_rel(spec["w"][ok], generic["w"][ok], 1e-12, what=f"{case} weights")
+    staged = ora.points(uni, 3, n, mom_in=spec["mom"])
+    compare_weights(spec, staged, f"{case}/specialized")
+    # iterate with training through the specialised kernels
+    st1, T1, h1, n1 = gpu.iterate(uni, 5000, a0=0, train=True)
+    gpu.specialize(False)
+    st0, T0, h0, n0 = gpu.iterate(uni, 5000, a0=0, train=True)
+    assert n1 == n0 and st1.n == st0.n
+    assert_rel(np.array([st1.sum, st1.sum2]), np.array([st0.sum, st0.sum2]), 1e-12, what=f"{case} iterate sums")
+    assert_rel(h1, h0, 1e-12, scale=max(np.abs(h0).max(), 1e-300), what=f"{case} histograms")
+
+
 def test_points_host_uniforms():
     proc, f = CASES["gg_4g"]()
     ch = lib.Channels(proc)
