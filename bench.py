@@ -103,7 +103,8 @@ def measured_traffic(workload, variant, pts_launch):
     """HBM bytes per k_main launch at this launch size, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), scaled by the
     counters' reading of a copy kernel of known size; None if absent."""
-    path = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}_{variant}.json")
+    kind = "specialized" if variant in ("specialized", "specialized-weight") else "generic"
+    path = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}_{kind}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -249,14 +250,15 @@ def main():
             torch.cuda.synchronize()
             return time.perf_counter() - t
         t_variant["generic"] = one_step(900)
+        names = {1: "specialized-forward", 2: "specialized-weight", 3: "specialized"}
         try:
-            compile_s = gpu.specialize(True)
-            t_variant["specialized"] = one_step(902)
-            if t_variant["specialized"] < t_variant["generic"]:
-                variant = "specialized"
-            else:
-                gpu.specialize(False)
+            for mode in (1, 2, 3):
+                compile_s = gpu.specialize(mode)
+                t_variant[names[mode]] = one_step(902 + 2 * mode)
+            variant = min(t_variant, key=t_variant.get)
+            gpu.specialize({v: k for k, v in names.items()}.get(variant, 0))
         except lib.ChiliError as e:
+            gpu.specialize(False)
             log(f"  specialisation unavailable: {e}")
         log(f"  kernel variant: {variant} (step s: {t_variant}, compile {compile_s} s)")
 

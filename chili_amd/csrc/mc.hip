@@ -518,7 +518,7 @@ struct chili_mc {
     double acc_rate = 0.25;      // running estimate of the cut acceptance
     // run-time specialised forward / weight kernels (channel programs as constants)
     std::unique_ptr<chili_jit::Module> jit;
-    bool jit_on = false;
+    bool jit_fwd = false, jit_main = false;
     double jit_compile_s = 0;
     // k_main timing with HIP events on the context's stream
     bool timing = false;
@@ -621,7 +621,7 @@ MCArgs make_args(chili_mc* m, const double* u, int64_t ustride, uint64_t seed, u
 
 template <int NT>
 void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, const double* z = nullptr) {
-    if (m->jit_on && !z) {
+    if (m->jit_fwd && !z) {
         MCArgs a = A;
         const int* seg = m->d_seg.ptr;
         const int* tile = m->d_tiles.ptr;
@@ -649,7 +649,7 @@ void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals
     const size_t lds = sizeof(double) * (D * m->B + kWaves * (8 + m->C)) + sizeof(short) * D * kTile;
     if (lds > 160 * 1024) throw chili_error(CHILI_ERANGE, "channel count x bins exceed the LDS of one workgroup");
     const int64_t rec = kStatW + m->C + (int64_t)m->d * m->B;
-    if (m->jit_on) {
+    if (m->jit_main) {
         MCArgs a = A;
         const int* seg2 = m->d_seg2.ptr;
         const int* tile2 = m->d_tiles2.ptr;
@@ -2009,9 +2009,10 @@ int chili_batch_tune(chili_batch* b, const uint64_t* seed, const uint32_t* strea
 // ---------------------------------------------------------------------------
 // Run-time specialisation (no reference counterpart): compile the forward and
 // weight kernels of this context with its channel programs as compile-time
-// constants (hipRTC, gfx950) and use them from now on (enable != 0), or go
-// back to the generic kernels (enable == 0).  compile_s (optional) receives
-// the compile time of the first enable.
+// constants (hipRTC, gfx950) and use them from now on: enable bit 0 selects
+// the specialised forward kernel, bit 1 the specialised weight kernel (0:
+// the generic kernels).  compile_s (optional) receives the compile time of
+// the first enable.
 int chili_mc_specialize(chili_mc* m, int enable, double* compile_s) {
     return guarded([&] {
         if (!m) throw chili_error(CHILI_EINVAL, "null context");
@@ -2021,7 +2022,8 @@ int chili_mc_specialize(chili_mc* m, int enable, double* compile_s) {
             chili_jit::compile(chili_jit::kernel_source(m->set.progs, m->NT), *mod, &m->jit_compile_s);
             m->jit = std::move(mod);
         }
-        m->jit_on = enable != 0;
+        m->jit_fwd = (enable & 1) != 0;
+        m->jit_main = (enable & 2) != 0;
         if (compile_s) *compile_s = m->jit_compile_s;
     });
 }

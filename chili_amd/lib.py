@@ -217,9 +217,12 @@ class MultiChannelGPU:
 
     def specialize(self, enable=True):
         """Switch to kernels compiled at run time with this context's channel
-        programs as constants (chili_mc_specialize); returns the compile time."""
+        programs as constants (chili_mc_specialize): True / 3 both kernels, 1
+        the forward kernel, 2 the weight kernel, False / 0 the generic ones.
+        Returns the compile time."""
+        mode = 3 if enable is True else (0 if enable is False else int(enable))
         t = C.c_double()
-        check(load().chili_mc_specialize(self.h, 1 if enable else 0, C.byref(t)))
+        check(load().chili_mc_specialize(self.h, mode, C.byref(t)))
         return t.value
 
     def timing(self, enable=True, reset=False):

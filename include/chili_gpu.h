@@ -187,8 +187,9 @@ int chili_mc_points(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t
                     double* f, double* val, double* ps, double* x, int32_t* bins, double* mom);
 /* Run-time specialisation (no reference counterpart): compile the forward
  * and weight kernels with this context's channel programs as compile-time
- * constants (hipRTC, gfx950) and use them (enable != 0), or return to the
- * generic kernels; compile_s receives the compile time.  Same results. */
+ * constants (hipRTC, gfx950); enable bit 0 uses the specialised forward
+ * kernel, bit 1 the specialised weight kernel, 0 the generic ones;
+ * compile_s receives the compile time.  Same results. */
 int chili_mc_specialize(chili_mc* m, int enable, double* compile_s);
 /* The generated specialised kernel source of a channel set (diagnostics),
  * and its run-time compilation without loading it (no GPU needed). */

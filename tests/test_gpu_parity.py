@@ -143,6 +143,13 @@ def test_specialized_kernels_parity(case):
     assert_rel(spec["w"][ok], generic["w"][ok], 1e-12, what=f"{case} weights")
     staged = ora.points(uni, 3, n, mom_in=spec["mom"])
     compare_weights(spec, staged, f"{case}/specialized")
+    # one kernel specialised, the other generic
+    for mode in (1, 2):
+        gpu.specialize(mode)
+        mixed = gpu.points(uni, 3, n)
+        assert np.array_equal(generic["status"], mixed["status"]), f"{case} mode {mode} status"
+        assert_rel(mixed["w"][ok], generic["w"][ok], 1e-12, what=f"{case} mode {mode} weights")
+    gpu.specialize(True)
     # iterate with training through the specialised kernels
     st1, T1, h1, n1 = gpu.iterate(uni, 5000, a0=0, train=True)
     gpu.specialize(False)
