@@ -79,16 +79,18 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
       << "    __device__ __forceinline__ static void each(const MCArgs& A, F&& f) {\n";
     for (size_t j = 0; j < progs.size(); ++j) s << "        f(" << j << ", chili_prog_" << j << "());\n";
     s << "    }\n};\n}  // namespace\n\n";
-    s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_main(MCArgs A, const int* __restrict__ seg2,\n"
-         "        const int* __restrict__ tile2, const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,\n"
-         "        double* __restrict__ psbuf, int64_t psstride, int train, double* __restrict__ part, int64_t rec,\n"
-         "        double* __restrict__ vals_out, int64_t vals_off, int64_t vals_cap, const uint32_t* __restrict__ sorted, PointRec R) {\n"
-         "    extern __shared__ double s_mem[];\n"
-         "    int ch, e0, e1;\n"
-         "    if (!locate_tile(A.C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;\n"
-         "    main_tile<NT, kRounds, JitLoop>(A, ch, e0, e1, s_mem, blockIdx.x, psbuf, psstride, train, comp, mom, nmax,\n"
-         "                                    part + (int64_t)blockIdx.x * rec, vals_out, vals_off, vals_cap, sorted, R);\n"
-         "}\n\n";
+    for (int ls = 0; ls < 2; ++ls)
+        s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_main" << (ls ? "_lds" : "")
+          << "(MCArgs A, const int* __restrict__ seg2,\n"
+             "        const int* __restrict__ tile2, const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,\n"
+             "        double* __restrict__ psbuf, int64_t psstride, int train, double* __restrict__ part, int64_t rec,\n"
+             "        double* __restrict__ vals_out, int64_t vals_off, int64_t vals_cap, const uint32_t* __restrict__ sorted, PointRec R) {\n"
+             "    extern __shared__ double s_mem[];\n"
+             "    int ch, e0, e1;\n"
+             "    if (!locate_tile(A.C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;\n"
+             "    main_tile<NT, kRounds, JitLoop, " << (ls ? "true" : "false") << ">(A, ch, e0, e1, s_mem, blockIdx.x, psbuf, psstride,\n"
+             "        train, comp, mom, nmax, part + (int64_t)blockIdx.x * rec, vals_out, vals_off, vals_cap, sorted, R);\n"
+             "}\n\n";
     s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward(MCArgs A, const int* __restrict__ seg_start,\n"
          "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"
          "        uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {\n"
@@ -139,6 +141,7 @@ void compile(const std::string& src, Module& out, double* seconds, bool load) {
     if (!load) return;
     HIP_CHECK(hipModuleLoadData(&out.mod, code.data()));
     HIP_CHECK(hipModuleGetFunction(&out.main, out.mod, "chili_k_main"));
+    HIP_CHECK(hipModuleGetFunction(&out.main_lds, out.mod, "chili_k_main_lds"));
     HIP_CHECK(hipModuleGetFunction(&out.forward, out.mod, "chili_k_forward"));
 }
 

@@ -11,7 +11,7 @@ namespace chili_jit {
 
 struct Module {
     hipModule_t mod = nullptr;
-    hipFunction_t main = nullptr, forward = nullptr;
+    hipFunction_t main = nullptr, main_lds = nullptr, forward = nullptr;
     Module() = default;
     Module(const Module&) = delete;
     Module& operator=(const Module&) = delete;

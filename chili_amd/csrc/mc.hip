@@ -248,7 +248,7 @@ __global__ void k_keep_segments(int C, const int* __restrict__ tile_start, const
     tile2[C] = t;
 }
 
-template <int NT>
+template <int NT, bool LS>
 __global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict__ seg2, const int* __restrict__ tile2,
                                                 const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
                                                 double* __restrict__ psbuf, int64_t psstride, int train,
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict_
     extern __shared__ double s_mem[];
     int ch, e0, e1;
     if (!locate_tile(A.C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;
-    main_tile<NT, kRounds, GenericLoop>(A, ch, e0, e1, s_mem, blockIdx.x, psbuf, psstride, train, comp, mom, nmax, part + (int64_t)blockIdx.x * rec,
+    main_tile<NT, kRounds, GenericLoop, LS>(A, ch, e0, e1, s_mem, blockIdx.x, psbuf, psstride, train, comp, mom, nmax, part + (int64_t)blockIdx.x * rec,
                            vals_out, vals_off, vals_cap, sorted, R);
 }
 
@@ -519,6 +519,7 @@ struct chili_mc {
     // run-time specialised forward / weight kernels (channel programs as constants)
     std::unique_ptr<chili_jit::Module> jit;
     bool jit_fwd = false, jit_main = false;
+    bool lds_search = true;      // weight kernel: search tables in LDS when they fit
     double jit_compile_s = 0;
     // k_main timing with HIP events on the context's stream
     bool timing = false;
@@ -646,8 +647,12 @@ template <int NT>
 void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals, int64_t vals_off, int64_t vals_cap,
                  const PointRec& R) {
     constexpr int D = 3 * (NT - 2) - 2;
-    const size_t lds = sizeof(double) * (D * m->B + kWaves * (8 + m->C)) + sizeof(short) * D * kTile;
+    size_t lds = sizeof(double) * (D * m->B + kWaves * (8 + m->C)) + sizeof(short) * D * kTile;
     if (lds > 160 * 1024) throw chili_error(CHILI_ERANGE, "channel count x bins exceed the LDS of one workgroup");
+    // small search tables go to LDS with the rest of the workgroup's state
+    const size_t srch_bytes = sizeof(double) * (size_t)m->C * D * m->P2;
+    const bool ls = m->lds_search && srch_bytes <= kLdsSearchMax;
+    if (ls) lds += srch_bytes;
     const int64_t rec = kStatW + m->C + (int64_t)m->d * m->B;
     if (m->jit_main) {
         MCArgs a = A;
@@ -666,12 +671,18 @@ void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals
         const uint32_t* sorted = m->d_sorted.ptr;
         PointRec r = R;
         void* args[] = {&a, &seg2, &tile2, &comp, &mom, &nmax, &ps, &psstride, &tr, &part, &rc, &v, &voff, &vcap, &sorted, &r};
-        HIP_CHECK(hipModuleLaunchKernel(m->jit->main, grid, 1, 1, kTile, 1, 1, (unsigned)lds, m->stream, args, nullptr));
+        HIP_CHECK(hipModuleLaunchKernel(ls ? m->jit->main_lds : m->jit->main, grid, 1, 1, kTile, 1, 1, (unsigned)lds, m->stream,
+                                        args, nullptr));
         return;
     }
-    hipLaunchKernelGGL(k_main<NT>, dim3(grid), dim3(kTile), lds, m->stream, A, m->d_seg2.ptr, m->d_tiles2.ptr, m->d_comp.ptr,
-                       m->d_mom.ptr, m->nmax, m->d_ps.ptr, (int64_t)grid * kTile, train, m->d_part.ptr, rec, vals, vals_off, vals_cap,
-                       m->d_sorted.ptr, R);
+    if (ls)
+        hipLaunchKernelGGL((k_main<NT, true>), dim3(grid), dim3(kTile), lds, m->stream, A, m->d_seg2.ptr, m->d_tiles2.ptr, m->d_comp.ptr,
+                           m->d_mom.ptr, m->nmax, m->d_ps.ptr, (int64_t)grid * kTile, train, m->d_part.ptr, rec, vals, vals_off,
+                           vals_cap, m->d_sorted.ptr, R);
+    else
+        hipLaunchKernelGGL((k_main<NT, false>), dim3(grid), dim3(kTile), lds, m->stream, A, m->d_seg2.ptr, m->d_tiles2.ptr, m->d_comp.ptr,
+                           m->d_mom.ptr, m->nmax, m->d_ps.ptr, (int64_t)grid * kTile, train, m->d_part.ptr, rec, vals, vals_off,
+                           vals_cap, m->d_sorted.ptr, R);
 }
 
 #define CHILI_DISPATCH_NT(NTV, CALL)           \

@@ -122,22 +122,29 @@ __device__ __forceinline__ void forward_one(const MCArgs& A, const ChanProg& p, 
 // ---------------------------------------------------------------------------
 constexpr int kRounds = 1;                 // sub-tiles of kTile points per k_main workgroup
 constexpr int kMainTile = kTile * kRounds;
+constexpr size_t kLdsSearchMax = 32 * 1024;   // largest search-table set staged in LDS (bytes)
 
 // One kept tile [e0, e1) of channel ch (RND sub-tiles of kTile points):
 // psbuf column slot * kTile + thread, partial record `out`.  Every thread of
-// the workgroup calls it (it synchronises the workgroup).
-template <int NT, int RND, class Loop>
+// the workgroup calls it (it synchronises the workgroup).  LS: the search
+// tables of all channels ([C][d][P2], a few KB) are first copied into LDS, so
+// that the dependent loads of the bin searches are LDS reads.
+template <int NT, int RND, class Loop, bool LS = false>
 __device__ __forceinline__ void main_tile(const MCArgs& A, int ch, int e0, int e1, double* s_mem, int64_t slot,
                                           double* __restrict__ psbuf, int64_t psstride, int train, const uint32_t* __restrict__ comp,
                                           const double* __restrict__ mom, int64_t nmax, double* __restrict__ out,
                                           double* __restrict__ vals_out, int64_t vals_off, int64_t vals_cap,
                                           const uint32_t* __restrict__ sorted, const PointRec& R) {
     constexpr int D = 3 * (NT - 2) - 2;
-    // LDS: hist [D][B], per-wave sums [kWaves][8 + C], bins [D][kTile] shorts
+    // LDS: hist [D][B], per-wave sums [kWaves][8 + C], bins [D][kTile] shorts,
+    // with LS the search tables [C][D][P2]
     const int B = A.B, C = A.C;
     double* s_hist = s_mem;
     double* s_small = s_hist + D * B;
     short* s_bins = (short*)(s_small + kWaves * (8 + C));
+    double* s_srch = (double*)(s_bins + D * kTile);
+    if (LS)
+        for (int i = threadIdx.x; i < C * D * A.P2; i += kTile) s_srch[i] = A.search[i];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t gt = slot * kTile + threadIdx.x;
     __syncthreads();                     // a previous tile's record may still be reading LDS
@@ -176,7 +183,8 @@ __device__ __forceinline__ void main_tile(const MCArgs& A, int ch, int e0, int e
             double vw;
             short* bo = (j == ch) ? (s_bins + threadIdx.x) : nullptr;
             const bool recx = R.status && valid && j == ch;
-            double ps = generate_weight<NT>(A, pj, j, P, &vw, bo, kTile, recx ? R.x + att : nullptr, recx ? R.bins + att : nullptr, R.n);
+            double ps = generate_weight<NT>(A, pj, j, P, &vw, bo, kTile, recx ? R.x + att : nullptr, recx ? R.bins + att : nullptr, R.n,
+                                            nullptr, LS ? s_srch : nullptr);
             wsum += A.alpha[j] / ps / vw;
             psbuf[(int64_t)j * psstride + gt] = ps;
             if (R.status && valid) R.ps[(int64_t)j * R.n + att] = ps;

@@ -216,13 +216,14 @@ CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* 
 template <int NT>
 CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, MomRF<2 * NT>& P, double* vw_out,
                                  short* bins_out, int bstride, double* x_out = nullptr, int32_t* bin_rec = nullptr,
-                                 int64_t xstride = 0, double* r_out = nullptr) {
+                                 int64_t xstride = 0, double* r_out = nullptr, const double* search = nullptr) {
     constexpr int ND = NT - 3;
     constexpr int NTT = NT - 2;
     const int nd = p.nd, nt = p.nt;
     const double Bd = (double)A.B;
     const double* __restrict__ edges = A.edges + (int64_t)ch * A.d * (A.B + 1);
-    const double* __restrict__ srch = A.search + (int64_t)ch * A.d * A.P2;
+    // search tables of every channel: device memory, or a workgroup's LDS copy
+    const double* __restrict__ srch = (search ? search : A.search) + (int64_t)ch * A.d * A.P2;
     double vw = 1.0;
     int slot = 0;
     auto take = [&](double x) {
