@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--flow-epochs", type=int, default=30, help="training epochs (the full schedule is 200)")
     ap.add_argument("--flow-batch", type=int, default=1 << 16)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
+    ap.add_argument("--flow-mix", default="0,0.3,0.5,0.7",
+                    help="defensive mixture weights of the uniform density to run (comma list; the best is reported)")
     ap.add_argument("--no-specialize", action="store_true",
                     help="keep the generic kernels (default: the autotuner compiles kernels with the channel programs as "
                          "constants and keeps them when a production step is faster)")
@@ -299,7 +301,17 @@ def main():
             st, na = gpu.flow_iterate(fl, uni, n, vals_ptr=vp, vals_cap=cap_)
             return na
 
-        flow = production(flow_step, "flow")
+        # proposal (1 - lam) q + lam: the uniform component bounds every
+        # weight by 1/lam times the plain one; the best lam is reported
+        scan = {}
+        for lam in [float(v) for v in args.flow_mix.split(",")]:
+            fl.set_mixture(lam)
+            scan[lam] = production(flow_step, f"flow mix {lam:g}")
+        lam_best = max(scan, key=lambda k: scan[k]["unweighted_per_s"])
+        flow = scan[lam_best]
+        flow.update(dict(mixture=lam_best, mixture_scan={f"{k:g}": dict(unweighted_per_s=v["unweighted_per_s"], eps_unw=v["eps_unw"],
+                                                                          w_rel_std=round(v["w_rel_std"], 4))
+                                                          for k, v in scan.items()}))
         flow.update(dict(layers=W["flow_layers"], hidden="3x64", spline_bins=16, train_points=int(nk), epochs=args.flow_epochs,
                          batch=args.flow_batch, lr=args.flow_lr, train_s=round(t_train, 2),
                          data_s=round(t1 - t0, 3), loss_first=losses[0] if losses else None,

@@ -668,6 +668,14 @@ int chili_flow_info(const chili_flow* f, int64_t* nparams, int* ndims, int* nlay
     });
 }
 
+int chili_flow_set_mixture(chili_flow* f, double lambda) {
+    return guarded([&] {
+        if (!f) throw chili_error(CHILI_EINVAL, "null flow");
+        if (!(lambda >= 0.0 && lambda <= 1.0)) throw chili_error(CHILI_EINVAL, "mixture weight must lie in [0, 1]");
+        f->eng.mix = lambda;
+    });
+}
+
 // Coupling masks: transformed dimensions of layer l (nB entries).
 int chili_flow_mask(const chili_flow* f, int layer, int32_t* B, int* nB) {
     return guarded([&] {

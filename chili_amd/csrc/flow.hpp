@@ -43,6 +43,7 @@ struct FlowEngine {
     hipStream_t stream = nullptr;
     int d = 0, L = 0, H = 64, nflows = 0;
     float shift = 0;   // softplus offset: slope 1 at zero input
+    double mix = 0;    // proposal q' = (1 - mix) q + mix (uniform density on the hypercube)
     size_t nparams = 0;
     std::vector<Layer> layers;
     std::vector<Flow> flows;

@@ -380,6 +380,22 @@ __global__ __launch_bounds__(kTile) void k_zgen(MCArgs A, int64_t n, const uint3
     for (int k = 0; k < D; ++k) z[e * D + k] = r[k];
 }
 
+// Defensive mixture: an attempt whose mixture uniform is below `mix` keeps
+// its uniforms as the hypercube point (z[e] = the attempt's first D uniforms)
+// instead of the flow's sample.
+template <int D>
+__global__ __launch_bounds__(kTile) void k_mix_restore(MCArgs A, int64_t n, const uint32_t* __restrict__ sorted, double mix,
+                                                       double* __restrict__ z) {
+    const int64_t e = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t att = A.a0 + sorted[e];
+    if (!(mixture_uniform(A, att) < mix)) return;
+    double r[D + 1];
+    load_uniforms<D>(A, att, r);
+#pragma unroll
+    for (int k = 0; k < D; ++k) z[e * D + k] = r[k];
+}
+
 // All-channel inverse of every kept point: f, a_j = alpha_j / ps_j / vegas_j
 // ([j][stride]) and the hypercube points r_j ([j][stride][D]).
 template <int NT>
@@ -419,7 +435,7 @@ struct FlowData {
 // W = 1 / sum_j a_j q_j, value = W == 0 ? 0 : f W (NaN -> 0, counted);
 // per-workgroup StatsData partials [n, min, max, sum, sum2, nf, nnan].
 __global__ __launch_bounds__(kTile) void k_combine(int C, int d, int64_t kept, int64_t stride, const double* __restrict__ abuf,
-                                                   const double* __restrict__ lq, const double* __restrict__ fbuf,
+                                                   const double* __restrict__ lq, double mix, const double* __restrict__ fbuf,
                                                    const double* __restrict__ rall, double* __restrict__ part,
                                                    double* __restrict__ vals_out, int64_t vals_off, int64_t vals_cap, FlowData FD,
                                                    const uint32_t* __restrict__ comp, const uint32_t* __restrict__ sorted, PointRec R) {
@@ -431,7 +447,8 @@ __global__ __launch_bounds__(kTile) void k_combine(int C, int d, int64_t kept, i
     if (valid) {
         double wsum = 0;
         for (int j = 0; j < C; ++j) {
-            const double q = lq ? exp(lq[(int64_t)j * stride + k]) : 1.0;
+            // the proposal's density: the flow's, blended with the uniform one
+            const double q = lq ? (1.0 - mix) * exp(lq[(int64_t)j * stride + k]) + mix : 1.0;
             wsum += abuf[(int64_t)j * stride + k] * q;
         }
         W = 1.0 / wsum;
@@ -917,6 +934,13 @@ void launch_zgen(chili_mc* m, const MCArgs& A, int64_t n) {
                        m->d_z.ptr);
 }
 template <int NT>
+void launch_mix_restore(chili_mc* m, const MCArgs& A, int64_t n, double mix) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    if (A.u) throw chili_error(CHILI_EINVAL, "a mixture proposal needs device-generated (Philox) uniforms");
+    hipLaunchKernelGGL(k_mix_restore<D>, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, n, m->d_sorted.ptr,
+                       mix, m->d_z.ptr);
+}
+template <int NT>
 void launch_geom(chili_mc* m, const MCArgs& A, int64_t kept) {
     hipLaunchKernelGGL(k_geom<NT>, dim3((unsigned)((kept + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, kept, m->d_comp.ptr,
                        m->d_mom.ptr, m->nmax, m->d_rall.ptr, m->d_abuf.ptr, m->d_fbuf.ptr, m->fmax);
@@ -946,6 +970,7 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
         hipLaunchKernelGGL(chili_nf::k_fill<double>, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, n, 0.0,
                            m->d_lqs.ptr);
         fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr);
+        if (fl->eng.mix > 0) CHILI_DISPATCH_NT(m->NT, launch_mix_restore<NT>(m, A, n, fl->eng.mix));
     }
     CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr));
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
@@ -964,6 +989,7 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
     }
     const int cg = (int)((kept + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_combine, dim3(cg), dim3(kTile), 0, m->stream, C, d, kept, m->fmax, m->d_abuf.ptr, fl ? m->d_lq.ptr : nullptr,
+                       fl ? fl->eng.mix : 0.0,
                        m->d_fbuf.ptr, m->d_rall.ptr, m->d_fpart.ptr, vals, vals_off, vals_cap, FD, m->d_comp.ptr, m->d_sorted.ptr, R);
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, m->stream, (int64_t)cg, m->d_fpart.ptr, m->d_tot.ptr);
 }

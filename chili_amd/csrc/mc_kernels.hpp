@@ -101,6 +101,13 @@ CHILI_DEV double channel_uniform(const MCArgs& A, uint64_t att) {
     philox_pair(A.seed, A.stream, att, (uint32_t)(A.d & ~1), u0, u1);
     return (A.d & 1) ? u1 : u0;
 }
+// Uniform d+1 of an attempt's Philox stream (device-generated uniforms only):
+// the component choice of a mixture proposal.
+CHILI_DEV double mixture_uniform(const MCArgs& A, uint64_t att) {
+    double u0, u1;
+    philox_pair(A.seed, A.stream, att, (uint32_t)((A.d + 1) & ~1), u0, u1);
+    return ((A.d + 1) & 1) ? u1 : u0;
+}
 CHILI_DEV int pick_channel(const MCArgs& A, double u) {      // lower_bound(cum, u)
     int lo = 0, n = A.C;
     if (n < 2) return 0;

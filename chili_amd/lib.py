@@ -26,7 +26,7 @@ EXPORTS = [
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
     "chili_flow_train", "chili_gemm_bf16_test",
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
-    "chili_batch_tune", "chili_mc_specialize", "chili_channels_kernel_source", "chili_channels_compile_check",
+    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
 
 
@@ -83,6 +83,7 @@ def load():
     L.chili_flow_mask.argtypes = [P, C.c_int, I32, C.POINTER(C.c_int)]
     L.chili_flow_get_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_params.argtypes = [P, C.c_int, F]
+    L.chili_flow_set_mixture.argtypes = [P, C.c_double]
     L.chili_flow_get_grads.argtypes = [P, C.c_int, F]
     L.chili_flow_log_density.argtypes = [P, C.c_int, I64, VP, VP, VP]
     L.chili_flow_sample.argtypes = [P, C.c_int, I64, VP, VP]
@@ -307,6 +308,11 @@ class FlowGPU:
         p = np.zeros(self.nparams, np.float32)
         check(load().chili_flow_get_params(self.h, flow, p.ctypes.data_as(C.POINTER(C.c_float))))
         return p
+
+    def set_mixture(self, lam):
+        """Proposal (1 - lam) q + lam (uniform on the hypercube) in the
+        multichannel flow path (chili_flow_set_mixture)."""
+        check(load().chili_flow_set_mixture(self.h, float(lam)))
 
     def set_params(self, flow, p):
         p = np.ascontiguousarray(p, dtype=np.float32)

@@ -295,3 +295,74 @@ def test_flow_path_unbiased():
     m2, e2 = mean_err(st2)
     print(f"plain {m1:.6e} +- {e1:.2e}; random flows {m2:.6e} +- {e2:.2e}")
     assert abs(m1 - m2) < 4 * np.hypot(e1, e2)
+
+
+def test_flow_mixture_points_vs_oracle():
+    """Defensive mixture (1 - lam) q + lam: attempts whose uniform number d+1
+    is below lam keep their uniforms as the hypercube point, the others take
+    the flow's sample; every weight uses the mixture density."""
+    proc, f, ch, gpu = adapted_context()
+    fl, ps = make_flow(gpu.ndims, 4, gpu.nchannels, 0.03, seed=17)
+    lam = 0.4
+    fl.set_mixture(lam)
+    ora = O.Oracle(proc, gpu.nbins, f)
+    ora.set_grids(gpu.grids())
+    ora.set_alpha(gpu.alpha())
+    alpha, grids = gpu.alpha(), gpu.grids()
+    fos = FO.FlowOracle(gpu.ndims, fl.masks(), mode="gpu")
+    uni = abi.philox_uniforms(SEED + 6, stream=7)
+    n, a0, d = 1500, 23, gpu.ndims
+    g = gpu.flow_points(fl, uni, a0, n)
+    z = np.array([[O.philox_uniform(uni.seed, uni.stream, a0 + i, k) for k in range(d)] for i in range(n)])
+    um = np.array([O.philox_uniform(uni.seed, uni.stream, a0 + i, d + 1) for i in range(n)])
+    pick = np.array([O.pick_channel(alpha, O.philox_uniform(uni.seed, uni.stream, a0 + i, d)) for i in range(n)])
+    assert np.array_equal(pick, g["pick"])
+    uni_comp = um < lam
+    assert 0.3 * n < uni_comp.sum() < 0.5 * n
+    assert np.array_equal(g["r"][uni_comp], z[uni_comp])
+    for c in range(gpu.nchannels):
+        sel = (pick == c) & ~uni_comp
+        if not sel.any():
+            continue
+        with torch.no_grad():
+            ro, _ = fos.sample(torch.tensor(ps[c], dtype=torch.float64), torch.tensor(z[sel]))
+        assert np.abs(ro.numpy() - g["r"][sel]).max() < 1e-3
+    ok = np.nonzero(g["status"] == 1)[0]
+    assert len(ok) > 100
+    dWst = []
+    for i in ok:
+        c = g["pick"][i]
+        x, _ = O.grid_forward(grids[c], g["r"][i].copy())
+        mom = ora.generate_point(c, x)
+        tot = 0.0
+        for j in range(gpu.nchannels):
+            psj, xj = ora.generate_weight(j, mom)
+            rj, vwj = O.grid_inverse(grids[j], xj)
+            tot += alpha[j] * ((1 - lam) * np.exp(g["lq"][i, j]) + lam) / psj / vwj
+        dWst.append(abs(1 / tot - g["w"][i]) / abs(1 / tot))
+    dWst = np.array(dWst)
+    print(f"mixture lam={lam}: {uni_comp.sum()} uniform-component attempts; staged rel dW median {np.median(dWst):.1e} "
+          f"max {dWst.max():.1e}")
+    assert np.median(dWst) <= 1e-10 and dWst.max() <= 1e-6
+    with pytest.raises(lib.ChiliError):
+        fl.set_mixture(1.5)
+
+
+def test_flow_mixture_unbiased():
+    """The mixture proposal gives the plain path's integral (independent
+    samples, 4 combined sigma)."""
+    proc, f, ch, gpu = adapted_context()
+    fl, _ = make_flow(gpu.ndims, 4, gpu.nchannels, 0.05, seed=23)
+    fl.set_mixture(0.3)
+    n = 400000
+    st1, _, _, _ = gpu.iterate(abi.philox_uniforms(SEED + 40, stream=1), n, train=False)
+    st2, _ = gpu.flow_iterate(fl, abi.philox_uniforms(SEED + 41, stream=1), n)
+
+    def mean_err(st):
+        m = st.sum / st.n
+        return m, np.sqrt((st.sum2 / st.n - m * m) / (st.n - 1))
+
+    m1, e1 = mean_err(st1)
+    m2, e2 = mean_err(st2)
+    print(f"plain {m1:.6e} +- {e1:.2e}; mixture flows {m2:.6e} +- {e2:.2e}")
+    assert abs(m1 - m2) < 4 * np.hypot(e1, e2)
