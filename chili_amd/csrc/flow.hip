@@ -217,13 +217,13 @@ __global__ void k_swizzle(const float* __restrict__ p, int64_t pW1, int64_t pb1,
 // Work-item table of a fused pass: flow j's rows start at start[j] (seg[j],
 // or j * stride) and number count[j] (seg[j+1] - seg[j], or ucount); items[]
 // prefix-sums ceil(count / chunk).
-__global__ void k_items(int nfl, const int* __restrict__ seg, int64_t ucount, int64_t stride, int64_t chunk, int64_t* start,
-                        int64_t* count, int* items) {
+__global__ void k_items(int nfl, const int* __restrict__ seg, const int* __restrict__ seg_end, int64_t ucount, int64_t stride,
+                        int64_t chunk, int64_t* start, int64_t* count, int* items) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int acc = 0;
     for (int j = 0; j < nfl; ++j) {
         const int64_t s0 = seg ? (int64_t)seg[j] : (int64_t)j * stride;
-        const int64_t c = seg ? (int64_t)seg[j + 1] - seg[j] : ucount;
+        const int64_t c = seg ? (int64_t)(seg_end ? seg_end[j] : seg[j + 1]) - seg[j] : ucount;
         start[j] = s0;
         count[j] = c;
         items[j] = acc;
@@ -512,7 +512,7 @@ void FlowEngine::launch_couple(bool inverse, int npair, dim3 g, dim3 b, size_t l
 }
 
 void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, int64_t ucount, int64_t stride, int64_t rows_bound,
-                            double* x, double* lq) {
+                            double* x, double* lq, const int* seg_end) {
     d_start.resize(nfl);
     d_count.resize(nfl);
     d_items.resize(nfl + 1);
@@ -520,7 +520,8 @@ void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, in
     // load, small enough to give every CU work on small passes
     int64_t chunk = ((rows_bound / 512 + 255) / 256) * 256;
     chunk = std::max<int64_t>(256, std::min<int64_t>(kChunk, chunk));
-    hipLaunchKernelGGL(k_items, dim3(1), dim3(1), 0, stream, nfl, seg, ucount, stride, chunk, d_start.ptr, d_count.ptr, d_items.ptr);
+    hipLaunchKernelGGL(k_items, dim3(1), dim3(1), 0, stream, nfl, seg, seg_end, ucount, stride, chunk, d_start.ptr, d_count.ptr,
+                       d_items.ptr);
     const unsigned grid = (unsigned)(rows_bound / chunk + nfl + 1);
     for (int q = 0; q < L; ++q) {
         const int l = inverse ? L - 1 - q : q;

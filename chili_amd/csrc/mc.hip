@@ -154,6 +154,44 @@ __global__ void k_segments(int C, const int* __restrict__ offs, int nblk, int64_
     tile_start[C] = t;
 }
 
+// Mixture proposal: bucket 2c + u of every attempt (c: channel, u = 1 when
+// its mixture uniform is below mix, the uniform component), so that after
+// the counting sort each channel's segment holds its flow attempts first.
+__global__ __launch_bounds__(kTile) void k_pick_mix(MCArgs A, int64_t n, double mix, uint16_t* __restrict__ pick,
+                                                    int* __restrict__ counts, int nblk) {
+    extern __shared__ int s_cnt[];
+    for (int c = threadIdx.x; c < 2 * A.C; c += blockDim.x) s_cnt[c] = 0;
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (i < n) {
+        const uint64_t att = A.a0 + (uint64_t)i;
+        const int c = pick_channel(A, channel_uniform(A, att));
+        const int b = 2 * c + (mixture_uniform(A, att) < mix ? 1 : 0);
+        pick[i] = (uint16_t)b;
+        atomicAdd(&s_cnt[b], 1);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < 2 * A.C; c += blockDim.x) counts[(int64_t)c * nblk + blockIdx.x] = s_cnt[c];
+}
+
+// Channel segments of the bucketed sort: seg_start / tile_start as
+// k_segments, flow_end[c]: end of channel c's flow attempts.
+__global__ void k_segments_mix(int C, const int* __restrict__ offs, int nblk, int64_t total, int* seg_start, int* tile_start,
+                               int* flow_end) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int t = 0;
+    for (int c = 0; c < C; ++c) {
+        int s = offs[(int64_t)(2 * c) * nblk];
+        int e = (c + 1 < C) ? offs[(int64_t)(2 * c + 2) * nblk] : (int)total;
+        seg_start[c] = s;
+        flow_end[c] = offs[(int64_t)(2 * c + 1) * nblk];
+        tile_start[c] = t;
+        t += (e - s + kTile - 1) / kTile;
+    }
+    seg_start[C] = (int)total;
+    tile_start[C] = t;
+}
+
 // ---------------------------------------------------------------------------
 // k_forward: VEGAS forward map, FSMapper::GeneratePoint and the cut for one
 // channel per workgroup.  Momenta of external legs go to mom[(i*4+k)*nmax + e].
@@ -380,22 +418,6 @@ __global__ __launch_bounds__(kTile) void k_zgen(MCArgs A, int64_t n, const uint3
     for (int k = 0; k < D; ++k) z[e * D + k] = r[k];
 }
 
-// Defensive mixture: an attempt whose mixture uniform is below `mix` keeps
-// its uniforms as the hypercube point (z[e] = the attempt's first D uniforms)
-// instead of the flow's sample.
-template <int D>
-__global__ __launch_bounds__(kTile) void k_mix_restore(MCArgs A, int64_t n, const uint32_t* __restrict__ sorted, double mix,
-                                                       double* __restrict__ z) {
-    const int64_t e = (int64_t)blockIdx.x * kTile + threadIdx.x;
-    if (e >= n) return;
-    const uint64_t att = A.a0 + sorted[e];
-    if (!(mixture_uniform(A, att) < mix)) return;
-    double r[D + 1];
-    load_uniforms<D>(A, att, r);
-#pragma unroll
-    for (int k = 0; k < D; ++k) z[e * D + k] = r[k];
-}
-
 // All-channel inverse of every kept point: f, a_j = alpha_j / ps_j / vegas_j
 // ([j][stride]) and the hypercube points r_j ([j][stride][D]).
 template <int NT>
@@ -523,6 +545,7 @@ struct chili_mc {
     DevBuf<double> d_edges, d_search, d_alpha, d_cum, d_sext, d_tot, d_part, d_mom, d_ps, d_u, d_gsum, d_hsum;
     DevBuf<chili_integrand> d_f;
     DevBuf<uint16_t> d_pick;
+    DevBuf<int> d_fend;          // flow path with a mixture: end of each channel's flow attempts
     DevBuf<uint32_t> d_sorted, d_comp;
     DevBuf<uint8_t> d_acc;
     DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff, d_grp2;
@@ -934,13 +957,6 @@ void launch_zgen(chili_mc* m, const MCArgs& A, int64_t n) {
                        m->d_z.ptr);
 }
 template <int NT>
-void launch_mix_restore(chili_mc* m, const MCArgs& A, int64_t n, double mix) {
-    constexpr int D = 3 * (NT - 2) - 2;
-    if (A.u) throw chili_error(CHILI_EINVAL, "a mixture proposal needs device-generated (Philox) uniforms");
-    hipLaunchKernelGGL(k_mix_restore<D>, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, n, m->d_sorted.ptr,
-                       mix, m->d_z.ptr);
-}
-template <int NT>
 void launch_geom(chili_mc* m, const MCArgs& A, int64_t kept) {
     hipLaunchKernelGGL(k_geom<NT>, dim3((unsigned)((kept + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, kept, m->d_comp.ptr,
                        m->d_mom.ptr, m->nmax, m->d_rall.ptr, m->d_abuf.ptr, m->d_fbuf.ptr, m->fmax);
@@ -957,20 +973,38 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
     const int nblk = (int)((n + kTile - 1) / kTile);
     MCArgs A = make_args(m, u, ustride, seed, stream, a0);
     A.ua0 = ua0;
-    hipLaunchKernelGGL(k_pick, dim3(nblk), dim3(kTile), sizeof(int) * C, m->stream, A, n, m->d_pick.ptr, m->d_counts.ptr, nblk);
-    scan_int(m, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * C);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kTile), sizeof(int) * kWaves * C, m->stream, C, n, m->d_pick.ptr, m->d_offs.ptr, nblk,
-                       m->d_sorted.ptr);
-    hipLaunchKernelGGL(k_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_offs.ptr, nblk, n, m->d_seg.ptr, m->d_tiles.ptr);
+    // with a mixture proposal the sort key is (channel, component): the
+    // uniform component's attempts close each channel's segment and keep
+    // their uniforms as the hypercube point
+    const bool mixing = fl && fl->eng.mix > 0;
+    if (mixing && u) throw chili_error(CHILI_EINVAL, "a mixture proposal needs device-generated (Philox) uniforms");
+    const int nb = mixing ? 2 * C : C;
+    m->d_counts.resize((size_t)nblk * nb);
+    m->d_offs.resize((size_t)nblk * nb);
+    if (mixing)
+        hipLaunchKernelGGL(k_pick_mix, dim3(nblk), dim3(kTile), sizeof(int) * nb, m->stream, A, n, fl->eng.mix, m->d_pick.ptr,
+                           m->d_counts.ptr, nblk);
+    else
+        hipLaunchKernelGGL(k_pick, dim3(nblk), dim3(kTile), sizeof(int) * C, m->stream, A, n, m->d_pick.ptr, m->d_counts.ptr, nblk);
+    scan_int(m, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * nb);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kTile), sizeof(int) * kWaves * nb, m->stream, nb, n, m->d_pick.ptr, m->d_offs.ptr,
+                       nblk, m->d_sorted.ptr);
+    if (mixing) {
+        m->d_fend.resize(C);
+        hipLaunchKernelGGL(k_segments_mix, dim3(1), dim3(1), 0, m->stream, C, m->d_offs.ptr, nblk, n, m->d_seg.ptr, m->d_tiles.ptr,
+                           m->d_fend.ptr);
+    } else {
+        hipLaunchKernelGGL(k_segments, dim3(1), dim3(1), 0, m->stream, C, m->d_offs.ptr, nblk, n, m->d_seg.ptr, m->d_tiles.ptr);
+    }
     const int tgrid = nblk + C;
     if (fl) {
         // z -> r through each channel's flow, on the channel-sorted segments
+        // (with mixing: only their flow parts)
         CHILI_DISPATCH_NT(m->NT, launch_zgen<NT>(m, A, n));
         chili_nf::StreamSwap sw(fl->eng, m->stream);
         hipLaunchKernelGGL(chili_nf::k_fill<double>, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, n, 0.0,
                            m->d_lqs.ptr);
-        fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr);
-        if (fl->eng.mix > 0) CHILI_DISPATCH_NT(m->NT, launch_mix_restore<NT>(m, A, n, fl->eng.mix));
+        fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr, mixing ? m->d_fend.ptr : nullptr);
     }
     CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr));
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
