@@ -169,6 +169,7 @@ def main():
     ap.add_argument("--flow-epochs", type=int, default=30, help="training epochs (the full schedule is 200)")
     ap.add_argument("--flow-batch", type=int, default=1 << 16)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
+    ap.add_argument("--flow-val-every", type=int, default=3, help="epochs between validation passes (model selection)")
     ap.add_argument("--flow-mix", default="0,0.3,0.5,0.7",
                     help="defensive mixture weights of the uniform density to run (comma list; the best is reported)")
     ap.add_argument("--no-specialize", action="store_true",
@@ -288,12 +289,30 @@ def main():
         t1 = time.perf_counter()
         losses = []
         done = 0
+        # model selection: after every --flow-val-every epochs the flows'
+        # unweighting efficiency <w>/w_max is measured on an independent
+        # sample (its own Philox streams); the best parameters are kept
+        nv = min(n, 1 << 22)
+        best = None
+        val_hist = []
         while done < args.flow_epochs:
-            e = min(10, args.flow_epochs - done)
+            e = min(args.flow_val_every, args.flow_epochs - done)
             losses += list(fl.train(nk, r.data_ptr(), beta.data_ptr(), w2.data_ptr(), args.flow_batch, e, args.flow_lr,
                                     seed=seed + done))
             done += e
-            log(f"  flow training: epoch {done}/{args.flow_epochs}, loss {losses[-1]:.5e} ({time.perf_counter() - t1:.1f} s)")
+            gpu.flow_iterate(fl, abi.philox_uniforms(seed, stream=700 + done), nv, vals_ptr=ctypes.c_void_p(vals.data_ptr()),
+                             vals_cap=nv)
+            v = vals[:nv]
+            eps_v = float(v.mean() / v.max())
+            val_hist.append(dict(epoch=done, loss=round(losses[-1], 3), eps_unw=round(eps_v, 5),
+                                 w_rel_std=round(float(v.std() / v.mean()), 4)))
+            if best is None or eps_v > best[0]:
+                best = (eps_v, done, [fl.params(j) for j in range(C)])
+            log(f"  flow training: epoch {done}/{args.flow_epochs}, loss {losses[-1]:.5e}, validation eps {eps_v:.4f} "
+                f"({time.perf_counter() - t1:.1f} s)")
+        for j in range(C):
+            fl.set_params(j, best[2][j])
+        log(f"  flow parameters of epoch {best[1]} kept (validation eps {best[0]:.4f})")
         t_train = time.perf_counter() - t1
         del r, beta, w2
 
@@ -312,6 +331,7 @@ def main():
         flow.update(dict(mixture=lam_best, mixture_scan={f"{k:g}": dict(unweighted_per_s=v["unweighted_per_s"], eps_unw=v["eps_unw"],
                                                                           w_rel_std=round(v["w_rel_std"], 4))
                                                           for k, v in scan.items()}))
+        flow.update(dict(selected_epoch=best[1], validation=val_hist))
         flow.update(dict(layers=W["flow_layers"], hidden="3x64", spline_bins=16, train_points=int(nk), epochs=args.flow_epochs,
                          batch=args.flow_batch, lr=args.flow_lr, train_s=round(t_train, 2),
                          data_s=round(t1 - t0, 3), loss_first=losses[0] if losses else None,
