@@ -357,7 +357,7 @@ def main():
                 compute="FP64 VALU; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s",
                 flop_eq_per_point=round(f_main, 1), points_per_launch=round(pts_launch, 1),
                 avg_launch_ms=round(avg_ms, 4), launches=tm["launches"],
-                forward_kernel_ms_per_step=round(tm["forward_ms"] / args.steps, 4))
+                forward_kernel_ms_per_step=round(tm["forward_ms"] / (args.steps + args.warmup), 4))
 
     # ---- CPU baseline: the scalar oracle on the host cores, bounded sample ----
     cpu = None
