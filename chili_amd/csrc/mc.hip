@@ -909,7 +909,10 @@ void iterate_loop(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n
     const int64_t max_attempts = 100000 * ncalls + (int64_t(1) << 24);
     while (need > 0) {
         if (att > max_attempts) throw chili_error(CHILI_ERANGE, "no attempts pass the cut: retry limit reached");
-        int64_t n = (int64_t)((double)need / std::max(m->acc_rate, 0.02) * 1.08) + 2048;
+        // attempts expected to give `need` points, with a margin of ~1% plus
+        // five binomial standard deviations; a shortfall runs another round
+        const double expect = (double)need / std::max(m->acc_rate, 0.02);
+        int64_t n = (int64_t)(expect * 1.01 + 5.0 * std::sqrt(expect)) + 2048;
         n = std::min(n, cap);
         if (ud) {
             n = std::min<int64_t>(n, ustride - att);
