@@ -208,6 +208,112 @@ def test_iterate_parity(case):
     assert abs(so2.sum - sg.sum) <= 1e-6 * abs(so2.sum)
 
 
+def degenerate_grids(rng, nch, d, B):
+    """Grids with runs of equal edges (zero-width bins) and edges on dyadic
+    values: the forward map then lands exactly on edges, where FindBin's
+    lower_bound - 1 semantics (count of edges strictly below x) decide.
+    Interior edges stay within [1/16, 15/16]: zero-width bins at 0 or 1 would
+    send points to the ends of the maps (s = smin, cos theta = +-1), where the
+    kinematics degenerate and the weights depend on rounding noise."""
+    g = np.zeros((nch, d, B + 1))
+    for c in range(nch):
+        for k in range(d):
+            inner = np.sort(np.clip(np.round(rng.uniform(0.0, 1.0, B - 1) * 16) / 16, 1 / 16, 15 / 16))
+            g[c, k] = np.concatenate([[0.0], inner, [1.0]])
+    return g
+
+
+@pytest.mark.parametrize("case", ["gg_3g", "tt_6j"])
+def test_points_parity_degenerate_grids(case):
+    """Zero-width bins and points exactly on edges: bins, weights and values
+    as the oracle's (the LDS-staged search included)."""
+    proc, f = CASES[case]()
+    ch = lib.Channels(proc)
+    B = 64
+    gpu = lib.MultiChannelGPU(ch, B, f)
+    ora = O.Oracle(proc, B, f)
+    rng = np.random.default_rng(17)
+    g = degenerate_grids(rng, gpu.nchannels, gpu.ndims, B)
+    a = random_alpha(rng, gpu.nchannels)
+    gpu.set_grids(g); ora.set_grids(g)
+    gpu.set_alpha(a); ora.set_alpha(a)
+    n = 4000
+    uni = abi.philox_uniforms(SEED + 21, stream=2)
+    gg = gpu.points(uni, 0, n)
+    # Dyadic edges put momenta exactly on cut boundaries (y = 0, dphi = pi/8,
+    # ...), where the last bit of a libm call decides the cut, so the forward
+    # stage is compared through its picks only; the weights are compared on
+    # the GPU's own momenta (staged), where the FindBin ties are decided.
+    assert np.array_equal(gg["pick"], ora.points(uni, 0, n)["pick"])
+    o = ora.points(uni, 0, n, mom_in=gg["mom"])
+    assert np.array_equal(gg["status"], o["status"])
+    live = np.nonzero((o["status"] == 1) & (o["ps"][np.arange(n), o["pick"]] > 0))[0]
+    assert len(live) > 200
+    # the device search is FindBin exactly: bins from the GPU's own inverse
+    # point by lower_bound over e_0..e_{B-1} minus one (clamped at 0; a NaN
+    # coordinate compares false everywhere, so lower_bound returns e_0)
+    def find_bin(e, x):
+        return 0 if np.isnan(x) else max(int(np.searchsorted(e, x, side="left")) - 1, 0)
+
+    for i in live:
+        e = g[gg["pick"][i]]
+        want = [find_bin(e[k, :B], gg["xpick"][i, k]) for k in range(gpu.ndims)]
+        assert np.array_equal(gg["bins"][i], want), f"point {i}: bins {gg['bins'][i]} vs FindBin {want}"
+    # the inverse points agree to 1e-12; a coordinate that sits on an edge can
+    # land on the other side of it in the oracle (last bits of its libm), and
+    # only there may the bins (and, for zero-width bins, the weights) differ:
+    # an edge must lie between the two values
+    assert_rel(gg["xpick"][live], o["xpick"][live], 1e-12, scale=1.0, what=f"{case} inverse point")
+    same = np.all(gg["bins"][live] == o["bins"][live], axis=1)
+    flips = live[~same]
+    for i in flips:
+        e = g[gg["pick"][i]]
+        for k in np.nonzero(gg["bins"][i] != o["bins"][i])[0]:
+            lo, hi = sorted((gg["xpick"][i, k], o["xpick"][i, k]))
+            assert np.any((e[k] >= lo) & (e[k] <= hi)), f"point {i} dim {k}: {lo!r} {hi!r} straddle no edge"
+    # grid-independent parts on every live point; the multichannel weight also
+    # holds the other channels' VEGAS densities, whose inverse coordinates can
+    # sit on edges as well (not recorded), so it is compared where it agrees
+    # with the picked channel's bins and must match for the large majority
+    assert_rel(gg["ps"][live], o["ps"][live], 1e-12, what=f"{case} phase-space weights")
+    assert_rel(gg["f"][live], o["f"][live], 1e-12, what=f"{case} integrand")
+    agree = live[same]
+    dw = np.abs(gg["w"][agree] - o["w"][agree]) / np.abs(o["w"][agree])
+    assert np.mean(dw <= 1e-12) > 0.98
+    on_edge = np.isin(gg["xpick"][live], np.unique(g)).sum()
+    print(f"{case}: {len(live)} live points, {on_edge} inverse coordinates exactly on an edge, {len(flips)} tie flips "
+          f"of the picked channel's bins; W within 1e-12 on {np.mean(dw <= 1e-12):.3f} of the rest")
+
+
+def test_full_size_properties():
+    """2->6 at the production size (2^24 accepted points): exact counts,
+    bit-identical repeat, the histogram identity sum_c sum_b h[c, k, b] =
+    sum of val^2 for every dimension, and the integral against an independent
+    smaller run."""
+    proc, f = CASES["tt_6j"]()
+    ch = lib.Channels(proc)
+    gpu = lib.MultiChannelGPU(ch, 64, f)
+    n = 1 << 24
+    uni = abi.philox_uniforms(SEED + 30, stream=4)
+    s1, T1, h1, na1 = gpu.iterate(uni, n, train=True)
+    s2, T2, h2, na2 = gpu.iterate(uni, n, train=True)
+    assert na1 == na2 and np.array_equal(stats_tuple(s1), stats_tuple(s2))
+    assert np.array_equal(T1, T2) and np.array_equal(h1, h2)
+    assert s1.n == na1
+    per_dim = h1.sum(axis=(0, 2))
+    assert_rel(per_dim, np.full_like(per_dim, s1.sum2), 1e-10, what="histogram identity")
+    s3, _, _, _ = gpu.iterate(abi.philox_uniforms(SEED + 31, stream=5), 1 << 20, train=False)
+
+    def mean_err(st):
+        m = st.sum / st.n
+        return m, np.sqrt((st.sum2 / st.n - m * m) / (st.n - 1))
+
+    m1, e1 = mean_err(s1)
+    m3, e3 = mean_err(s3)
+    print(f"2^24: {m1:.6e} +- {e1:.1e} ({na1} attempts); 2^20: {m3:.6e} +- {e3:.1e}")
+    assert abs(m1 - m3) < 5 * np.hypot(e1, e3)
+
+
 def test_last_bin_reachable():
     """x in the top bin (e_{B-1}, 1] maps to bin B-1 for power-of-two and
     other bin counts alike."""
