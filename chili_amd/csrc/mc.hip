@@ -543,6 +543,7 @@ struct chili_mc {
     // device buffers
     DevBuf<ChanProg> d_progs;
     DevBuf<double> d_edges, d_search, d_alpha, d_cum, d_sext, d_tot, d_part, d_mom, d_ps, d_u, d_gsum, d_hsum;
+    DevBuf<uint16_t> d_cells;      // cell index of the search tables [C][d][kCells + 1]
     DevBuf<chili_integrand> d_f;
     DevBuf<uint16_t> d_pick;
     DevBuf<int> d_fend;          // flow path with a mixture: end of each channel's flow attempts
@@ -604,6 +605,19 @@ void upload_state(chili_mc* m) {
             for (int b = 1; b < B; ++b) srch[((size_t)c * d + k) * P2 + b - 1] = m->grids[((size_t)c * d + k) * (B + 1) + b];
     m->d_edges.upload(m->grids, m->stream);
     m->d_search.upload(srch, m->stream);
+    // cell index: t[j] = interior edges strictly below j / kCells, t[kCells]
+    // = those below 1 (exact: j / kCells is a dyadic double)
+    std::vector<uint16_t> cells((size_t)C * d * (kCells + 1));
+    for (size_t r = 0; r < (size_t)C * d; ++r) {
+        const double* e = m->grids.data() + r * (B + 1);
+        int cnt = 0;
+        for (int j = 0; j <= kCells; ++j) {
+            const double y = (double)j / kCells;
+            while (cnt < B - 1 && e[cnt + 1] < y) ++cnt;
+            cells[r * (kCells + 1) + j] = (uint16_t)cnt;
+        }
+    }
+    m->d_cells.upload(cells, m->stream);
     // cumulative alpha as std::discrete_distribution builds it (random.tcc:2656-2677)
     std::vector<double> cum(C, 1.0);
     if (C >= 2) {
@@ -642,6 +656,7 @@ MCArgs make_args(chili_mc* m, const double* u, int64_t ustride, uint64_t seed, u
     A.progs = m->d_progs.ptr;
     A.edges = m->d_edges.ptr;
     A.search = m->d_search.ptr;
+    A.cells = m->d_cells.ptr;
     A.alpha = m->d_alpha.ptr;
     A.cum = m->d_cum.ptr;
     A.sext = m->d_sext.ptr;

@@ -50,6 +50,7 @@ struct MCArgs {
     const ChanProg* progs;
     const double* edges;     // [C][d][B+1]
     const double* search;    // [C][d][P2] interior edges e_1..e_{B-1} padded with +inf
+    const uint16_t* cells;   // [C][d][kCells + 1]: interior edges below j / kCells (null: plain search)
     const double* alpha;     // [C]
     const double* cum;       // [C] cumulative alpha (last entry 1.0)
     const double* sext;      // [ntot] external masses^2 (0 for incoming)
@@ -127,6 +128,22 @@ CHILI_DEV int find_bin(const double* __restrict__ s, int P2, double x) {
     int lo = 0;
     for (int step = P2 >> 1; step > 0; step >>= 1)
         lo += (s[lo + step - 1] < x) ? step : 0;
+    return lo;
+}
+
+// The same count through a uniform cell index: for x in [j/K, (j+1)/K) it
+// lies between t[j] and t[j+1] (the counts at the cell's ends), so a short
+// search over that range replaces most of the dependent loads.  x outside
+// [0, 1) (or NaN) takes the full search.
+constexpr int kCells = 256;
+CHILI_DEV int find_bin_cells(const double* __restrict__ s, const uint16_t* __restrict__ t, int P2, double x) {
+    if (!(x >= 0.0 && x < 1.0)) return find_bin(s, P2, x);
+    const int j = (int)(x * (double)kCells);   // exact: kCells is a power of two
+    int lo = t[j], n = (int)t[j + 1] - lo;
+    while (n > 0) {
+        const int h = n >> 1;
+        if (s[lo + h] < x) { lo += h + 1; n -= h + 1; } else { n = h; }
+    }
     return lo;
 }
 
@@ -233,8 +250,9 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, Mom
     const double* __restrict__ srch = (search ? search : A.search) + (int64_t)ch * A.d * A.P2;
     double vw = 1.0;
     int slot = 0;
+    const uint16_t* __restrict__ cells = (search || !A.cells) ? nullptr : A.cells + (int64_t)ch * A.d * (kCells + 1);
     auto take = [&](double x) {
-        int b = find_bin(srch + slot * A.P2, A.P2, x);
+        int b = cells ? find_bin_cells(srch + slot * A.P2, cells + slot * (kCells + 1), A.P2, x) : find_bin(srch + slot * A.P2, A.P2, x);
         const double* e = edges + slot * (A.B + 1);
         vw *= (e[b + 1] - e[b]) * Bd;
         if (bins_out) bins_out[slot * bstride] = (short)b;

@@ -223,10 +223,11 @@ def degenerate_grids(rng, nch, d, B):
     return g
 
 
-@pytest.mark.parametrize("case", ["gg_3g", "tt_6j"])
+@pytest.mark.parametrize("case", ["gg_3g", "gg_4g", "tt_6j"])
 def test_points_parity_degenerate_grids(case):
     """Zero-width bins and points exactly on edges: bins, weights and values
-    as the oracle's (the LDS-staged search included)."""
+    as the oracle's (the LDS-staged search for 2->3 and 2->6, the cell-indexed
+    search in device memory for 2->4)."""
     proc, f = CASES[case]()
     ch = lib.Channels(proc)
     B = 64
