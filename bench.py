@@ -170,6 +170,8 @@ def main():
     ap.add_argument("--flow-batch", type=int, default=1 << 16)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-val-every", type=int, default=3, help="epochs between validation passes (model selection)")
+    ap.add_argument("--gen-events", type=int, default=1 << 23,
+                    help="unweighted events to generate against the production w_max (0: skip)")
     ap.add_argument("--flow-mix", default="0,0.3,0.5,0.7",
                     help="defensive mixture weights of the uniform density to run (comma list; the best is reported)")
     ap.add_argument("--no-specialize", action="store_true",
@@ -278,6 +280,28 @@ def main():
     unw_s = plain["unweighted_per_s"]
     ms_step = plain["ms_per_step"]
     atts = n * args.steps / plain["acceptance"]
+
+    # ---- unweighted events actually generated (device unweighting stage
+    # against the production w_max): the events/s the derived figure above
+    # stands for ----
+    if args.gen_events > 0:
+        ne = args.gen_events
+        ev = torch.empty(4 * gpu.ntot * ne, dtype=torch.float64, device="cuda:0")
+        ew = torch.empty(ne, dtype=torch.float64, device="cuda:0")
+        gpu.generate(abi.philox_uniforms(seed, stream=3000), 1 << 16, wmax, ctypes.c_void_p(ev.data_ptr()),
+                     ctypes.c_void_p(ew.data_ptr()), ne)                      # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gst, gatt = gpu.generate(abi.philox_uniforms(seed, stream=3001), ne, wmax, ctypes.c_void_p(ev.data_ptr()),
+                                 ctypes.c_void_p(ew.data_ptr()), ne)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        plain["generated"] = dict(events=ne, attempts=int(gatt), seconds=round(dt, 4), events_per_s=round(ne / dt, 1),
+                                  wmax=wmax, overweight=int((ew.abs() > 1).sum()),
+                                  note="chili_mc_generate: the MultiChannel loop with an unweighting PostProcess, "
+                                       "events (momenta, weights) written to HBM")
+        log(f"  [generate] {ne} unweighted events in {dt:.3f} s: {ne / dt:.3e} events/s ({gatt} attempts)")
+        del ev, ew
 
     # ---- flow leg ----
     flow = None

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kTile) void k_pick_mix(MCArgs A, int64_t n, double 
     if (i < n) {
         const uint64_t att = A.a0 + (uint64_t)i;
         const int c = pick_channel(A, channel_uniform(A, att));
-        const int b = 2 * c + (mixture_uniform(A, att) < mix ? 1 : 0);
+        const int b = 2 * c + (aux_uniform(A, att) < mix ? 1 : 0);
         pick[i] = (uint16_t)b;
         atomicAdd(&s_cnt[b], 1);
     }
@@ -528,6 +528,75 @@ __global__ __launch_bounds__(256) void k_stats_final(int64_t np, const double* _
     if (threadIdx.x == 0) stat_merge(tot, s[0]);
 }
 
+// ---------------------------------------------------------------------------
+// Unweighting stage: MultiChannel::operator() with a PostProcess that keeps a
+// point of value v with probability min(1, |v| / wmax) (MultiChannel.hh:
+// 240-245: a rejected point adds a zero and is retried).  The decision uses
+// the attempt's uniform number d+1.
+// ---------------------------------------------------------------------------
+// Per kept point (compaction order): accept flag, value and momentum column,
+// scattered to attempt order.
+__global__ __launch_bounds__(kTile) void k_unw(MCArgs A, int64_t kept, const uint32_t* __restrict__ comp,
+                                               const uint32_t* __restrict__ sorted, const double* __restrict__ val, double wmax,
+                                               uint8_t* __restrict__ flag, double* __restrict__ gv, uint32_t* __restrict__ gpos) {
+    const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (k >= kept) return;
+    const uint32_t sp = comp[k];
+    const uint32_t a = sorted[sp];
+    const double v = val[k];
+    flag[a] = (aux_uniform(A, A.a0 + a) < fabs(v) / wmax) ? 1 : 0;
+    gv[a] = v;
+    gpos[a] = sp;
+}
+
+// StatsData partial of the accepted events among attempts [0, used) of one
+// block of kTile attempts: [n, min, max, sum, sum2, nf, nnan].
+__global__ __launch_bounds__(kTile) void k_gen_stats(int64_t used, const uint8_t* __restrict__ flag, const double* __restrict__ gv,
+                                                     double* __restrict__ part) {
+    __shared__ double s_part[kWaves][kStatW];
+    const int64_t a = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool ok = a < used && flag[a];
+    const double v = ok ? gv[a] : 0.0;
+    const double n = wave_sum(ok ? 1.0 : 0.0), s1 = wave_sum(v), s2 = wave_sum(v * v);
+    const double nf = wave_sum((ok && v != 0) ? 1.0 : 0.0);
+    const double mn = wave_min(ok ? v : __DBL_MAX__), mx = wave_max(ok ? v : __DBL_MIN__);
+    if (lane == 0) {
+        double* q = s_part[w];
+        q[0] = n; q[1] = mn; q[2] = mx; q[3] = s1; q[4] = s2; q[5] = nf; q[6] = 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc[kStatW] = {0, __DBL_MAX__, __DBL_MIN__, 0, 0, 0, 0};
+        for (int ww = 0; ww < kWaves; ++ww) stat_merge(acc, s_part[ww]);
+        for (int q = 0; q < kStatW; ++q) part[(int64_t)blockIdx.x * kStatW + q] = acc[q];
+    }
+}
+
+// Accepted events among attempts [0, used) in attempt order: momenta of the
+// external legs, SoA ev[(i*4+k)][cap] from column base + rank, and the event
+// weight sign(v) max(1, |v| / wmax).
+__global__ __launch_bounds__(kTile) void k_gen_events(int NT, int64_t used, const uint8_t* __restrict__ flag, const int* __restrict__ blkoff,
+                                                      const uint32_t* __restrict__ gpos, const double* __restrict__ gv, double wmax,
+                                                      const double* __restrict__ mom, int64_t nmax, double* __restrict__ ev,
+                                                      double* __restrict__ ew, int64_t cap, int64_t base) {
+    __shared__ int s_cnt[kWaves];
+    const int64_t a = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool ok = a < used && flag[a];
+    const uint64_t mask = __ballot(ok);
+    if (lane == 0) s_cnt[w] = __popcll(mask);
+    __syncthreads();
+    if (!ok) return;
+    int rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
+    for (int ww = 0; ww < w; ++ww) rank += s_cnt[ww];
+    const int64_t col = base + blkoff[blockIdx.x] + rank;
+    const uint32_t sp = gpos[a];
+    for (int i = 0; i < 4 * NT; ++i) ev[(int64_t)i * cap + col] = mom[(int64_t)i * nmax + sp];
+    const double v = gv[a];
+    ew[col] = (fabs(v) > wmax) ? v / wmax : copysign(1.0, v);
+}
+
 // ===========================================================================
 // host context
 // ===========================================================================
@@ -547,6 +616,12 @@ struct chili_mc {
     DevBuf<chili_integrand> d_f;
     DevBuf<uint16_t> d_pick;
     DevBuf<int> d_fend;          // flow path with a mixture: end of each channel's flow attempts
+    // unweighting stage: values of the kept points, per-attempt flag / value /
+    // momentum column, block offsets and partials
+    DevBuf<double> d_gval, d_gv, d_gpart;
+    DevBuf<uint8_t> d_gflag;
+    DevBuf<uint32_t> d_gpos;
+    DevBuf<int> d_gblkoff;
     DevBuf<uint32_t> d_sorted, d_comp;
     DevBuf<uint8_t> d_acc;
     DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff, d_grp2;
@@ -561,6 +636,7 @@ struct chili_mc {
     std::unique_ptr<chili_jit::Module> jit;
     bool jit_fwd = false, jit_main = false;
     bool lds_search = true;      // weight kernel: search tables in LDS when they fit
+    double gen_rate = 0.01;      // unweighting stage: kept events per attempt (running estimate)
     double jit_compile_s = 0;
     // k_main timing with HIP events on the context's stream
     bool timing = false;
@@ -792,21 +868,22 @@ void harvest_events(chili_mc* m) {
 
 // Cutoff (the attempt at which `need` accepted attempts are reached; host
 // sync) and stable compaction of the kept entries of the forward tiles.
-void cutoff_and_compact(chili_mc* m, int64_t n, int nblk, int tgrid, int64_t need, int64_t& kept, int64_t& used) {
-    const int C = m->C;
-    hipLaunchKernelGGL(k_acc_count, dim3(nblk), dim3(kTile), 0, m->stream, m->d_acc.ptr, n, m->d_blkacc.ptr);
+// The attempt at which `need` flags (attempt order) are reached: kept =
+// flags up to it, used = attempts up to it (all n when fewer are set); the
+// per-block flag counts stay in d_blkacc.
+void find_cutoff(chili_mc* m, const uint8_t* flags, int64_t n, int nblk, int64_t need, int64_t& kept, int64_t& used) {
+    hipLaunchKernelGGL(k_acc_count, dim3(nblk), dim3(kTile), 0, m->stream, flags, n, m->d_blkacc.ptr);
     std::vector<int> blkacc(nblk);
     m->d_blkacc.download(blkacc.data(), nblk, m->stream);
     HIP_CHECK(hipStreamSynchronize(m->stream));
     int64_t run = 0;
     int64_t cut = n;
-    kept = 0;
     for (int b = 0; b < nblk; ++b) {
         if (run + blkacc[b] >= need) {
             std::vector<uint8_t> fl(kTile);
             const int64_t base = (int64_t)b * kTile;
             const int64_t cnt = std::min<int64_t>(kTile, n - base);
-            HIP_CHECK(hipMemcpyAsync(fl.data(), m->d_acc.ptr + base, cnt, hipMemcpyDeviceToHost, m->stream));
+            HIP_CHECK(hipMemcpyAsync(fl.data(), flags + base, cnt, hipMemcpyDeviceToHost, m->stream));
             HIP_CHECK(hipStreamSynchronize(m->stream));
             for (int64_t i = 0; i < cnt; ++i) {
                 run += fl[i];
@@ -818,6 +895,12 @@ void cutoff_and_compact(chili_mc* m, int64_t n, int nblk, int tgrid, int64_t nee
     }
     kept = std::min<int64_t>(run, need);
     used = cut;
+}
+
+void cutoff_and_compact(chili_mc* m, int64_t n, int nblk, int tgrid, int64_t need, int64_t& kept, int64_t& used) {
+    const int C = m->C;
+    find_cutoff(m, m->d_acc.ptr, n, nblk, need, kept, used);
+    const int64_t cut = used;
     hipLaunchKernelGGL(k_keep_count, dim3(tgrid), dim3(kTile), 0, m->stream, C, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr, m->d_acc.ptr,
                        (uint32_t)cut, m->d_kept.ptr);
     scan_int(m, m->d_kept.ptr, m->d_keptoff.ptr, (int64_t)tgrid);
@@ -827,16 +910,11 @@ void cutoff_and_compact(chili_mc* m, int64_t n, int nblk, int tgrid, int64_t nee
                        m->d_tiles2.ptr, kMainTile);
 }
 
-// One sub-batch of attempts [a0, a0 + n): returns kept (accepted below the
-// cutoff), the cutoff (attempts consumed) and NaN count, accumulating into d_tot.
-void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, uint64_t seed, uint32_t stream, uint64_t a0,
-                  int64_t n, int64_t need, int train, double* vals, int64_t vals_off, int64_t vals_cap, int64_t& kept, int64_t& used,
-                  const PointRec& R = PointRec{}) {
-    ensure_capacity(m, n);
+// Channel picks, the stable sort by channel and the forward map + cut of n
+// attempts; returns the forward grid (tiles).
+int pick_sort_forward(chili_mc* m, const MCArgs& A, int64_t n, const PointRec& R) {
     const int C = m->C;
     const int nblk = (int)((n + kTile - 1) / kTile);
-    MCArgs A = make_args(m, u, ustride, seed, stream, a0);
-    A.ua0 = ua0;
     hipLaunchKernelGGL(k_pick, dim3(nblk), dim3(kTile), sizeof(int) * C, m->stream, A, n, m->d_pick.ptr, m->d_counts.ptr, nblk);
     scan_int(m, m->d_counts.ptr, m->d_offs.ptr, (int64_t)nblk * C);
     hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kTile), sizeof(int) * kWaves * C, m->stream, C, n, m->d_pick.ptr, m->d_offs.ptr, nblk,
@@ -853,6 +931,68 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
         HIP_CHECK(hipEventRecord(fev.second, m->stream));
         m->fev_live.push_back(fev);
     }
+    return tgrid;
+}
+
+// One sub-batch of the unweighting stage over attempts [a0, a0 + n): every
+// point passing the cut is weighted (production: no training sums), the
+// accept / reject decisions are made per attempt, the cutoff is the attempt
+// of the need-th accepted event; StatsData of the accepted events goes to
+// d_tot, the events to ev / ew from column got.
+void run_subbatch_gen(chili_mc* m, uint64_t seed, uint32_t stream, uint64_t a0, int64_t n, int64_t need, double wmax, double* ev,
+                      double* ew, int64_t cap, int64_t got, int64_t& kept, int64_t& used) {
+    ensure_capacity(m, n);
+    const int C = m->C;
+    const int nblk = (int)((n + kTile - 1) / kTile);
+    MCArgs A = make_args(m, nullptr, 0, seed, stream, a0);
+    const int tgrid = pick_sort_forward(m, A, n, PointRec{});
+    int64_t passed = 0, all = 0;
+    cutoff_and_compact(m, n, nblk, tgrid, std::numeric_limits<int64_t>::max(), passed, all);
+    m->d_gflag.resize(n);
+    m->d_gv.resize(n);
+    m->d_gpos.resize(n);
+    hipLaunchKernelGGL(chili_nf::k_fill<uint8_t>, dim3(nblk), dim3(kTile), 0, m->stream, n, (uint8_t)0, m->d_gflag.ptr);
+    if (passed > 0) {
+        m->d_gval.resize(passed);
+        const int mgrid = (int)((passed + kMainTile - 1) / kMainTile) + C;
+        std::pair<hipEvent_t, hipEvent_t> evt{};
+        if (m->timing) {
+            evt = take_events(m);
+            HIP_CHECK(hipEventRecord(evt.first, m->stream));
+        }
+        CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, 0, m->d_gval.ptr, 0, passed, PointRec{}));
+        if (m->timing) {
+            HIP_CHECK(hipEventRecord(evt.second, m->stream));
+            m->ev_live.push_back(evt);
+            m->ev_points.push_back(passed);
+        }
+        hipLaunchKernelGGL(k_unw, dim3((unsigned)((passed + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, passed, m->d_comp.ptr,
+                           m->d_sorted.ptr, m->d_gval.ptr, wmax, m->d_gflag.ptr, m->d_gv.ptr, m->d_gpos.ptr);
+    }
+    find_cutoff(m, m->d_gflag.ptr, n, nblk, need, kept, used);
+    if (got + kept > cap) throw chili_error(CHILI_ERANGE, "event buffer too small");
+    const int ublk = (int)((used + kTile - 1) / kTile);
+    if (ublk == 0) return;
+    m->d_gpart.resize((size_t)ublk * kStatW);
+    hipLaunchKernelGGL(k_gen_stats, dim3(ublk), dim3(kTile), 0, m->stream, used, m->d_gflag.ptr, m->d_gv.ptr, m->d_gpart.ptr);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, m->stream, (int64_t)ublk, m->d_gpart.ptr, m->d_tot.ptr);
+    m->d_gblkoff.resize(ublk);
+    scan_int(m, m->d_blkacc.ptr, m->d_gblkoff.ptr, (int64_t)ublk);
+    hipLaunchKernelGGL(k_gen_events, dim3(ublk), dim3(kTile), 0, m->stream, m->NT, used, m->d_gflag.ptr, m->d_gblkoff.ptr, m->d_gpos.ptr,
+                       m->d_gv.ptr, wmax, m->d_mom.ptr, m->nmax, ev, ew, cap, got);
+}
+
+// One sub-batch of attempts [a0, a0 + n): returns kept (accepted below the
+// cutoff), the cutoff (attempts consumed) and NaN count, accumulating into d_tot.
+void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, uint64_t seed, uint32_t stream, uint64_t a0,
+                  int64_t n, int64_t need, int train, double* vals, int64_t vals_off, int64_t vals_cap, int64_t& kept, int64_t& used,
+                  const PointRec& R = PointRec{}) {
+    ensure_capacity(m, n);
+    const int C = m->C;
+    const int nblk = (int)((n + kTile - 1) / kTile);
+    MCArgs A = make_args(m, u, ustride, seed, stream, a0);
+    A.ua0 = ua0;
+    const int tgrid = pick_sort_forward(m, A, n, R);
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
     if (kept > 0) {
         const int mgrid = (int)((kept + kMainTile - 1) / kMainTile) + C;
@@ -907,7 +1047,10 @@ const double* stage_uniforms(chili_mc* m, const chili_uniforms* uni, int64_t& us
 // the iteration's StatsData with a zero per rejected attempt.
 template <class F>
 void iterate_loop(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, int64_t cap_max, std::vector<double>& tot,
-                  int64_t& att, chili_stats* stats, F&& sub) {
+                  int64_t& att, chili_stats* stats, F&& sub, double* rate_ptr = nullptr, double rate_floor = 0.02) {
+    // running estimate of accepted points per attempt (the loop's own, or
+    // one the caller keeps for a different acceptance such as unweighting)
+    double& rate = rate_ptr ? *rate_ptr : m->acc_rate;
     int64_t ustride = 0;
     const double* ud = stage_uniforms(m, uni, ustride);
     const size_t ntot = kStatW + m->C + (size_t)m->C * m->d * m->B;
@@ -926,7 +1069,7 @@ void iterate_loop(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n
         if (att > max_attempts) throw chili_error(CHILI_ERANGE, "no attempts pass the cut: retry limit reached");
         // attempts expected to give `need` points, with a margin of ~1% plus
         // five binomial standard deviations; a shortfall runs another round
-        const double expect = (double)need / std::max(m->acc_rate, 0.02);
+        const double expect = (double)need / std::max(rate, rate_floor);
         int64_t n = (int64_t)(expect * 1.01 + 5.0 * std::sqrt(expect)) + 2048;
         n = std::min(n, cap);
         if (ud) {
@@ -944,7 +1087,7 @@ void iterate_loop(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n
         need -= kept - nan_new;
         got += kept;
         att += used;
-        if (used > 0) m->acc_rate = 0.5 * m->acc_rate + 0.5 * std::max(0.005, (double)kept / (double)used);
+        if (used > 0) rate = 0.5 * rate + 0.5 * std::max(0.5 * rate_floor, (double)kept / (double)used);
     }
     HIP_CHECK(hipMemcpyAsync(tot.data(), m->d_tot.ptr, ntot * sizeof(double), hipMemcpyDeviceToHost, m->stream));
     HIP_CHECK(hipStreamSynchronize(m->stream));
@@ -1700,6 +1843,32 @@ int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_
                      });
         if (T && train) std::copy(tot.begin() + kStatW, tot.begin() + kStatW + m->C, T);
         if (hist && train) std::copy(tot.begin() + kStatW + m->C, tot.end(), hist);
+        if (n_attempts) *n_attempts = att;
+    });
+}
+
+// MultiChannel::operator() (MultiChannel.hh:196-251) without training and
+// with an unweighting PostProcess (:240-245): a point of value v passing the
+// cut is kept with probability min(1, |v| / wmax) (the attempt's Philox
+// uniform number d+1), else it adds a zero and the loop retries, until
+// nevents are kept.  Device outputs: ev [4 ntot][cap] momenta of the kept
+// events in attempt order, ew [cap] their weights sign(v) max(1, |v| / wmax).
+int chili_mc_generate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t nevents, double wmax, double* ev_dev,
+                      double* ew_dev, int64_t cap, chili_stats* stats, int64_t* n_attempts) {
+    return guarded([&] {
+        if (!m || !uni || !stats || !ev_dev || !ew_dev) throw chili_error(CHILI_EINVAL, "null argument");
+        if (nevents <= 0) throw chili_error(CHILI_EINVAL, "nevents must be positive");
+        if (!(wmax > 0) || !std::isfinite(wmax)) throw chili_error(CHILI_EINVAL, "wmax must be positive and finite");
+        if (cap < nevents) throw chili_error(CHILI_ERANGE, "event buffer too small");
+        if (uni->u) throw chili_error(CHILI_EINVAL, "the unweighting stage needs device-generated (Philox) uniforms");
+        select_device(m->device);
+        std::vector<double> tot;
+        int64_t att = 0;
+        iterate_loop(m, uni, a0, nevents, int64_t(1) << 23, tot, att, stats,
+                     [&](const double*, int64_t, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept, int64_t& used) {
+                         run_subbatch_gen(m, uni->seed, uni->stream, ab, n, need, wmax, ev_dev, ew_dev, cap, got, kept, used);
+                     },
+                     &m->gen_rate, 1e-4);
         if (n_attempts) *n_attempts = att;
     });
 }

@@ -103,8 +103,9 @@ CHILI_DEV double channel_uniform(const MCArgs& A, uint64_t att) {
     return (A.d & 1) ? u1 : u0;
 }
 // Uniform d+1 of an attempt's Philox stream (device-generated uniforms only):
-// the component choice of a mixture proposal.
-CHILI_DEV double mixture_uniform(const MCArgs& A, uint64_t att) {
+// the component choice of a mixture proposal, the accept / reject decision of
+// the unweighting stage.
+CHILI_DEV double aux_uniform(const MCArgs& A, uint64_t att) {
     double u0, u1;
     philox_pair(A.seed, A.stream, att, (uint32_t)((A.d + 1) & ~1), u0, u1);
     return ((A.d + 1) & 1) ? u1 : u0;

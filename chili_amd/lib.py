@@ -26,7 +26,7 @@ EXPORTS = [
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
     "chili_flow_train", "chili_gemm_bf16_test",
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
-    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_channels_kernel_source", "chili_channels_compile_check",
+    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_mc_generate", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
 
 
@@ -84,6 +84,8 @@ def load():
     L.chili_flow_get_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_mixture.argtypes = [P, C.c_double]
+    L.chili_mc_generate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_double, VP, VP, I64, C.POINTER(abi.Stats),
+                                    C.POINTER(C.c_int64)]
     L.chili_flow_get_grads.argtypes = [P, C.c_int, F]
     L.chili_flow_log_density.argtypes = [P, C.c_int, I64, VP, VP, VP]
     L.chili_flow_sample.argtypes = [P, C.c_int, I64, VP, VP]
@@ -215,6 +217,14 @@ class MultiChannelGPU:
         check(load().chili_mc_iterate(self.h, C.byref(uni), a0, ncalls, 1 if train else 0, C.byref(st), _d(T), _d(hist),
                                       C.byref(na), vals_ptr, vals_cap))
         return st, T, hist, na.value
+
+    def generate(self, uni, nevents, wmax, ev_ptr, ew_ptr, cap, a0=0):
+        """Unweighted events (chili_mc_generate): device buffers ev [4 ntot][cap],
+        ew [cap] (raw pointers); returns (stats, attempts)."""
+        st = abi.Stats()
+        na = C.c_int64()
+        check(load().chili_mc_generate(self.h, C.byref(uni), a0, nevents, wmax, ev_ptr, ew_ptr, cap, C.byref(st), C.byref(na)))
+        return st, na.value
 
     def specialize(self, enable=True):
         """Switch to kernels compiled at run time with this context's channel

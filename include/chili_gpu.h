@@ -178,6 +178,17 @@ int chili_mc_get_alpha(const chili_mc* m, double* alpha);
  * vals_dev (device, optional): value of every kept point, at most vals_cap. */
 int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, int train, chili_stats* stats,
                      double* T, double* hist, int64_t* n_attempts, double* vals_dev, int64_t vals_cap);
+/* MultiChannel::operator() (MultiChannel.hh:196-251) without training and
+ * with an unweighting PostProcess (MultiChannel.hh:240-245): a point of value
+ * v that passes the cut is kept with probability min(1, |v| / wmax), decided
+ * by the attempt's Philox uniform number d+1 (device-generated uniforms
+ * only); a rejected point adds a zero to stats and the loop retries until
+ * nevents are kept (a NaN value counts as 0, i.e. one rejection).  Device
+ * outputs: ev [4 ntot][cap] FP64 momenta of the kept events in attempt order
+ * (external legs, incoming first), ew [cap] event weights
+ * sign(v) max(1, |v| / wmax).  cap >= nevents. */
+int chili_mc_generate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t nevents, double wmax, double* ev_dev,
+                      double* ew_dev, int64_t cap, chili_stats* stats, int64_t* n_attempts);
 /* Per-attempt records of attempts [a0, a0+n) (no retry loop): the batch FFI
  * view (include/Interfaces/nanobind.hh:23-50) of MultiChannel::GeneratePoint
  * / GenerateWeight (MultiChannel.hh:141-165).  Host outputs, each optional:

@@ -1240,6 +1240,39 @@ int oracle_mc_iterate2(void* h, const chili_uniforms* U, uint64_t a0, int64_t nc
     return 0;
 }
 
+// MultiChannel::operator() without training and with an unweighting
+// PostProcess (MultiChannel.hh:240-245): a point of value v passing the cut
+// is kept when the attempt's uniform number d+1 is below |v| / wmax, else it
+// adds a zero and is retried.  This build's definition for a NaN value: 0,
+// so one rejection (the reference would decrement its counter twice).
+// events [nevents][ntot][4], ew [nevents] = sign(v) max(1, |v| / wmax).
+int oracle_mc_generate2(void* h, const chili_uniforms* U, uint64_t a0, int64_t nevents, double wmax, const double* mom_in,
+                        int64_t n_in, chili_stats* stats, int64_t* n_attempts, double* events, double* ew) {
+    OProc* o = (OProc*)h;
+    const size_t d = o->maps[0].ndims();
+    if (U->u) return -3;
+    Stats res;
+    uint64_t a = a0;
+    for (int64_t i = 0; i < nevents; ++i) {
+        if (mom_in && (int64_t)(a - a0) >= n_in) return -2;
+        Attempt A;
+        run_attempt(o, U, a, A, d, mom_in ? mom_in + (size_t)(a - a0) * o->ntot * 4 : nullptr);
+        const double u = pick_uniform(U, a, (uint32_t)(d + 1));
+        ++a;
+        if (A.status == 0) { --i; res.add(0); continue; }
+        const double v = A.val;
+        if (!(u < std::fabs(v) / wmax)) { --i; res.add(0); continue; }
+        res.add(v);
+        if (events)
+            for (size_t p = 0; p < o->ntot; ++p)
+                for (int q = 0; q < 4; ++q) events[((size_t)i * o->ntot + p) * 4 + q] = D(A.mom[p].c[q]);
+        if (ew) ew[i] = (std::fabs(v) > wmax) ? v / wmax : std::copysign(1.0, v);
+    }
+    res.out(stats);
+    *n_attempts = (int64_t)(a - a0);
+    return 0;
+}
+
 int oracle_mc_iterate(void* h, const chili_uniforms* U, uint64_t a0, int64_t ncalls, int train,
                       chili_stats* stats, double* T, double* hist, int64_t* n_attempts) {
     return oracle_mc_iterate2(h, U, a0, ncalls, train, nullptr, 0, stats, T, hist, n_attempts);

@@ -27,6 +27,8 @@ def lib(count=False):
         L.oracle_destroy.argtypes = [P]
         L.oracle_channel_desc.argtypes = [P, C.c_int, C.POINTER(abi.ChannelDesc)]
         L.oracle_set_grids.argtypes = [P, D, C.c_int]
+        L.oracle_mc_generate2.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_double, D, I64, C.POINTER(abi.Stats),
+                                          C.POINTER(C.c_int64), D, D]
         L.oracle_get_grids.argtypes = [P, D]
         L.oracle_set_alpha.argtypes = [P, D]
         L.oracle_get_alpha.argtypes = [P, D]
@@ -158,6 +160,20 @@ class Oracle:
         out = np.zeros(5)
         self.L.oracle_count_points(self.h, C.byref(uni), a0, n, _d(out))
         return dict(fwd_arith=out[0], fwd_trans=out[1], wgt_arith=out[2], wgt_trans=out[3], accepted=out[4], attempts=n)
+
+    def generate(self, uni, nevents, wmax, a0=0, mom_in=None):
+        """MultiChannel loop with the unweighting PostProcess; returns
+        (stats, attempts, events [nevents][ntot][4], event weights)."""
+        st = abi.Stats()
+        na = C.c_int64()
+        ev = np.zeros((nevents, self.ntot, 4))
+        ew = np.zeros(nevents)
+        mi = None if mom_in is None else np.ascontiguousarray(mom_in, dtype=np.float64)
+        rc = self.L.oracle_mc_generate2(self.h, C.byref(uni), a0, nevents, wmax, _d(mi) if mi is not None else None,
+                                        0 if mi is None else mi.shape[0], C.byref(st), C.byref(na), _d(ev), _d(ew))
+        if rc != 0:
+            raise RuntimeError(f"oracle_mc_generate rc={rc}")
+        return st, na.value, ev, ew
 
     def iterate(self, uni, ncalls, a0=0, train=True, mom_in=None):
         """One MultiChannel iteration; mom_in (n, ntot, 4) replaces the forward map."""
