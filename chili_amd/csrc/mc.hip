@@ -597,6 +597,58 @@ __global__ __launch_bounds__(kTile) void k_gen_events(int NT, int64_t used, cons
     ew[col] = (fabs(v) > wmax) ? v / wmax : copysign(1.0, v);
 }
 
+// ---------------------------------------------------------------------------
+// Batch evaluation (the reference's ChiliCPU.evaluate / CallChili,
+// nanobind.hh:23-50, Tensorflow.cc:22-25): per row of randoms, one channel's
+// FSMapper::GeneratePoint on the randoms as they are (no VEGAS map), the
+// PreProcess cut (0 when it fails) and val = f * GenerateWeight (that
+// channel's phase-space weight; 0 when it is 0).
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_eval_single(MCArgs A, int ch, const double* __restrict__ rans, int64_t n, int64_t ld,
+                                                       double* __restrict__ out) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (i >= n) return;
+    double r[D + 1];
+#pragma unroll
+    for (int k = 0; k < D; ++k) r[k] = rans[i * ld + k];
+    r[D] = 0.0;
+    const ChanProg& p = A.progs[ch];
+    MomRF<2 * NT> P;
+    double m2[2 * NT];
+    generate_point<NT>(A, p, r, P, m2);
+    if (!cut_pass<NT>(A.f, P, A.sqrts)) { out[i] = 0.0; return; }
+    double vw;
+    const double ps = generate_weight<NT>(A, p, ch, P, &vw, nullptr, 0);
+    const double fv = integrand<NT>(A.f, P);
+    out[i] = (ps == 0) ? 0.0 : fv * ps;
+}
+
+// Rows [n][ld] of randoms -> SoA uniforms [k][n] (k < nk); counts entries
+// outside [0, 1) into bad.
+__global__ __launch_bounds__(kTile) void k_rows_to_soa(const double* __restrict__ rans, int64_t n, int64_t ld, int nk,
+                                                       double* __restrict__ u, int* __restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (i >= n) return;
+    int nb = 0;
+    for (int k = 0; k < nk; ++k) {
+        const double v = rans[i * ld + k];
+        nb += (v >= 0.0 && v < 1.0) ? 0 : 1;
+        u[(int64_t)k * n + i] = v;
+    }
+    if (nb) atomicAdd(bad, nb);
+}
+
+// Values of the kept points (compaction order) to attempt order; attempts
+// that failed the cut keep the 0 they were filled with.
+__global__ __launch_bounds__(kTile) void k_vals_to_rows(int64_t kept, const uint32_t* __restrict__ comp, const uint32_t* __restrict__ sorted,
+                                                        const double* __restrict__ val, double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (k >= kept) return;
+    out[sorted[comp[k]]] = val[k];
+}
+
 // ===========================================================================
 // host context
 // ===========================================================================
@@ -1870,6 +1922,60 @@ int chili_mc_generate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64
                      },
                      &m->gen_rate, 1e-4);
         if (n_attempts) *n_attempts = att;
+    });
+}
+
+// Batch evaluation on device buffers (the reference's ChiliCPU.evaluate /
+// CallChili, nanobind.hh:23-50, Tensorflow.cc:22-25): rans [n][ld] row-major.
+// channel in [0, C): that channel's mapping on the randoms as given (ld >= d),
+// out = f * phase-space weight (0 when the cut fails), the reference's
+// Evaluate.  channel = -1 (multichannel): column d picks the channel, the
+// VEGAS map of that channel, every channel's inverse map and the
+// multichannel weight as in chili_mc_iterate (ld >= d + 1, randoms in
+// [0, 1)); out = f * W, 0 when the cut fails.  No retries: one value per row.
+int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t n, int64_t ld, double* out_dev) {
+    return guarded([&] {
+        if (!m || !rans_dev || !out_dev) throw chili_error(CHILI_EINVAL, "null argument");
+        if (n <= 0 || n > (int64_t(1) << 26)) throw chili_error(CHILI_ERANGE, "n must be in [1, 2^26]");
+        if (channel < -1 || channel >= m->C) throw chili_error(CHILI_EINVAL, "channel out of range");
+        if (ld < m->d + (channel < 0 ? 1 : 0)) throw chili_error(CHILI_EINVAL, "row stride shorter than the randoms needed");
+        select_device(m->device);
+        if (channel >= 0) {
+            MCArgs A = make_args(m, nullptr, 0, 0, 0, 0);
+            const unsigned g = (unsigned)((n + kTile - 1) / kTile);
+            CHILI_DISPATCH_NT(m->NT, hipLaunchKernelGGL((k_eval_single<NT>), dim3(g), dim3(kTile), 0, m->stream, A, channel, rans_dev,
+                                                        n, ld, out_dev));
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipStreamSynchronize(m->stream));
+            return;
+        }
+        const int nk = m->d + 1;
+        m->d_u.resize((size_t)nk * n);
+        DevBuf<int> bad;
+        bad.resize(1);
+        const unsigned g = (unsigned)((n + kTile - 1) / kTile);
+        hipLaunchKernelGGL(chili_nf::k_fill<int>, dim3(1), dim3(kTile), 0, m->stream, (int64_t)1, 0, bad.ptr);
+        hipLaunchKernelGGL(k_rows_to_soa, dim3(g), dim3(kTile), 0, m->stream, rans_dev, n, ld, nk, m->d_u.ptr, bad.ptr);
+        int nbad = 0;
+        bad.download(&nbad, 1, m->stream);
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+        if (nbad) throw chili_error(CHILI_EINVAL, "randoms must lie in [0, 1)");
+        hipLaunchKernelGGL(chili_nf::k_fill<double>, dim3(g), dim3(kTile), 0, m->stream, n, 0.0, out_dev);
+        ensure_capacity(m, n);
+        const int nblk = (int)((n + kTile - 1) / kTile);
+        MCArgs A = make_args(m, m->d_u.ptr, n, 0, 0, 0);
+        const int tgrid = pick_sort_forward(m, A, n, PointRec{});
+        int64_t passed = 0, all = 0;
+        cutoff_and_compact(m, n, nblk, tgrid, std::numeric_limits<int64_t>::max(), passed, all);
+        if (passed > 0) {
+            m->d_gval.resize(passed);
+            const int mgrid = (int)((passed + kMainTile - 1) / kMainTile) + m->C;
+            CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, 0, m->d_gval.ptr, 0, passed, PointRec{}));
+            hipLaunchKernelGGL(k_vals_to_rows, dim3((unsigned)((passed + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, passed,
+                               m->d_comp.ptr, m->d_sorted.ptr, m->d_gval.ptr, out_dev);
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(m->stream));
     });
 }
 

@@ -26,7 +26,7 @@ EXPORTS = [
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
     "chili_flow_train", "chili_gemm_bf16_test",
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
-    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_mc_generate", "chili_channels_kernel_source", "chili_channels_compile_check",
+    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_mc_generate", "chili_mc_evaluate", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
 
 
@@ -84,6 +84,7 @@ def load():
     L.chili_flow_get_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_mixture.argtypes = [P, C.c_double]
+    L.chili_mc_evaluate.argtypes = [P, C.c_int, VP, I64, I64, VP]
     L.chili_mc_generate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_double, VP, VP, I64, C.POINTER(abi.Stats),
                                     C.POINTER(C.c_int64)]
     L.chili_flow_get_grads.argtypes = [P, C.c_int, F]
@@ -217,6 +218,23 @@ class MultiChannelGPU:
         check(load().chili_mc_iterate(self.h, C.byref(uni), a0, ncalls, 1 if train else 0, C.byref(st), _d(T), _d(hist),
                                       C.byref(na), vals_ptr, vals_cap))
         return st, T, hist, na.value
+
+    def evaluate(self, rans, channel=0):
+        """Batch op of the reference's ChiliCPU.evaluate / CallChili on the
+        device (chili_mc_evaluate): rans is a 2-D float64 device tensor (torch,
+        or any DLPack producer) with unit column stride, [n][d] for a single
+        channel's mapping, [n][d+1] for channel=-1 (multichannel); returns a
+        torch tensor [n] on the same device."""
+        import torch
+        t = rans if isinstance(rans, torch.Tensor) else torch.from_dlpack(rans)
+        if t.dtype != torch.float64 or not t.is_cuda or t.dim() != 2 or t.stride(1) != 1:
+            raise ChiliError("rans must be a 2-D float64 device tensor with contiguous rows")
+        if t.shape[1] < self.ndims + (1 if channel < 0 else 0):
+            raise ChiliError(f"rans needs {self.ndims + (1 if channel < 0 else 0)} columns")
+        out = torch.empty(t.shape[0], dtype=torch.float64, device=t.device)
+        check(load().chili_mc_evaluate(self.h, int(channel), C.c_void_p(t.data_ptr()), t.shape[0], t.stride(0),
+                                       C.c_void_p(out.data_ptr())))
+        return out
 
     def generate(self, uni, nevents, wmax, ev_ptr, ew_ptr, cap, a0=0):
         """Unweighted events (chili_mc_generate): device buffers ev [4 ntot][cap],

@@ -189,6 +189,15 @@ int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_
  * sign(v) max(1, |v| / wmax).  cap >= nevents. */
 int chili_mc_generate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t nevents, double wmax, double* ev_dev,
                       double* ew_dev, int64_t cap, chili_stats* stats, int64_t* n_attempts);
+/* Batch evaluation on device buffers, the reference's batch FFI
+ * (ChiliCPU.evaluate, include/Interfaces/nanobind.hh:23-50; CallChili,
+ * src/Interfaces/Tensorflow.cc:22-25): rans [n][ld] row-major FP64, out [n].
+ * channel in [0, C): that channel's FSMapper on the randoms as given
+ * (ld >= d), out = f * GenerateWeight, 0 when PreProcess fails -- the
+ * reference's Evaluate.  channel = -1 (multichannel): column d picks the
+ * channel, then VEGAS map, mapper, cut and the multichannel weight as in
+ * chili_mc_iterate (ld >= d + 1, randoms in [0, 1)); one value per row. */
+int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t n, int64_t ld, double* out_dev);
 /* Per-attempt records of attempts [a0, a0+n) (no retry loop): the batch FFI
  * view (include/Interfaces/nanobind.hh:23-50) of MultiChannel::GeneratePoint
  * / GenerateWeight (MultiChannel.hh:141-165).  Host outputs, each optional:
