@@ -19,7 +19,7 @@ ERRORS = {
 }
 
 F_UNIT, F_PROPPROD, F_RESONANT, F_EEMUMU = 0, 1, 2, 3
-CUTS_NONE, CUTS_PT_ETA_DR = 0, 1
+CUTS_NONE, CUTS_PT_ETA_DR, CUTS_KT_JETS = 0, 1, 2
 
 
 class Process(C.Structure):

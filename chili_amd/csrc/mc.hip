@@ -209,7 +209,32 @@ __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restri
     forward_one<NT, ZS>(A, A.progs[ch], ch, e, a, A.a0 + a, mom, nmax, acc, R, zsrc);
 }
 
-// Accepted attempts per block of kTile attempts (attempt order).
+// kT-jet PreProcess (CHILI_CUTS_KT_JETS), after the forward kernel's own
+// checks: the ŝ limit and the clustering of the final state of every
+// attempt that passed; a failure clears its flag (and record).
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_ktcut(const chili_integrand* __restrict__ f, int64_t n, const uint32_t* __restrict__ sorted,
+                                                 const double* __restrict__ mom, int64_t nmax, uint8_t* __restrict__ acc, PointRec R) {
+    const int64_t e = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t a = sorted[e];
+    if (!acc[a]) return;
+    V4 p[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+        p[i] = mk(mom[(int64_t)(4 * i) * nmax + e], mom[(int64_t)(4 * i + 1) * nmax + e], mom[(int64_t)(4 * i + 2) * nmax + e],
+                  mom[(int64_t)(4 * i + 3) * nmax + e]);
+    bool ok = true;
+    const double smax = f->ptmin;
+    if (smax > 0 && vmass2(vadd(p[0], p[1])) > smax * smax) ok = false;        // example.cc:29-32
+    if (ok) ok = kt_all_jets<NT - 2>(p + 2, f->drmin);
+    if (!ok) {
+        acc[a] = 0;
+        if (R.status) R.status[a] = 0;
+    }
+}
+
+// Accepted attempts per block of kTile attempts (attempt order).// Accepted attempts per block of kTile attempts (attempt order).
 __global__ __launch_bounds__(kTile) void k_acc_count(const uint8_t* __restrict__ acc, int64_t n, int* __restrict__ out) {
     __shared__ int s[kWaves];
     int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
@@ -962,6 +987,14 @@ void cutoff_and_compact(chili_mc* m, int64_t n, int nblk, int tgrid, int64_t nee
                        m->d_tiles2.ptr, kMainTile);
 }
 
+// The kT-jet PreProcess pass (no-op for the other cut kinds).
+void launch_ktcut(chili_mc* m, int64_t n, const PointRec& R) {
+    if (m->f.cuts != CHILI_CUTS_KT_JETS) return;
+    const unsigned g = (unsigned)((n + kTile - 1) / kTile);
+    CHILI_DISPATCH_NT(m->NT, hipLaunchKernelGGL((k_ktcut<NT>), dim3(g), dim3(kTile), 0, m->stream, m->d_f.ptr, n, m->d_sorted.ptr,
+                                                m->d_mom.ptr, m->nmax, m->d_acc.ptr, R));
+}
+
 // Channel picks, the stable sort by channel and the forward map + cut of n
 // attempts; returns the forward grid (tiles).
 int pick_sort_forward(chili_mc* m, const MCArgs& A, int64_t n, const PointRec& R) {
@@ -983,6 +1016,7 @@ int pick_sort_forward(chili_mc* m, const MCArgs& A, int64_t n, const PointRec& R
         HIP_CHECK(hipEventRecord(fev.second, m->stream));
         m->fev_live.push_back(fev);
     }
+    launch_ktcut(m, n, R);
     return tgrid;
 }
 
@@ -1220,6 +1254,7 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
         fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr, mixing ? m->d_fend.ptr : nullptr);
     }
     CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr));
+    launch_ktcut(m, n, R);
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
     if (kept == 0) return;
     CHILI_DISPATCH_NT(m->NT, launch_geom<NT>(m, A, kept));
@@ -1783,6 +1818,8 @@ int chili_mc_create(const chili_channels* ch, int nbins, const chili_integrand* 
         if (ch->set.topo.empty()) throw chili_error(CHILI_EINVAL, "process has no channels");
         if (ch->set.ntot < 4 || ch->set.ntot > 8) throw chili_error(CHILI_ERANGE, "device path supports 2->2 .. 2->6");
         if (ch->set.topo.size() > 4096) throw chili_error(CHILI_ERANGE, "at most 4096 channels");
+        if (f->cuts < CHILI_CUTS_NONE || f->cuts > CHILI_CUTS_KT_JETS) throw chili_error(CHILI_EINVAL, "unknown cut kind");
+        if (f->cuts == CHILI_CUTS_KT_JETS && !(f->drmin > 0)) throw chili_error(CHILI_EINVAL, "kT jets need a radius drmin > 0");
         select_device(device);
         auto m = std::make_unique<chili_mc>();
         m->device = device;
@@ -2158,6 +2195,8 @@ int chili_batch_create(int njobs, chili_mc* const* jobs, chili_batch** out) {
             if (!jobs[j]) throw chili_error(CHILI_EINVAL, "null job");
             if (jobs[j]->device != b->device) throw chili_error(CHILI_EINVAL, "jobs of a batch share one device");
             if (jobs[j]->B != b->B) throw chili_error(CHILI_EINVAL, "jobs of a batch share one bin count");
+            if (jobs[j]->f.cuts == CHILI_CUTS_KT_JETS)
+                throw chili_error(CHILI_EINVAL, "the kT-jet cut runs in the single-context path only");
             b->jobs.push_back(jobs[j]);
         }
         select_device(b->device);

@@ -355,6 +355,60 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, Mom
 // --------------------------------------------------------------------------
 CHILI_DEV double pair_s(const V4& a, const V4& b) { return vmass2(vadd(a, b)); }
 
+// JetCluster (src/Tools/JetCluster.cc:8-78; Q2i, Q2ij JetCluster.hh; R2
+// JetCluster.cc:80-84) for the question the kT-jet cut asks: do all N
+// partons end as jets, i.e. does the inclusive kT sequence merge none of
+// them?  Until a first merge no momentum changes, so every distance is
+// computed once; each step scans the remaining partons in their order as the
+// reference's loop does (beam distance of i, then pairs (i, j < i), strict <,
+// the first remaining parton's beam distance as the start except in the first
+// scan, which starts from DBL_MAX) and a pair at the minimum is that merge.
+template <int N>
+CHILI_DEV bool kt_all_jets(const V4* p, double R) {
+    double pt2[N], y[N], ph[N], d[N][N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        pt2[i] = vpt2(p[i]);
+        y[i] = vrap(p[i]);
+        ph[i] = vphi(p[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        d[i][i] = pt2[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            const double dy = y[i] - y[j], dphi = ph[i] - ph[j];   // phi in [0, 2 pi), not wrapped (JetCluster.cc:82)
+            const double mn = (pt2[j] < pt2[i]) ? pt2[j] : pt2[i];
+            d[i][j] = mn * ((dy * dy + dphi * dphi) / R / R);
+        }
+    }
+    unsigned rem = (1u << N) - 1;
+#pragma unroll
+    for (int step = 0; step < N; ++step) {
+        int ii = 0, jj = 0;
+        double dmin = 0;
+        bool seen = false;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            if (!((rem >> i) & 1u)) continue;
+            if (!seen) {
+                seen = true;
+                ii = jj = i;
+                dmin = (step == 0) ? __DBL_MAX__ : d[i][i];
+            }
+            if (d[i][i] < dmin) { dmin = d[i][i]; ii = i; jj = i; }
+#pragma unroll
+            for (int j = 0; j < i; ++j) {
+                if (!((rem >> j) & 1u)) continue;
+                if (d[i][j] < dmin) { dmin = d[i][j]; ii = i; jj = j; }
+            }
+        }
+        if (ii != jj) return false;
+        rem &= ~(1u << ii);
+    }
+    return true;
+}
+
 template <int NT>
 CHILI_DEV bool cut_pass(const chili_integrand* __restrict__ f, const MomRF<2 * NT>& PR, double sqrts) {
     V4 P[NT];
@@ -364,6 +418,8 @@ CHILI_DEV bool cut_pass(const chili_integrand* __restrict__ f, const MomRF<2 * N
     if (isnan(P[0].t)) return false;
     // physical incoming momenta: x1, x2 <= 1 (incoming legs carry E = -x sqrt(s)/2)
     if (!(-2 * P[0].t <= sqrts) || !(-2 * P[1].t <= sqrts)) return false;
+    // kT jets: the clustering is a pass of its own over the kept momenta (k_ktcut)
+    if (f->cuts == CHILI_CUTS_KT_JETS) return true;
     const double ptmin = f->ptmin, etamax = f->etamax, drmin = f->drmin;
     double y[NT], ph[NT];
     bool ok = true;

@@ -137,8 +137,15 @@ def integrand_unit(cuts=None):
 
 
 def _set_cuts(f, cuts):
+    """cuts: None, (ptmin, etamax, drmin), or ("kt", R[, sqrt_s_max]) for the
+    reference example's kT-jet PreProcess (every parton a jet of radius R)."""
     if cuts is None:
         f.cuts = abi.CUTS_NONE
+    elif cuts[0] == "kt":
+        f.cuts = abi.CUTS_KT_JETS
+        f.drmin = cuts[1]
+        f.ptmin = cuts[2] if len(cuts) > 2 else 0.0
+        f.etamax = 0.0
     else:
         f.cuts = abi.CUTS_PT_ETA_DR
         f.ptmin, f.etamax, f.drmin = cuts

@@ -110,7 +110,11 @@ enum {
 };
 enum {
     CHILI_CUTS_NONE = 0,
-    CHILI_CUTS_PT_ETA_DR = 1  /* x1, x2 <= 1; pT > ptmin, |y| < etamax, dR > drmin on the final state */
+    CHILI_CUTS_PT_ETA_DR = 1, /* x1, x2 <= 1; pT > ptmin, |y| < etamax, dR > drmin on the final state */
+    CHILI_CUTS_KT_JETS = 2    /* the reference example's PreProcess (src/example.cc:24-37): finite incoming
+                                 momenta with x1, x2 <= 1, (p1 + p2)^2 <= ptmin^2 when ptmin > 0, and the kT
+                                 clustering of the final state with radius R = drmin (JetCluster,
+                                 src/Tools/JetCluster.cc:8-84) returns one jet per parton */
 };
 typedef struct chili_integrand {
     int32_t kind;

@@ -811,11 +811,76 @@ struct Mapper {
 // Device-style integrands and the PreProcess cut (this project's definitions;
 // the reference takes them as user callbacks, Integrand.hh:41-47).
 // ---------------------------------------------------------------------------
+// JetCluster::operator() (src/Tools/JetCluster.cc:8-78), the reference's loop
+// with its position list and distance table: inclusive kT with
+// Q2i = pT^2, Q2ij = min(pT_i^2, pT_j^2) R2_ij, R2 = (dy^2 + dphi^2) / R / R
+// (phi in [0, 2 pi), unwrapped; JetCluster.hh, JetCluster.cc:80-84).  The
+// number of jets it returns for the final state p[2..].
+static size_t jet_cluster_count(const std::vector<V4>& parts, double R) {
+    std::vector<V4> tmp(parts.begin() + 2, parts.end());
+    size_t njets = 0;
+    size_t n = tmp.size();
+    auto q2i = [&](const V4& a) { return D(vpt2(a)); };
+    auto q2ij = [&](const V4& a, const V4& b) {
+        const double dy = D(vrap(a)) - D(vrap(b)), dphi = D(vphi(a)) - D(vphi(b));
+        return std::min(q2i(a), q2i(b)) * ((dy * dy + dphi * dphi) / R / R);
+    };
+    std::vector<size_t> pos(n);                       // remaining partons, in order
+    for (size_t i = 0; i < n; ++i) pos[i] = i;
+    std::vector<std::vector<double>> dist(n, std::vector<double>(n));
+    size_t ii = 0, jj = 0;
+    double dmin = std::numeric_limits<double>::max();
+    for (size_t i = 0; i < n; ++i) {
+        dist[i][i] = q2i(tmp[i]);
+        if (dist[i][i] < dmin) { dmin = dist[i][i]; ii = jj = i; }
+        for (size_t j = 0; j < i; ++j) {
+            dist[i][j] = q2ij(tmp[i], tmp[j]);
+            if (dist[i][j] < dmin) { dmin = dist[i][j]; ii = i; jj = j; }
+        }
+    }
+    while (true) {
+        if (ii != jj) tmp[pos[jj]] = vadd(tmp[pos[jj]], tmp[pos[ii]]);       // merge i into j
+        if (n == 0) break;
+        --n;
+        if (ii == jj) ++njets;                                               // beam distance: a jet
+        pos.erase(pos.begin() + (long)ii);
+        // distances of the parton now at position jj (when jj is past the
+        // end the reference recomputes entries of the removed parton, which
+        // no later scan reads)
+        if (jj < n) {
+            const size_t jx = pos[jj];
+            dist[jx][jx] = q2i(tmp[jx]);
+            for (size_t j = 0; j < n; ++j) {
+                const size_t o = pos[j];
+                if (o < jx) dist[jx][o] = q2ij(tmp[jx], tmp[o]);
+                else if (o > jx) dist[o][jx] = q2ij(tmp[jx], tmp[o]);
+            }
+        }
+        ii = jj = 0;
+        if (n == 0) continue;
+        dmin = dist[pos[0]][pos[0]];
+        for (size_t i = 0; i < n; ++i) {
+            const size_t ix = pos[i];
+            if (dist[ix][ix] < dmin) { dmin = dist[ix][ix]; ii = jj = i; }
+            for (size_t j = 0; j < i; ++j) {
+                const double dij = dist[ix][pos[j]];
+                if (dij < dmin) { dmin = dij; ii = i; jj = j; }
+            }
+        }
+    }
+    return njets;
+}
+
 static bool cut_pass(const chili_integrand& f, const std::vector<V4>& p, double sqrts) {
     if (f.cuts == CHILI_CUTS_NONE) return true;
     if (r_isnan(p[0].c[0])) return false;
     // physical incoming momenta: x1, x2 <= 1 (incoming legs carry E = -x sqrt(s)/2)
     if (!(D(-2 * p[0].c[0]) <= sqrts) || !(D(-2 * p[1].c[0]) <= sqrts)) return false;
+    if (f.cuts == CHILI_CUTS_KT_JETS) {
+        // the reference example's PreProcess, src/example.cc:24-37
+        if (f.ptmin > 0 && D(vmass2(vadd(p[0], p[1]))) > f.ptmin * f.ptmin) return false;
+        return jet_cluster_count(p, f.drmin) >= p.size() - 2;
+    }
     const size_t n = p.size();
     for (size_t i = 2; i < n; ++i) {
         if (!(D(vpt(p[i])) > f.ptmin)) return false;
@@ -1271,6 +1336,13 @@ int oracle_mc_generate2(void* h, const chili_uniforms* U, uint64_t a0, int64_t n
     res.out(stats);
     *n_attempts = (int64_t)(a - a0);
     return 0;
+}
+
+// Jets JetCluster returns for momenta [n][4] (incoming first).
+int oracle_jet_count(const double* mom, int n, double R) {
+    std::vector<V4> p(n);
+    for (int i = 0; i < n; ++i) p[i] = mk(mom[4 * i], mom[4 * i + 1], mom[4 * i + 2], mom[4 * i + 3]);
+    return (int)jet_cluster_count(p, R);
 }
 
 int oracle_mc_iterate(void* h, const chili_uniforms* U, uint64_t a0, int64_t ncalls, int train,

@@ -277,6 +277,13 @@ def stats(x):
     return st, err
 
 
+def jet_count(mom, R):
+    """Number of jets the reference's JetCluster(R) returns for momenta
+    (n, 4), incoming legs first."""
+    m = np.ascontiguousarray(mom, dtype=np.float64)
+    return lib().oracle_jet_count(_d(m), m.shape[0], C.c_double(R))
+
+
 def integrand(f, mom, sqrts):
     m = np.ascontiguousarray(mom, dtype=np.float64)
     ok = C.c_int()

@@ -40,6 +40,9 @@ CASES = {
     "gg_5g": lambda: (P.gluon_process(5, smax=4), P.integrand_propprod()),
     "tt_6j": lambda: (P.tt_process(smax=5), P.integrand_resonant()),
     "gg_4g_nocut": lambda: (P.gluon_process(4, smax=3), P.integrand_unit(None)),
+    # the reference example's PreProcess: every parton a kT jet (R = 0.4)
+    "gg_4g_kt": lambda: (P.gluon_process(4, smax=3), P.integrand_propprod(cuts=("kt", 0.4, 900.0))),
+    "tt_6j_kt": lambda: (P.tt_process(smax=5), P.integrand_resonant(cuts=("kt", 0.4))),
 }
 
 
@@ -173,7 +176,7 @@ def test_points_host_uniforms():
     compare_weights(gg, ora.points(uni, 0, n, mom_in=gg["mom"]), "host uniforms")
 
 
-@pytest.mark.parametrize("case", ["gg_2g", "gg_4g", "gg_5g", "tt_6j"])
+@pytest.mark.parametrize("case", ["gg_2g", "gg_4g", "gg_5g", "tt_6j", "gg_4g_kt"])
 def test_iterate_parity(case):
     """Stage 3: the retry loop and the accumulations (StatsData, alpha
     training, per-channel f^2 histograms), the oracle replaying the GPU's
@@ -313,6 +316,30 @@ def test_full_size_properties():
     m3, e3 = mean_err(s3)
     print(f"2^24: {m1:.6e} +- {e1:.1e} ({na1} attempts); 2^20: {m3:.6e} +- {e3:.1e}")
     assert abs(m1 - m3) < 5 * np.hypot(e1, e3)
+
+
+def test_kt_jet_cut_rejects_merged_partons():
+    """The kT-jet PreProcess removes exactly the attempts whose clustering
+    merges partons: on the same attempts without any cut, the GPU's kT
+    decisions equal the oracle's JetCluster on the GPU's own momenta."""
+    proc, _ = CASES["gg_4g_kt"]()
+    ch = lib.Channels(proc)
+    gk = lib.MultiChannelGPU(ch, 64, P.integrand_propprod(cuts=("kt", 0.4, 900.0)))
+    g0 = lib.MultiChannelGPU(ch, 64, P.integrand_propprod(cuts=None))
+    n = 6000
+    uni = abi.philox_uniforms(SEED + 40, stream=6)
+    rk, r0 = gk.points(uni, 0, n), g0.points(uni, 0, n)
+    base = r0["status"] != 0
+    with np.errstate(invalid="ignore"):
+        phys = base & (-2 * r0["mom"][:, 0, 0] <= SQRTS) & (-2 * r0["mom"][:, 1, 0] <= SQRTS)
+        shat = (r0["mom"][:, 0] + r0["mom"][:, 1])
+        shat = shat[:, 0] ** 2 - np.sum(shat[:, 1:] ** 2, axis=1)
+    want = np.zeros(n, bool)
+    for i in np.nonzero(phys & (shat <= 900.0 ** 2))[0]:
+        want[i] = O.jet_count(r0["mom"][i], 0.4) == r0["mom"].shape[1] - 2
+    assert np.array_equal(rk["status"] != 0, want)
+    print(f"kT jets: {want.sum()} of {phys.sum()} physical attempts keep every parton a jet")
+    assert 0 < want.sum() < phys.sum()
 
 
 def test_last_bin_reachable():

@@ -23,6 +23,11 @@ sign or a transposed index fails:
   integrands / cut ......... explicit numpy restatement of the definitions
   plain VEGAS loop ......... integral vs scipy quadrature of the closed-form sigma(s)
   retry loop ............... iterate() against the per-attempt records
+  JetCluster (kT cut) ...... independent textbook inclusive kT (400 random
+                             final states, two radii), closed forms (dR vs R,
+                             unwrapped phi)
+  unweighting loop ......... (GPU tests) the device stage against it on the
+                             same streams
 """
 import math
 import os
@@ -445,3 +450,83 @@ def test_retry_loop_accounting():
         for k in range(o.ndims):
             H[rec["pick"][i], k, rec["bins"][i, k]] += v2[i]
     assert np.allclose(hist, H, rtol=1e-12, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# JetCluster restatement (kT-jet PreProcess) against an independent textbook
+# inclusive kT: repeatedly the smallest of d_iB = pT_i^2 and
+# d_ij = min(pT_i^2, pT_j^2) (dy^2 + dphi^2) / R^2 (phi in [0, 2 pi), not
+# wrapped, as JetCluster.cc:80-84); a beam minimum makes a jet, a pair merges
+# (four-momentum sum).
+# ---------------------------------------------------------------------------
+def _textbook_kt_jets(mom, R):
+    import math
+
+    def pt2(p):
+        return p[1] * p[1] + p[2] * p[2]
+
+    def rap(p):
+        return 99.0 if p[0] == 0 else math.log((p[0] + p[3]) / (p[0] - p[3])) / 2
+
+    def phi(p):
+        a = math.atan2(p[2], p[1])
+        return a + 2 * math.pi if a < 0 else a
+
+    ps = [list(map(float, p)) for p in mom[2:]]
+    jets = 0
+    while ps:
+        best = (float("inf"), None, None)
+        for i in range(len(ps)):
+            if pt2(ps[i]) < best[0]:
+                best = (pt2(ps[i]), i, i)
+            for j in range(i):
+                dy, dp = rap(ps[i]) - rap(ps[j]), phi(ps[i]) - phi(ps[j])
+                d = min(pt2(ps[i]), pt2(ps[j])) * ((dy * dy + dp * dp) / R / R)
+                if d < best[0]:
+                    best = (d, i, j)
+        _, i, j = best
+        if i == j:
+            jets += 1
+        else:
+            ps[j] = [a + b for a, b in zip(ps[j], ps[i])]
+        del ps[i]
+    return jets
+
+
+def _random_final_state(rng, n, collinear=False):
+    ps = [[0.0, 0, 0, 0], [0.0, 0, 0, 0]]
+    for k in range(n):
+        pt, y, ph = rng.uniform(5, 200), rng.uniform(-3, 3), rng.uniform(0, 2 * np.pi)
+        if collinear and k > 0 and rng.random() < 0.5:
+            y0, ph0 = ps[-1][4], ps[-1][5]
+            y, ph = y0 + rng.normal(0, 0.2), (ph0 + rng.normal(0, 0.2)) % (2 * np.pi)
+        ps.append([pt * np.cosh(y), pt * np.cos(ph), pt * np.sin(ph), pt * np.sinh(y), y, ph])
+    return np.array([p[:4] for p in ps])
+
+
+def test_jet_cluster_vs_textbook_kt():
+    rng = np.random.default_rng(41)
+    counts = []
+    for trial in range(400):
+        n = 2 + trial % 5
+        mom = _random_final_state(rng, n, collinear=trial % 2 == 1)
+        for R in (0.4, 1.0):
+            got = O.jet_count(mom, R)
+            assert got == _textbook_kt_jets(mom, R), (trial, R)
+            counts.append(got < n)
+    # both outcomes occur
+    assert 0.1 < np.mean(counts) < 0.9
+
+
+def test_jet_cluster_closed_forms():
+    # two partons: one jet iff dR^2 < R^2 (then d_12 < min pT^2)
+    for dphi, want in ((0.3, 1), (0.5, 2)):
+        mom = np.array([[0.0] * 4, [0.0] * 4,
+                        [50.0, 50.0, 0.0, 0.0],
+                        [40.0, 40.0 * np.cos(dphi), 40.0 * np.sin(dphi), 0.0]])
+        assert O.jet_count(mom, 0.4) == want
+    # phi is not wrapped: partons at phi = 0.1 and 2 pi - 0.1 are 2 pi - 0.2 apart
+    mom = np.array([[0.0] * 4, [0.0] * 4,
+                    [50.0, 50.0 * np.cos(0.1), 50.0 * np.sin(0.1), 0.0],
+                    [40.0, 40.0 * np.cos(-0.1), 40.0 * np.sin(-0.1), 0.0]])
+    assert O.jet_count(mom, 0.4) == 2
