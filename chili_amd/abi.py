@@ -92,6 +92,13 @@ class Stats(C.Structure):
         return ((self.sum2 / self.n - m * m) / (self.n - 1)) ** 0.5
 
 
+class CheckpointState(C.Structure):
+    """chili_checkpoint_state: MultiChannelParams, min_diff, result count."""
+    _fields_ = [("ncalls", C.c_uint64), ("niterations", C.c_uint64), ("rtol", C.c_double), ("nrefine", C.c_uint64),
+                ("beta", C.c_double), ("min_alpha", C.c_double), ("refine_size", C.c_double), ("max_bins", C.c_uint64),
+                ("min_diff", C.c_double), ("nresults", C.c_int64)]
+
+
 class Uniforms(C.Structure):
     _fields_ = [("u", C.POINTER(C.c_double)), ("stride", C.c_int64),
                 ("seed", C.c_uint64), ("stream", C.c_uint32), ("pad_", C.c_uint32)]

@@ -26,7 +26,7 @@ EXPORTS = [
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
     "chili_flow_train", "chili_gemm_bf16_test",
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
-    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_mc_generate", "chili_mc_evaluate", "chili_channels_kernel_source", "chili_channels_compile_check",
+    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_mc_generate", "chili_mc_evaluate", "chili_checkpoint_write", "chili_checkpoint_info", "chili_checkpoint_read", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
 
 
@@ -85,6 +85,10 @@ def load():
     L.chili_flow_set_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_mixture.argtypes = [P, C.c_double]
     L.chili_mc_evaluate.argtypes = [P, C.c_int, VP, I64, I64, VP]
+    L.chili_checkpoint_write.argtypes = [P, C.c_int, D, D, C.POINTER(abi.CheckpointState), C.POINTER(abi.Stats), C.c_char_p]
+    L.chili_checkpoint_info.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int64)]
+    L.chili_checkpoint_read.argtypes = [P, C.c_char_p, D, D, C.POINTER(abi.CheckpointState), C.POINTER(abi.Stats), I64]
     L.chili_mc_generate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_double, VP, VP, I64, C.POINTER(abi.Stats),
                                     C.POINTER(C.c_int64)]
     L.chili_flow_get_grads.argtypes = [P, C.c_int, F]
@@ -485,6 +489,37 @@ def grid_split(edges, nsplit=2):
     out = np.zeros((e.shape[0], nsplit * (e.shape[1] - 1) + 1))
     check(load().chili_grid_split(e.shape[0], e.shape[1] - 1, _d(e), nsplit, _d(out)))
     return out
+
+
+# ---- checkpoints: results.chili in the reference's layout -------------------
+STATE_DEFAULTS = dict(ncalls=10000, niterations=10, rtol=1e-5, nrefine=1, beta=0.25, min_alpha=1e-5, refine_size=1.5,
+                      max_bins=200, min_diff=0.0)   # MultiChannelParams defaults, MultiChannel.hh:38-43
+
+
+def save_checkpoint(path, channels, grids, best_weights, state=None, results=()):
+    """chili_checkpoint_write: grids [C][d][B+1], best_weights [C], state
+    (MultiChannelParams fields and min_diff, defaults as the reference's),
+    results: StatsData of the iterations (abi.Stats)."""
+    g = np.ascontiguousarray(grids, dtype=np.float64)
+    w = np.ascontiguousarray(best_weights, dtype=np.float64)
+    st = abi.CheckpointState(**{**STATE_DEFAULTS, **(state or {})}, nresults=len(results))
+    res = (abi.Stats * max(len(results), 1))(*results)
+    check(load().chili_checkpoint_write(channels.h, g.shape[-1] - 1, _d(g), _d(w), C.byref(st), res, str(path).encode()))
+
+
+def load_checkpoint(path, channels):
+    """chili_checkpoint_read: (grids [C][d][B+1] and best_weights [C] in the
+    channels' order, state dict, list of StatsData)."""
+    L = load()
+    nc, nd, nb, nr = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+    check(L.chili_checkpoint_info(str(path).encode(), C.byref(nc), C.byref(nd), C.byref(nb), C.byref(nr)))
+    g = np.zeros((nc.value, nd.value, nb.value + 1))
+    w = np.zeros(nc.value)
+    st = abi.CheckpointState()
+    res = (abi.Stats * max(nr.value, 1))()
+    check(L.chili_checkpoint_read(channels.h, str(path).encode(), _d(g), _d(w), C.byref(st), res, nr.value))
+    state = {k: getattr(st, k) for k, _ in abi.CheckpointState._fields_ if k != "nresults"}
+    return g, w, state, [res[i] for i in range(nr.value)]
 
 
 def alpha_adapt(alpha, train, beta=0.25, min_alpha=1e-5):

@@ -264,6 +264,37 @@ int    chili_alpha_adapt(int n, double* alpha, const double* train, double beta,
 /* MultiChannel::MaxDifference (src/Channel/MultiChannel.cc:138-154). */
 double chili_max_difference(int n, const double* train);
 
+/* ---- checkpoints: results.chili, the reference's file (MultiChannel::SaveAs /
+ *      LoadFrom, include/Channel/MultiChannel.hh:65-86), in its byte layout ---- */
+/* MultiChannel's own part of the file: MultiChannelParams
+ * (MultiChannel.hh:29-49, serialised MultiChannel.cc:48-59), min_diff and the
+ * number of StatsData entries of its summary (MultiChannel.cc:21-33). */
+typedef struct chili_checkpoint_state {
+    uint64_t ncalls, niterations;
+    double   rtol;
+    uint64_t nrefine;
+    double   beta, min_alpha, refine_size;
+    uint64_t max_bins;
+    double   min_diff;
+    int64_t  nresults;
+} chili_checkpoint_state;
+/* Writes results.chili for the channels of ch: per channel its grid
+ * (AdaptiveMap::Serialize, AdaptiveMap.cc:23-32; grids [C][d][nbins+1]) and
+ * its FSMapper (Channel.cc:7-20, ChannelNode.cc:35-48), then the MultiChannel
+ * (MultiChannel.cc:94-107) with best_weights [C] and results [nresults]. */
+int chili_checkpoint_write(const chili_channels* ch, int nbins, const double* grids, const double* best_weights,
+                           const chili_checkpoint_state* st, const chili_stats* results, const char* path);
+/* Sizes of a results.chili: channels, dimensions, bins, summary results. */
+int chili_checkpoint_info(const char* path, int* nchannels, int* ndims, int* nbins, int64_t* nresults);
+/* Reads results.chili (MultiChannel::Load, MultiChannel.hh:82-86) for the
+ * channels of ch: the file's channels are matched to ch's by description (the
+ * reference's channel order may differ), grids [C][d][nbins+1] and
+ * best_weights [C] come back in ch's order (best_weights are the alpha the
+ * reference resumes with, MultiChannel.cc:88).  The cuts are ch's: the file
+ * holds none (Channel.hh:49); its pT minima must equal ch's. */
+int chili_checkpoint_read(const chili_channels* ch, const char* path, double* grids, double* best_weights,
+                          chili_checkpoint_state* st, chili_stats* results, int64_t results_cap);
+
 /* ---- learned proposal: per-channel coupling-layer spline flows on the
  *      VEGAS hypercube.  No counterpart in the reference; the sampler it
  *      feeds replaces MultiChannel::GeneratePoint's uniforms
