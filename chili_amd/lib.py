@@ -38,6 +38,14 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # PyTorch-ROCm ships its own HIP runtime under the same soname as the
+    # system's; whichever loads first serves the whole process, and torch's
+    # device code only initialises on its own.  Load torch (when present)
+    # before this library so that both share torch's runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ChiliError(f"{LIB_PATH} is not built (run `python -m chili_amd.build`); there is no CPU fallback")
     L = C.CDLL(LIB_PATH)
