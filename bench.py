@@ -357,6 +357,19 @@ def main():
             scan[lam] = production(flow_step, f"flow mix {lam:g}")
         lam_best = max(scan, key=lambda k: scan[k]["unweighted_per_s"])
         flow = scan[lam_best]
+        # algorithmic BF16 MFMA work of the conditioner networks per step:
+        # flow-component attempts through the picked channel's inverse flow,
+        # every kept point through every channel's flow; per row and layer
+        # 2 (nA 64 + 2 64^2 + 64 nB (3K - 1)) flops
+        per_row = sum(2 * (len(A) * 64 + 2 * 64 * 64 + 64 * (d - len(A)) * 47) for A in
+                      [[i for i in range(d) if i not in set(Bl)] for Bl in fl.masks()])
+        atts_step = n / flow["acceptance"]
+        rows_step = atts_step * (1.0 - lam_best) + n * C
+        mlp_tflops = rows_step * per_row / (flow["ms_per_step"] * 1e-3) / 1e12
+        flow["mfma"] = dict(mlp_flops_per_row=per_row, rows_per_step=round(rows_step), achieved_tflops=round(mlp_tflops, 2),
+                            peak_tflops=2500.0, frac=round(mlp_tflops / 2500.0, 5),
+                            note="whole flow step (spline transforms, VEGAS and phase-space maps included) against the "
+                                 "dense BF16 MFMA peak")
         flow.update(dict(mixture=lam_best, mixture_scan={f"{k:g}": dict(unweighted_per_s=v["unweighted_per_s"], eps_unw=v["eps_unw"],
                                                                           w_rel_std=round(v["w_rel_std"], 4))
                                                           for k, v in scan.items()}))

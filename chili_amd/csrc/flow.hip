@@ -195,7 +195,7 @@ __global__ void k_swizzle(const float* __restrict__ p, int64_t pW1, int64_t pb1,
         } else {                                // W4: [3 npair tiles][s][lane][8]
             const int64_t f = (i - FL.oW4) >> 9;
             const int T = (int)(f / 4), s = (int)(f % 4), P = T / 3, tt = T % 3;
-            const int rp = 32 * tt + (lane & 31), ds = rp / 48, q = rp % 48, dim = 2 * P + ds;
+            const int fr = lane & 31, ds = (fr >> 2) & 1, q = 16 * tt + 4 * (fr >> 3) + (fr & 3), dim = 2 * P + ds;
             if (q < kRaw && dim < nB) v = p[pW4 + (int64_t)phi_k(s, h, jj) * out + dim * kRaw + q];
         }
         w[i] = f2bf(v);
@@ -487,17 +487,32 @@ void FlowEngine::backward(int f, int64_t nb, const double* dlq) {
     }
 }
 
+// Waves per workgroup: 8, or 4 where that keeps more waves resident per CU
+// given the kernel's registers and this layer's LDS (the weights are loaded
+// once per workgroup, so larger groups are preferred on ties; multiples of
+// four waves spread evenly over a CU's four SIMDs).
 template <bool INV, int NP>
-static void launch_couple_np(dim3 g, dim3 b, size_t lds, hipStream_t st, const CoupleArgs& a) {
-    HIP_CHECK(hipFuncSetAttribute((const void*)k_couple<kSplineBins, INV, NP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_couple<kSplineBins, INV, NP>), g, b, lds, st, a);
+static void launch_couple_np(dim3 g, size_t lds, hipStream_t st, const CoupleArgs& a) {
+    const void* fn = (const void*)k_couple<kSplineBins, INV, NP>;
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int nw = 8, best = 0;
+    for (int w : {8, 4}) {
+        int nb = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64 * w, lds));
+        if (nb * w > best) {
+            best = nb * w;
+            nw = w;
+        }
+    }
+    if (best == 0) throw chili_error(CHILI_ERANGE, "coupling layer does not fit one CU");
+    hipLaunchKernelGGL((k_couple<kSplineBins, INV, NP>), g, dim3(64 * nw), lds, st, a);
 }
 
-void FlowEngine::launch_couple(bool inverse, int npair, dim3 g, dim3 b, size_t lds, const CoupleArgs& a) {
+void FlowEngine::launch_couple(bool inverse, int npair, dim3 g, size_t lds, const CoupleArgs& a) {
 #define CHILI_COUPLE_CASE(NPV)                                                        \
     case NPV:                                                                         \
-        if (inverse) launch_couple_np<true, NPV>(g, b, lds, stream, a);              \
-        else launch_couple_np<false, NPV>(g, b, lds, stream, a);                     \
+        if (inverse) launch_couple_np<true, NPV>(g, lds, stream, a);                 \
+        else launch_couple_np<false, NPV>(g, lds, stream, a);                        \
         break;
     switch (npair) {
         CHILI_COUPLE_CASE(1)
@@ -528,10 +543,8 @@ void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, in
         const Layer& ly = layers[l];
         const FusedLayout FL = fused_layout(ly.nB);
         const size_t wbytes = ((FL.wel * 2 + 15) / 16) * 16, bbytes = ((FL.bel + 3) / 4) * 16;
-        const size_t stage = (size_t)32 * kStagePitch * sizeof(float);
-        const int nw = (int)std::min<size_t>(8, (163840 - wbytes - bbytes) / stage);
-        if (nw < 1) throw chili_error(CHILI_ERANGE, "coupling layer does not fit the LDS of one workgroup");
-        const size_t lds = wbytes + bbytes + nw * stage;
+        const size_t lds = wbytes + bbytes;
+        if (lds > 163840) throw chili_error(CHILI_ERANGE, "coupling layer does not fit the LDS of one workgroup");
         CoupleArgs a{};
         a.x = x;
         a.lq = lq;
@@ -555,7 +568,7 @@ void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, in
             a.B[i] = i < ly.nB ? ly.B[i] : 0;
         }
         a.shift = shift;
-        launch_couple(inverse, FL.npair, dim3(grid), dim3(64 * nw), lds, a);
+        launch_couple(inverse, FL.npair, dim3(grid), lds, a);
     }
 }
 

@@ -78,7 +78,7 @@ struct FlowEngine {
     // Fused passes over nfl flows (flow0 ...): rows of flow j at seg[j] ..
     // seg[j+1] (seg device ints) or j * stride .. + ucount; x in place, lq
     // accumulated; rows_bound >= total rows.
-    void launch_couple(bool inverse, int npair, dim3 g, dim3 b, size_t lds, const CoupleArgs& a);
+    void launch_couple(bool inverse, int npair, dim3 g, size_t lds, const CoupleArgs& a);
     void fused_pass(bool inverse, int nfl, int flow0, const int* seg, int64_t ucount, int64_t stride, int64_t rows_bound, double* x,
                     double* lq, const int* seg_end = nullptr);
     double loss_grad(int64_t nb, const double* r, const double* beta, const double* w2);

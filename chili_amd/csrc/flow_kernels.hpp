@@ -162,58 +162,88 @@ struct SplineBin {
     double X0, Y0, w, h, d0, d1;
 };
 
+static_assert(kMinW == kMinH, "one knot offset table serves both axes");
+
+// i minW in FP32, the same value whether i is a constant or a register
+__device__ __forceinline__ float knot_offset(int i) { return (float)((double)i * kMinW); }
+
+// Largest float not above v (v in [0, 1]).  For a float knot X,
+// v >= X holds exactly when f32_down(v) >= X, so the bin search compares in
+// FP32 and still agrees with the FP64 transform.
+__device__ __forceinline__ float f32_down(double v) {
+    const float f = (float)v;
+    return ((double)f > v) ? __builtin_bit_cast(float, __builtin_bit_cast(int, f) - 1) : f;
+}
+
 // Bin containing v (searched over X, or over Y when by_y); ew/eh receive the
 // exponentials of the logits (max subtracted), rw/rh the reciprocals of their sums.
-template <int K>
-__device__ __forceinline__ SplineBin spline_bin(const float* __restrict__ u, float shift, double v, bool by_y, float* ew, float* eh,
+// u is the 3K - 1 raw parameters (a pointer, or an array held in registers:
+// every index below is a compile-time constant).
+template <int K, class U>
+__device__ __forceinline__ SplineBin spline_bin(const U& u, float shift, double v, bool by_y, float* ew, float* eh,
                                                 float& rw, float& rh) {
     float mw = u[0], mh = u[K];
 #pragma unroll
     for (int i = 1; i < K; ++i) { mw = fmaxf(mw, u[i]); mh = fmaxf(mh, u[K + i]); }
     // softmax exponentials on the hardware exp2 (arguments <= 0, no overflow;
     // relative error a few ulp, which only moves the knots: the density stays
-    // normalised)
+    // normalised), with their prefix sums S_i = sum_{j<i} e_j
     constexpr float kLog2e = 1.4426950408889634f;
-    float sw = 0.f, sh = 0.f;
+    const float mwl = -mw * kLog2e, mhl = -mh * kLog2e;
+    float SW[K + 1], SH[K + 1];
+    SW[0] = 0.f;
+    SH[0] = 0.f;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-        ew[i] = __builtin_amdgcn_exp2f((u[i] - mw) * kLog2e); sw += ew[i];
-        eh[i] = __builtin_amdgcn_exp2f((u[K + i] - mh) * kLog2e); sh += eh[i];
+        ew[i] = __builtin_amdgcn_exp2f(fmaf(u[i], kLog2e, mwl));
+        eh[i] = __builtin_amdgcn_exp2f(fmaf(u[K + i], kLog2e, mhl));
+        SW[i + 1] = SW[i] + ew[i];
+        SH[i + 1] = SH[i] + eh[i];
     }
-    rw = 1.f / sw;
-    rh = 1.f / sh;
-    const float cw = (float)(1.0 - K * kMinW), ch = (float)(1.0 - K * kMinH);
-    float X = 0.f, Y = 0.f, X0 = 0.f, Y0 = 0.f, X1 = 1.f, Y1 = 1.f;
+    rw = 1.f / SW[K];
+    rh = 1.f / SH[K];
+    // knots X_i = fma((1 - K minW) / sum, S_i, i minW) for 0 < i < K (the
+    // same expression wherever a knot is needed), X_0 = 0, X_K = 1; Y alike
+    const float aw = (float)(1.0 - K * kMinW) * rw, ah = (float)(1.0 - K * kMinH) * rh;
+    const float as = by_y ? ah : aw, ao = by_y ? aw : ah;
+    // exact: v >= X  <=>  vd >= X for every float X, so that the bin of v
+    // agrees with the FP64 transform (forward and inverse round-trip)
+    const float vd = f32_down(v);
+    float E = 0.f, E0 = 0.f, E1 = 1.f, P0 = 0.f, P1 = 0.f;
+    float ud0 = 0.f, ud1 = 0.f;          // derivative logits at the bin's left / right knot
     int k = 0;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-        const float Xn = (i == K - 1) ? 1.f : X + ((float)kMinW + cw * (ew[i] * rw));
-        const float Yn = (i == K - 1) ? 1.f : Y + ((float)kMinH + ch * (eh[i] * rh));
-        // compared in FP64 (the knots are exact there), so that the bin of v
-        // agrees with the FP64 transform: forward and inverse round-trip
-        const bool in = (i == 0) || (by_y ? (v >= (double)Y) : (v >= (double)X));
+        // searched axis: knot i is E, knot i + 1 is En
+        const float En = (i == K - 1) ? 1.f : fmaf(as, by_y ? SH[i + 1] : SW[i + 1], knot_offset(i + 1));
+        const bool in = (i == 0) || (vd >= E);
         k = in ? i : k;                  // selects, not branches
-        X0 = in ? X : X0;
-        Y0 = in ? Y : Y0;
-        X1 = in ? Xn : X1;
-        Y1 = in ? Yn : Y1;
-        X = Xn;
-        Y = Yn;
+        E0 = in ? E : E0;
+        E1 = in ? En : E1;
+        P0 = in ? (by_y ? SW[i] : SH[i]) : P0;
+        P1 = in ? (by_y ? SW[i + 1] : SH[i + 1]) : P1;
+        if (i >= 1) ud0 = in ? u[2 * K + i - 1] : ud0;
+        if (i <= K - 2) ud1 = in ? u[2 * K + i] : ud1;
+        E = En;
     }
+    // the other axis' knots k and k + 1 from their prefix sums
+    const float O0 = (k == 0) ? 0.f : fmaf(ao, P0, knot_offset(k));
+    const float O1 = (k == K - 1) ? 1.f : fmaf(ao, P1, knot_offset(k + 1));
+    const float X0 = by_y ? O0 : E0, X1 = by_y ? O1 : E1, Y0 = by_y ? E0 : O0, Y1 = by_y ? E1 : O1;
     SplineBin b;
     b.k = k;
     b.X0 = (double)X0;
     b.Y0 = (double)Y0;
     b.w = (double)X1 - b.X0;
     b.h = (double)Y1 - b.Y0;
-    b.d0 = (k == 0) ? 1.0 : kMinD + (double)softplusf(u[2 * K + k - 1] + shift);
-    b.d1 = (k == K - 1) ? 1.0 : kMinD + (double)softplusf(u[2 * K + k] + shift);
+    b.d0 = (k == 0) ? 1.0 : kMinD + (double)softplusf(ud0 + shift);
+    b.d1 = (k == K - 1) ? 1.0 : kMinD + (double)softplusf(ud1 + shift);
     return b;
 }
 
-// Forward transform x -> y, logdet = log dy/dx.
-template <int K>
-__device__ __forceinline__ double rqs_forward(const float* __restrict__ u, float shift, double x, double& logdet) {
+// Forward transform x -> y, jac = dy/dx.
+template <int K, class U>
+__device__ __forceinline__ double rqs_forward_j(const U& u, float shift, double x, double& jac) {
     float ew[K], eh[K], rw, rh;
     x = fmin(fmax(x, 0.0), 1.0);
     const SplineBin b = spline_bin<K>(u, shift, x, false, ew, eh, rw, rh);
@@ -221,13 +251,22 @@ __device__ __forceinline__ double rqs_forward(const float* __restrict__ u, float
     const double xi = (x - b.X0) * iw, s = b.h * iw, t = xi * (1 - xi);
     const double iden = 1.0 / (s + (b.d1 + b.d0 - 2 * s) * t);
     const double D = b.d1 * xi * xi + 2 * s * t + b.d0 * (1 - xi) * (1 - xi);
-    logdet = log(s * s * D * iden * iden);
+    jac = s * s * D * iden * iden;
     return b.Y0 + b.h * (s * xi * xi + b.d0 * t) * iden;
 }
 
-// Inverse transform y -> x; logdet receives log dy/dx at that x.
-template <int K>
-__device__ __forceinline__ double rqs_inverse(const float* __restrict__ u, float shift, double y, double& logdet) {
+// Forward transform x -> y, logdet = log dy/dx.
+template <int K, class U>
+__device__ __forceinline__ double rqs_forward(const U& u, float shift, double x, double& logdet) {
+    double jac;
+    const double y = rqs_forward_j<K>(u, shift, x, jac);
+    logdet = log(jac);
+    return y;
+}
+
+// Inverse transform y -> x; jac receives dy/dx at that x.
+template <int K, class U>
+__device__ __forceinline__ double rqs_inverse_j(const U& u, float shift, double y, double& jac) {
     float ew[K], eh[K], rw, rh;
     y = fmin(fmax(y, 0.0), 1.0);
     const SplineBin b = spline_bin<K>(u, shift, y, true, ew, eh, rw, rh);
@@ -241,8 +280,17 @@ __device__ __forceinline__ double rqs_inverse(const float* __restrict__ u, float
     const double t = xi * (1 - xi);
     const double iden = 1.0 / (s + E * t);
     const double D = b.d1 * xi * xi + 2 * s * t + b.d0 * (1 - xi) * (1 - xi);
-    logdet = log(s * s * D * iden * iden);
+    jac = s * s * D * iden * iden;
     return b.X0 + xi * b.w;
+}
+
+// Inverse transform y -> x; logdet receives log dy/dx at that x.
+template <int K, class U>
+__device__ __forceinline__ double rqs_inverse(const U& u, float shift, double y, double& logdet) {
+    double jac;
+    const double x = rqs_inverse_j<K>(u, shift, y, jac);
+    logdet = log(jac);
+    return x;
 }
 
 // Reverse-mode gradient of (y, logdet) = rqs_forward(x): given gy = dL/dy and
@@ -323,13 +371,15 @@ __device__ __forceinline__ double rqs_backward(const float* __restrict__ u, floa
 //     phi(s, h, j) = 16 s + 8 (j / 4) + 4 h + j % 4     (h = l / 32);
 // the swizzled weights use the same k order, so activations never leave
 // registers between hidden layers.  The output layer is evaluated per pair of
-// transformed dimensions (96 rows = 2 x 48, 47 used), staged through LDS
-// ([point][row], pitch 97 floats) and each lane then runs the FP64 spline of
-// one (point, dimension).
+// transformed dimensions (three 32-row tiles, 2 x 48 rows, 47 used per
+// dimension); its weight rows are ordered so that row f of tile t holds
+// parameter 16 t + 4 (f / 8) + f % 4 of dimension (f / 4) % 2, which leaves
+// lane half h with all parameters of dimension 2P + h in its own registers
+// (parameter q in register q % 16 of tile q / 16): each lane then runs the
+// spline of one (point, dimension) with no exchange between lanes.
 // ===========================================================================
 constexpr int kHid = 64;
 constexpr int kPairRows = 96;
-constexpr int kStagePitch = 97;
 constexpr int kRaw = 3 * 16 - 1;        // raw parameters per dimension (K = 16)
 
 __host__ __device__ constexpr int phi_k(int s, int h, int j) { return 16 * s + 8 * (j / 4) + 4 * h + (j % 4); }
@@ -373,24 +423,37 @@ __device__ __forceinline__ bf16x8 lds_frag(const bf16* base, int lane) {
     return *reinterpret_cast<const bf16x8*>(base + lane * 8);
 }
 
-// bias + ReLU of two 32-feature tiles, packed as the four B fragments of the
-// next layer
-__device__ __forceinline__ void relu_pack(const f32x16& a0, const f32x16& a1, const float* __restrict__ bias, int h, bf16x8* hb) {
+// Accumulators of a 32-feature tile starting at the layer's biases: lane half
+// h holds features 8 m + 4 h + i (m, i = 0..3) of the tile in register 4 m + i.
+__device__ __forceinline__ f32x16 bias_tile(const float* __restrict__ bias, int h) {
+    f32x16 c;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const float4 b4 = *reinterpret_cast<const float4*>(bias + 8 * m + 4 * h);
+        c[4 * m] = b4.x;
+        c[4 * m + 1] = b4.y;
+        c[4 * m + 2] = b4.z;
+        c[4 * m + 3] = b4.w;
+    }
+    return c;
+}
+
+// ReLU of two 32-feature tiles (biases already in), packed as the four B
+// fragments of the next layer
+__device__ __forceinline__ void relu_pack(const f32x16& a0, const f32x16& a1, bf16x8* hb) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const f32x16& a = (s < 2) ? a0 : a1;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int r = 8 * (s & 1) + j;
-            float v = a[r] + bias[phi_k(s, h, j)];
+            const float v = a[8 * (s & 1) + j];
             hb[s][j] = f2bf(v > 0.f ? v : 0.f);
         }
     }
 }
 
 // NP = pairs of transformed dimensions (ceil(nB / 2)), a compile-time
-// constant so that the per-lane operands of a tile live in registers; the
-// next tile's inputs are fetched while the current one computes.
+// constant so that the per-lane operands of a tile live in registers.
 template <int K, bool INV, int NP>
 __global__ __launch_bounds__(512) void k_couple(CoupleArgs a) {
     extern __shared__ __align__(16) unsigned char s_raw[];
@@ -411,7 +474,6 @@ __global__ __launch_bounds__(512) void k_couple(CoupleArgs a) {
     float* sB = reinterpret_cast<float*>(s_raw + ((FL.wel * 2 + 15) / 16) * 16);
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, h = lane >> 5, pl = lane & 31;
-    float* sStage = sB + ((FL.bel + 3) / 4) * 4 + wave * 32 * kStagePitch;
     {
         const int64_t blk = (int64_t)flow * a.L + a.layer;
         const uint4* gw = reinterpret_cast<const uint4*>(a.w + blk * a.wstride);
@@ -444,14 +506,11 @@ __global__ __launch_bounds__(512) void k_couple(CoupleArgs a) {
         for (int P = 0; P < NP; ++P) nxb[P] = (pv && bidx[P] >= 0) ? xp[bidx[P]] : 0.0;
         nlq = (pv && h == 0) ? a.lq[row] : 0.0;
     };
-    if (wave < ntile) fetch(wave);
-    f32x16 z;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) z[i] = 0.f;
     for (int64_t t = wave; t < ntile; t += nw) {
         const int64_t row = start + t * 32 + pl;
         const bool pv = t * 32 + pl < cnt;
         double* xp = a.x + row * a.d;
+        fetch(t);
         double xb[NP];
 #pragma unroll
         for (int P = 0; P < NP; ++P) xb[P] = nxb[P];
@@ -460,54 +519,62 @@ __global__ __launch_bounds__(512) void k_couple(CoupleArgs a) {
         bf16x8 b0;
 #pragma unroll
         for (int j = 0; j < 8; ++j) b0[j] = f2bf((float)nxa[j]);
-        if (t + nw < ntile) fetch(t + nw);
-        f32x16 c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW1, lane), b0, z, 0, 0, 0);
-        f32x16 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW1 + 512, lane), b0, z, 0, 0, 0);
+        f32x16 c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW1, lane), b0, bias_tile(sB + FL.ob1, h), 0, 0, 0);
+        f32x16 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW1 + 512, lane), b0, bias_tile(sB + FL.ob1 + 32, h),
+                                                            0, 0, 0);
         bf16x8 hb[4];
-        relu_pack(c0, c1, sB + FL.ob1, h, hb);
+        relu_pack(c0, c1, hb);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const bf16* Wq = sW + (q == 0 ? FL.oW2 : FL.oW3);
-            c0 = z;
-            c1 = z;
+            const float* bq = sB + (q == 0 ? FL.ob2 : FL.ob3);
+            c0 = bias_tile(bq, h);
+            c1 = bias_tile(bq + 32, h);
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(Wq + (0 * 4 + s) * 512, lane), hb[s], c0, 0, 0, 0);
                 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(Wq + (1 * 4 + s) * 512, lane), hb[s], c1, 0, 0, 0);
             }
-            relu_pack(c0, c1, sB + (q == 0 ? FL.ob2 : FL.ob3), h, hb);
+            relu_pack(c0, c1, hb);
         }
-        // output layer and splines, one pair of transformed dimensions at a time
-        double ldsum = 0.0;
+        // output layer and splines, one pair of transformed dimensions at a
+        // time: lane half h receives the 48 raw parameters of dimension
+        // 2P + h in c[q / 16][q % 16] (the swizzled W4 rows put them there),
+        // the accumulators starting at their biases
+        double jprod = 1.0;              // product of this lane's dy/dx (one log per tile)
 #pragma unroll
         for (int P = 0; P < NP; ++P) {
-            f32x16 c[3] = {z, z, z};
+            f32x16 c[3];
+            const float4* bq = reinterpret_cast<const float4*>(sB + FL.ob4 + kPairRows * P + 48 * h);
+#pragma unroll
+            for (int tt = 0; tt < 3; ++tt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float4 b4 = bq[4 * tt + v];
+                    c[tt][4 * v] = b4.x;
+                    c[tt][4 * v + 1] = b4.y;
+                    c[tt][4 * v + 2] = b4.z;
+                    c[tt][4 * v + 3] = b4.w;
+                }
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int tt = 0; tt < 3; ++tt)
                     c[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW4 + ((int64_t)(3 * P + tt) * 4 + s) * 512, lane),
                                                                     hb[s], c[tt], 0, 0, 0);
-#pragma unroll
-            for (int tt = 0; tt < 3; ++tt)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int rr = 32 * tt + 8 * (r >> 2) + 4 * h + (r & 3);
-                    sStage[pl * kStagePitch + rr] = c[tt][r] + sB[FL.ob4 + kPairRows * P + rr];
-                }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (pv && bidx[P] >= 0) {
-                double ld;
-                const float* u = sStage + pl * kStagePitch + 48 * h;
-                xp[bidx[P]] = INV ? rqs_inverse<K>(u, a.shift, xb[P], ld) : rqs_forward<K>(u, a.shift, xb[P], ld);
-                ldsum += ld;
+                float u[48];
+#pragma unroll
+                for (int q = 0; q < 48; ++q) u[q] = c[q >> 4][q & 15];
+                double jac;
+                xp[bidx[P]] = INV ? rqs_inverse_j<K>(u, a.shift, xb[P], jac) : rqs_forward_j<K>(u, a.shift, xb[P], jac);
+                jprod *= jac;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // one pair's parameters live at a time: keep the scheduler from
+            // hoisting the next pair's products above this spline
+            __builtin_amdgcn_sched_barrier(0);
         }
+        double ldsum = log(jprod);
         ldsum += __shfl_xor(ldsum, 32);
         if (pv && h == 0) a.lq[row] = lq0 + ldsum;
     }

@@ -15,8 +15,9 @@ Conventions restated here (the product computes the same):
     K width logits, K height logits, K-1 interior derivative inputs;
   * conditioner: xa = x[A]; h = relu(xa W1 + b1), two more ReLU layers,
     raw = h W4 + b4;
-  * knots (FP32): sm = softmax, w_i = minW + (1-K minW) sm_i, X_0 = 0,
-    X_{i+1} = X_i + w_i (sequential FP32 sums), X_K = 1, bin search in FP64;
+  * knots: X_i = i minW + (1-K minW) sum_{j<i} softmax_j, X_0 = 0, X_K = 1
+    (in FP32 as fma((1-K minW)/S_K, S_i, i minW) of the sequential prefix
+    sums S_i of the exponentials), bin search exact against the FP32 knots;
     transform and log-Jacobian in FP64 from those knots; derivatives d_0 = d_K = 1,
     d_i = minD + softplus_f32(u_i + shift), shift = log(expm1(1 - minD)) so
     that the zero network is the identity;
@@ -113,21 +114,27 @@ class FlowOracle:
         K = self.K
         if self.mode == "gpu":
             u32 = u.to(torch.float32)
+            cw = torch.tensor(1.0 - K * MIN_W, dtype=torch.float32)
+
+            def axis(e):
+                # S_i = sequential FP32 prefix sums of the exponentials,
+                # a = (1 - K minW) / S_K in FP32, knot i = fma(a, S_i, i minW)
+                # rounded once to FP32 (evaluated in FP64, where the product
+                # is exact), knot 0 = 0, knot K = 1
+                S = [torch.zeros_like(e[..., 0])]
+                for i in range(K):
+                    S.append(S[-1] + e[..., i])
+                a = cw * (1.0 / S[K])
+                out = [torch.zeros_like(e[..., 0])]
+                for i in range(1, K):
+                    c = float(np.float32(i * MIN_W))
+                    out.append((a.to(torch.float64) * S[i].to(torch.float64) + c).to(torch.float32))
+                out.append(torch.ones_like(e[..., 0]))
+                return torch.stack(out, -1).to(torch.float64)
+
             ew = torch.exp(u32[..., :K] - u32[..., :K].max(-1, keepdim=True).values.detach())
             eh = torch.exp(u32[..., K:2 * K] - u32[..., K:2 * K].max(-1, keepdim=True).values.detach())
-            sw = ew * (1.0 / ew.sum(-1, keepdim=True))
-            sh = eh * (1.0 / eh.sum(-1, keepdim=True))
-            cw = torch.tensor(1.0 - K * MIN_W, dtype=torch.float32)
-            w = torch.tensor(MIN_W, dtype=torch.float32) + cw * sw
-            h = torch.tensor(MIN_H, dtype=torch.float32) + cw * sh
-            X, Y = [torch.zeros_like(w[..., 0])], [torch.zeros_like(h[..., 0])]
-            for i in range(K - 1):
-                X.append(X[-1] + w[..., i])
-                Y.append(Y[-1] + h[..., i])
-            X.append(torch.ones_like(w[..., 0]))
-            Y.append(torch.ones_like(h[..., 0]))
-            X = torch.stack(X, -1).to(torch.float64)
-            Y = torch.stack(Y, -1).to(torch.float64)
+            X, Y = axis(ew), axis(eh)
             dz = torch.nn.functional.softplus(u32[..., 2 * K:] + SHIFT32).to(torch.float64)
             one = torch.ones_like(X[..., :1])
             return X, Y, torch.cat([one, MIN_D + dz, one], -1)
