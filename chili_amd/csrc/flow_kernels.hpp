@@ -439,16 +439,25 @@ __device__ __forceinline__ f32x16 bias_tile(const float* __restrict__ bias, int 
 }
 
 // ReLU of two 32-feature tiles (biases already in), packed as the four B
-// fragments of the next layer
+// fragments of the next layer.  Rounding to bf16 keeps the sign, so the ReLU
+// is applied after it, on pairs of bf16 bit patterns as signed 16-bit
+// integers (every negative value and -0 has the sign bit set and becomes +0).
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void relu_pack(const f32x16& a0, const f32x16& a1, bf16x8* hb) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const f32x16& a = (s < 2) ? a0 : a1;
+        u32x4 w;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const float v = a[8 * (s & 1) + j];
-            hb[s][j] = f2bf(v > 0.f ? v : 0.f);
+        for (int m = 0; m < 4; ++m) {
+            const f32x2 f = {a[8 * (s & 1) + 2 * m], a[8 * (s & 1) + 2 * m + 1]};
+            const i16x2 v = __builtin_bit_cast(i16x2, __builtin_convertvector(f, bf16x2));
+            w[m] = __builtin_bit_cast(unsigned, __builtin_elementwise_max(v, (i16x2)0));
         }
+        hb[s] = __builtin_bit_cast(bf16x8, w);
     }
 }
 
