@@ -70,7 +70,9 @@ def test_flow_identity_at_init():
     assert float((z - r).abs().max()) < 1e-7
 
 
-@pytest.mark.parametrize("d,L", [(10, 6), (16, 8), (4, 2)])
+# odd d leaves one lane half of the last dimension pair idle; d = 22 has six
+# pairs per layer (the widest coupling kernel)
+@pytest.mark.parametrize("d,L", [(10, 6), (16, 8), (4, 2), (7, 4), (3, 2), (22, 2)])
 def test_flow_density_vs_oracle(d, L):
     fl, ps = make_flow(d, L, 1, 0.05, seed=d)
     o = FO.FlowOracle(d, fl.masks(), mode="gpu")
@@ -98,8 +100,8 @@ def test_flow_density_vs_oracle(d, L):
     assert dl.max() < 0.02 * spread
 
 
-def test_flow_sample_inverts_density():
-    d = 10
+@pytest.mark.parametrize("d", [10, 7])
+def test_flow_sample_inverts_density(d):
     fl, _ = make_flow(d, 6, 1, 0.05, seed=4)
     n = 8192
     z = torch.rand(n, d, dtype=torch.float64, device="cuda")
