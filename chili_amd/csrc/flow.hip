@@ -527,13 +527,16 @@ void FlowEngine::launch_couple(bool inverse, int npair, dim3 g, size_t lds, cons
 }
 
 void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, int64_t ucount, int64_t stride, int64_t rows_bound,
-                            double* x, double* lq, const int* seg_end) {
+                            double* x, double* lq, const int* seg_end, int64_t rows_hint) {
     d_start.resize(nfl);
     d_count.resize(nfl);
     d_items.resize(nfl + 1);
     // rows per work item: large enough to amortise the workgroup's weight
-    // load, small enough to give every CU work on small passes
-    int64_t chunk = ((rows_bound / 512 + 255) / 256) * 256;
+    // load, small enough to give every CU several items at the expected row
+    // count (rows_hint, e.g. the flow component of a mixture; the grid still
+    // covers rows_bound)
+    const int64_t rows_exp = (rows_hint > 0) ? std::min(rows_hint, rows_bound) : rows_bound;
+    int64_t chunk = ((rows_exp / 1024 + 255) / 256) * 256;
     chunk = std::max<int64_t>(256, std::min<int64_t>(kChunk, chunk));
     hipLaunchKernelGGL(k_items, dim3(1), dim3(1), 0, stream, nfl, seg, seg_end, ucount, stride, chunk, d_start.ptr, d_count.ptr,
                        d_items.ptr);

@@ -80,7 +80,7 @@ struct FlowEngine {
     // accumulated; rows_bound >= total rows.
     void launch_couple(bool inverse, int npair, dim3 g, size_t lds, const CoupleArgs& a);
     void fused_pass(bool inverse, int nfl, int flow0, const int* seg, int64_t ucount, int64_t stride, int64_t rows_bound, double* x,
-                    double* lq, const int* seg_end = nullptr);
+                    double* lq, const int* seg_end = nullptr, int64_t rows_hint = -1);
     double loss_grad(int64_t nb, const double* r, const double* beta, const double* w2);
     void adam_step(double lr);
 };

@@ -40,6 +40,10 @@ __global__ __launch_bounds__(kTile) void k_pick(MCArgs A, int64_t n, uint16_t* _
     for (int c = threadIdx.x; c < A.C; c += blockDim.x) counts[(int64_t)c * nblk + blockIdx.x] = s_cnt[c];
 }
 
+// Attempts per sub-batch of the learned-proposal path (its buffers hold
+// C x attempts x d doubles twice: the r_j and the flows' working copy)
+constexpr int64_t kFlowSubMax = int64_t(1) << 23;
+
 // ---------------------------------------------------------------------------
 // Exclusive scan of an int array in three passes: sums of 1024-element blocks,
 // a single-workgroup scan of those sums, then each block's scan plus its
@@ -1251,7 +1255,8 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
         chili_nf::StreamSwap sw(fl->eng, m->stream);
         hipLaunchKernelGGL(chili_nf::k_fill<double>, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, n, 0.0,
                            m->d_lqs.ptr);
-        fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr, mixing ? m->d_fend.ptr : nullptr);
+        fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr, mixing ? m->d_fend.ptr : nullptr,
+                           mixing ? (int64_t)((1.0 - fl->eng.mix) * (double)n) + 1 : n);
     }
     CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr));
     launch_ktcut(m, n, R);
@@ -2105,7 +2110,7 @@ int chili_mc_flow_iterate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0;
-        iterate_loop(m, uni, a0, ncalls, int64_t(1) << 21, tot, att, stats,
+        iterate_loop(m, uni, a0, ncalls, kFlowSubMax, tot, att, stats,
                      [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
                          int64_t& used) {
                          run_subbatch_flow(m, fl, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, vals_dev, got, vals_cap,
@@ -2128,7 +2133,7 @@ int chili_mc_flow_data(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, u
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0, total_kept = 0;
-        iterate_loop(m, uni, a0, ncalls, int64_t(1) << 21, tot, att, stats,
+        iterate_loop(m, uni, a0, ncalls, kFlowSubMax, tot, att, stats,
                      [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
                          int64_t& used) {
                          run_subbatch_flow(m, fl, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, nullptr, 0, 0,
