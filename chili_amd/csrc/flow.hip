@@ -75,7 +75,7 @@ __global__ void k_add_rows(int64_t n, int nB, const double* __restrict__ ldt, do
 // entry and dL/dx on exit for the B dims; draw / drawb receive dL/draw (fp32
 // for the bias sums, bf16 for the next product).
 template <int K>
-__global__ void k_spline_bwd(int64_t n, int d, const double* __restrict__ x, const float* __restrict__ raw, int ldr,
+__global__ __launch_bounds__(kTh) void k_spline_bwd(int64_t n, int d, const double* __restrict__ x, const float* __restrict__ raw, int ldr,
                              const int* __restrict__ Bd, int nB, float shift, const double* __restrict__ glq,
                              double* __restrict__ gx, float* __restrict__ draw, bf16* __restrict__ drawb) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,9 +83,15 @@ __global__ void k_spline_bwd(int64_t n, int d, const double* __restrict__ x, con
     const int64_t i = t / nB;
     const int b = (int)(t - i * nB);
     const int c = Bd[b];
-    float* du = draw + i * ldr + b * (3 * K - 1);
+    float du[3 * K - 1];                 // registers: every index is a constant
     gx[i * d + c] = rqs_backward<K>(raw + i * ldr + b * (3 * K - 1), shift, x[i * d + c], gx[i * d + c], glq[i], du);
-    for (int q = 0; q < 3 * K - 1; ++q) drawb[i * ldr + b * (3 * K - 1) + q] = f2bf(du[q]);
+    float* dr = draw + i * ldr + b * (3 * K - 1);
+    bf16* drb = drawb + i * ldr + b * (3 * K - 1);
+#pragma unroll
+    for (int q = 0; q < 3 * K - 1; ++q) {
+        dr[q] = du[q];
+        drb[q] = f2bf(du[q]);
+    }
 }
 
 // gh (bf16) = relu'(h) * g (fp32)
@@ -121,13 +127,18 @@ __global__ __launch_bounds__(256) void k_colsum_part(int64_t n, int m, const flo
     if (ph == 0 && c < m) part[(int64_t)blockIdx.x * m + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) + s[2][threadIdx.x]) + s[3][threadIdx.x];
 }
 
-// dst[k] += sum_z part[z][k], z in order
-__global__ void k_sum_parts(int64_t cnt, int nz, const float* __restrict__ part, float* __restrict__ dst) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= cnt) return;
-    float s = 0;
-    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * cnt + k];
-    dst[k] += s;
+// dst[k] += sum_z part[z][k] in a fixed order: four interleaved sums per
+// element (z mod 4, one per wave of the workgroup), then combined in order.
+__global__ __launch_bounds__(256) void k_sum_parts(int64_t cnt, int nz, const float* __restrict__ part, float* __restrict__ dst) {
+    __shared__ float s[4][64];
+    const int t = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int64_t k = (int64_t)blockIdx.x * 64 + t;
+    float acc = 0;
+    if (k < cnt)
+        for (int z = ph; z < nz; z += 4) acc += part[(int64_t)z * cnt + k];
+    s[ph][t] = acc;
+    __syncthreads();
+    if (ph == 0 && k < cnt) dst[k] += ((s[0][t] + s[1][t]) + s[2][t]) + s[3][t];
 }
 
 // Weighted-variance loss over C flows; loss partial per block; dlq[j][i].
@@ -395,14 +406,14 @@ void FlowEngine::gemm_wgrad(int64_t n, int in, int out, const bf16* A, int lda, 
     dim3 g((out + kGT - 1) / kGT, (in + kGT - 1) / kGT, nz);
     hipLaunchKernelGGL((k_gemm<true, false>), g, dim3(kGThreads), 0, stream, in, out, (int)n, A, lda, G, ldg, part.ptr, out,
                        (const float*)nullptr, 0, (bf16*)nullptr, 0, chunk, (int64_t)in * out);
-    hipLaunchKernelGGL(k_sum_parts, dim3(nblk((int64_t)in * out)), dim3(kTh), 0, stream, (int64_t)in * out, nz, part.ptr, dW);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblk((int64_t)in * out, 64)), dim3(256), 0, stream, (int64_t)in * out, nz, part.ptr, dW);
 }
 void FlowEngine::colsum(int64_t n, int m, const float* g, float* db) {
-    const int64_t chunk = 256;
+    const int64_t chunk = 1024;
     const int nz = (int)((n + chunk - 1) / chunk);
     part2.resize((size_t)nz * m);
     hipLaunchKernelGGL(k_colsum_part, dim3(nz, (m + 63) / 64), dim3(256), 0, stream, n, m, g, chunk, part2.ptr);
-    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(m)), dim3(kTh), 0, stream, (int64_t)m, nz, part2.ptr, db);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(m, 64)), dim3(256), 0, stream, (int64_t)m, nz, part2.ptr, db);
 }
 
 // Density direction over nb points: x0 = r (fp64 [nb x d]) -> z = x_L; lq.
