@@ -623,14 +623,52 @@ double FlowEngine::loss_grad(int64_t nb, const double* r, const double* beta, co
     hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, dlqb.ptr, lpart.ptr);
     std::vector<double> lp(nbk);
     lpart.download(lp.data(), nbk, stream);
-    for (int f = 0; f < nflows; ++f) {
-        density(f, nb, r + (size_t)f * nb * d, lqb.ptr + (size_t)f * nb, nullptr, true);
-        backward(f, nb, dlqb.ptr + (size_t)f * nb);
+    // per flow (independent: its own slices of lqb / dlqb and its own
+    // gradients): density keeping the activations, then backward; with
+    // several flows, round robin over the training lanes
+    const int K = std::min(kTrainLanes, nflows);
+    if (K > 1) {
+        ensure_lanes(K);
+        HIP_CHECK(hipEventRecord(ev_ready, stream));
+        for (int k = 0; k < K; ++k) HIP_CHECK(hipStreamWaitEvent(lanes[k]->stream, ev_ready, 0));
+        for (int f = 0; f < nflows; ++f) {
+            LaneSwap sw(*this, *lanes[f % K]);
+            density(f, nb, r + (size_t)f * nb * d, lqb.ptr + (size_t)f * nb, nullptr, true);
+            backward(f, nb, dlqb.ptr + (size_t)f * nb);
+        }
+        for (int k = 0; k < K; ++k) {
+            HIP_CHECK(hipEventRecord(lanes[k]->done, lanes[k]->stream));
+            HIP_CHECK(hipStreamWaitEvent(stream, lanes[k]->done, 0));
+        }
+    } else {
+        density(0, nb, r, lqb.ptr, nullptr, true);
+        backward(0, nb, dlqb.ptr);
     }
     HIP_CHECK(hipStreamSynchronize(stream));
     double s = 0;
     for (double v : lp) s += v;
     return s / (double)nb;
+}
+
+void FlowEngine::ensure_lanes(int k) {
+    if (!ev_ready) HIP_CHECK(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    while ((int)lanes.size() < k) {
+        auto ln = std::make_unique<Lane>();
+        HIP_CHECK(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&ln->done, hipEventDisableTiming));
+        lanes.push_back(std::move(ln));
+    }
+}
+
+void FlowEngine::release_lanes() {
+    for (auto& ln : lanes) {
+        if (ln->stream) (void)hipStreamSynchronize(ln->stream);
+        if (ln->done) (void)hipEventDestroy(ln->done);
+        if (ln->stream) (void)hipStreamDestroy(ln->stream);
+    }
+    lanes.clear();
+    if (ev_ready) (void)hipEventDestroy(ev_ready);
+    ev_ready = nullptr;
 }
 
 void FlowEngine::adam_step(double lr) {
@@ -681,6 +719,7 @@ int chili_flow_destroy(chili_flow* f) {
         select_device(f->eng.device);
         hipStream_t s = f->eng.stream;
         if (s) hipStreamSynchronize(s);
+        f->eng.release_lanes();
         delete f;
         if (s) hipStreamDestroy(s);
     });

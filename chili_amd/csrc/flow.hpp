@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -38,6 +40,15 @@ struct Workspace {
     DevBuf<double> ldt;            // per (point, transformed dim) log-Jacobians
 };
 
+// A side stream with its own workspace: a training step runs the per-flow
+// passes (density with activations kept, backward) of several flows at once.
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    Workspace ws;
+    DevBuf<float> part, part2;
+};
+
 struct FlowEngine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -58,6 +69,10 @@ struct FlowEngine {
     DevBuf<int> d_items;
     DevBuf<double> xwork;
     static constexpr int64_t kChunk = 4096;   // largest rows per work item of a fused pass
+    // training lanes (created on first use, released with the engine)
+    static constexpr int kTrainLanes = 4;
+    std::vector<std::unique_ptr<Lane>> lanes;
+    hipEvent_t ev_ready = nullptr;
 
     int ws_lda() const { int m = 0; for (auto& l : layers) m = m > l.ldA ? m : l.ldA; return m; }
     int ws_out() const { int m = 0; for (auto& l : layers) m = m > l.out ? m : l.out; return m; }
@@ -83,6 +98,23 @@ struct FlowEngine {
                     double* lq, const int* seg_end = nullptr, int64_t rows_hint = -1);
     double loss_grad(int64_t nb, const double* r, const double* beta, const double* w2);
     void adam_step(double lr);
+    void ensure_lanes(int k);
+    void release_lanes();
+};
+
+// Runs a flow engine's per-flow passes on a lane (its stream, workspace and
+// split-K buffers) for a scope.
+struct LaneSwap {
+    FlowEngine& e;
+    Lane& l;
+    LaneSwap(FlowEngine& eng, Lane& lane) : e(eng), l(lane) { swap_all(); }
+    ~LaneSwap() { swap_all(); }
+    void swap_all() {
+        std::swap(e.stream, l.stream);
+        std::swap(e.ws, l.ws);
+        std::swap(e.part, l.part);
+        std::swap(e.part2, l.part2);
+    }
 };
 
 // Runs a flow engine's work on another stream (same device) for a scope.

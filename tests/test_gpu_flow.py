@@ -131,8 +131,10 @@ def test_flow_normalised_2d():
     assert abs(q.mean() - 1) < 5e-5
 
 
-def test_flow_gradients_vs_oracle():
-    d, L, C = 6, 4, 2
+# C = 5: more flows than training lanes (round robin over four streams)
+@pytest.mark.parametrize("C", [2, 5])
+def test_flow_gradients_vs_oracle(C):
+    d, L = 6, 4
     fl, ps = make_flow(d, L, C, 0.05, seed=8)
     o = [FO.FlowOracle(d, fl.masks(), mode="gpu") for _ in range(C)]
     rng = np.random.default_rng(11)
