@@ -483,23 +483,72 @@ struct FlowData {
     int64_t off, cap;
 };
 
+// Own-channel ranges of the density pass: a kept point whose attempt took
+// channel j's flow component (sorted positions [seg[j], fend[j]); without a
+// mixture fend = seg[j + 1]) already carries log q_j from the sampling pass.
+// comp is increasing, so those points are the kept indices own[2j] ..
+// own[2j+1]; channel j's density rows are the other kept points in order
+// (row i is kept point i below own[2j], else i + own[2j+1] - own[2j]), at
+// rows dseg[j] .. dend[j] of the [C][stride] layout.
+__global__ void k_own_ranges(int C, int64_t kept, const uint32_t* __restrict__ comp, const int* __restrict__ seg,
+                             const int* __restrict__ fend, int64_t stride, int64_t* __restrict__ own, int* __restrict__ dseg,
+                             int* __restrict__ dend) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= C) return;
+    auto lower = [&](uint32_t v) {
+        int64_t lo = 0, hi = kept;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (comp[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    const int64_t a = lower((uint32_t)seg[j]), b = lower((uint32_t)(fend ? fend[j] : seg[j + 1]));
+    own[2 * j] = a;
+    own[2 * j + 1] = b;
+    dseg[j] = (int)((int64_t)j * stride);
+    dend[j] = (int)((int64_t)j * stride + kept - (b - a));
+}
+
+// Density rows of every channel (see k_own_ranges) from the r_j records:
+// channel blockIdx.y, one double per thread (both sides contiguous).
+__global__ void k_dens_rows(int64_t kept, int d, int64_t stride, const int64_t* __restrict__ own, const double* __restrict__ rall,
+                            double* __restrict__ xw) {
+    const int j = blockIdx.y;
+    const int64_t lo = own[2 * j], skip = own[2 * j + 1] - lo;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (kept - skip) * d) return;
+    const int64_t base = (int64_t)j * stride * d;
+    xw[base + e] = rall[base + ((e < lo * d) ? e : e + skip * d)];
+}
+
 // W = 1 / sum_j a_j q_j, value = W == 0 ? 0 : f W (NaN -> 0, counted);
 // per-workgroup StatsData partials [n, min, max, sum, sum2, nf, nnan].
+// log q_j of kept point k: lq[j][k], or with own ranges (k_own_ranges) the
+// sampling pass's lqs for the picked channel's flow component and the
+// compacted density rows for the rest.
 __global__ __launch_bounds__(kTile) void k_combine(int C, int d, int64_t kept, int64_t stride, const double* __restrict__ abuf,
                                                    const double* __restrict__ lq, double mix, const double* __restrict__ fbuf,
                                                    const double* __restrict__ rall, double* __restrict__ part,
                                                    double* __restrict__ vals_out, int64_t vals_off, int64_t vals_cap, FlowData FD,
-                                                   const uint32_t* __restrict__ comp, const uint32_t* __restrict__ sorted, PointRec R) {
+                                                   const uint32_t* __restrict__ comp, const uint32_t* __restrict__ sorted, PointRec R,
+                                                   const int64_t* __restrict__ own, const double* __restrict__ lqs) {
     __shared__ double s_part[kWaves][kStatW];
     const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double val = 0, W = 0;
     bool valid = k < kept, isnanv = false;
+    auto lqv = [&](int j) -> double {
+        if (!own) return lq[(int64_t)j * stride + k];
+        const int64_t lo = own[2 * j], hi = own[2 * j + 1];
+        if (k >= lo && k < hi) return lqs[comp[k]];
+        return lq[(int64_t)j * stride + (k < lo ? k : k - (hi - lo))];
+    };
     if (valid) {
         double wsum = 0;
         for (int j = 0; j < C; ++j) {
             // the proposal's density: the flow's, blended with the uniform one
-            const double q = lq ? (1.0 - mix) * exp(lq[(int64_t)j * stride + k]) + mix : 1.0;
+            const double q = lq ? (1.0 - mix) * exp(lqv(j)) + mix : 1.0;
             wsum += abuf[(int64_t)j * stride + k] * q;
         }
         W = 1.0 / wsum;
@@ -523,7 +572,7 @@ __global__ __launch_bounds__(kTile) void k_combine(int C, int d, int64_t kept, i
             R.f[att] = fv;
             R.val[att] = val;
             if (R.ps)
-                for (int j = 0; j < C; ++j) R.ps[(int64_t)j * R.n + att] = lq ? lq[(int64_t)j * stride + k] : 0.0;
+                for (int j = 0; j < C; ++j) R.ps[(int64_t)j * R.n + att] = lq ? lqv(j) : 0.0;
         }
     }
     double n = valid ? 1.0 : 0.0, s1 = val, s2 = val * val, nf = (valid && val != 0) ? 1.0 : 0.0, nn = isnanv ? 1.0 : 0.0;
@@ -710,6 +759,8 @@ struct chili_mc {
     // learned-proposal path: z/r [fmax][d], r_j [C][fmax][d], a_j and log q_j
     // [C][fmax], f [fmax], sampling log-density [fmax], StatsData partials
     DevBuf<double> d_z, d_rall, d_abuf, d_lq, d_fbuf, d_lqs, d_fpart, d_xw;
+    DevBuf<int64_t> d_own;         // own-channel kept ranges [C][2] of the density pass
+    DevBuf<int> d_dseg, d_dend;    // density rows of each channel
     int64_t fmax = 0;
     int64_t nmax = 0;            // attempts per sub-batch
     double acc_rate = 0.25;      // running estimate of the cut acceptance
@@ -1263,21 +1314,38 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
     if (kept == 0) return;
     CHILI_DISPATCH_NT(m->NT, launch_geom<NT>(m, A, kept));
+    // own ranges need the [C][fmax] row layout within the int offsets of a pass
+    const bool own = fl && (int64_t)C * m->fmax < (int64_t)INT32_MAX;
     if (fl) {
         // log q_j(r_j) of every kept point and channel; the flows move a copy
-        // of r_j (the training records keep the originals)
+        // of r_j (the training records keep the originals).  The picked
+        // channel's flow-component points are skipped (their log q is the
+        // sampling pass's).
         chili_nf::StreamSwap sw(fl->eng, m->stream);
         const int64_t rows = (int64_t)C * m->fmax;
         m->d_xw.resize((size_t)rows * d);
-        HIP_CHECK(hipMemcpyAsync(m->d_xw.ptr, m->d_rall.ptr, (size_t)rows * d * sizeof(double), hipMemcpyDeviceToDevice, m->stream));
         hipLaunchKernelGGL(chili_nf::k_fill<double>, dim3((unsigned)((rows + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, rows, 0.0,
                            m->d_lq.ptr);
-        fl->eng.fused_pass(false, C, 0, nullptr, kept, m->fmax, (int64_t)C * kept, m->d_xw.ptr, m->d_lq.ptr);
+        if (own) {
+            m->d_own.resize((size_t)2 * C);
+            m->d_dseg.resize(C);
+            m->d_dend.resize(C);
+            hipLaunchKernelGGL(k_own_ranges, dim3((C + 63) / 64), dim3(64), 0, m->stream, C, kept, m->d_comp.ptr, m->d_seg.ptr,
+                               mixing ? m->d_fend.ptr : nullptr, m->fmax, m->d_own.ptr, m->d_dseg.ptr, m->d_dend.ptr);
+            hipLaunchKernelGGL(k_dens_rows, dim3((unsigned)((kept * d + kTile - 1) / kTile), (unsigned)C), dim3(kTile), 0, m->stream, kept, d,
+                               m->fmax, m->d_own.ptr, m->d_rall.ptr, m->d_xw.ptr);
+            const int64_t expect = (int64_t)((double)C * kept - (1.0 - fl->eng.mix) * (double)kept) + 1;
+            fl->eng.fused_pass(false, C, 0, m->d_dseg.ptr, 0, 0, (int64_t)C * kept, m->d_xw.ptr, m->d_lq.ptr, m->d_dend.ptr, expect);
+        } else {
+            HIP_CHECK(hipMemcpyAsync(m->d_xw.ptr, m->d_rall.ptr, (size_t)rows * d * sizeof(double), hipMemcpyDeviceToDevice, m->stream));
+            fl->eng.fused_pass(false, C, 0, nullptr, kept, m->fmax, (int64_t)C * kept, m->d_xw.ptr, m->d_lq.ptr);
+        }
     }
     const int cg = (int)((kept + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_combine, dim3(cg), dim3(kTile), 0, m->stream, C, d, kept, m->fmax, m->d_abuf.ptr, fl ? m->d_lq.ptr : nullptr,
                        fl ? fl->eng.mix : 0.0,
-                       m->d_fbuf.ptr, m->d_rall.ptr, m->d_fpart.ptr, vals, vals_off, vals_cap, FD, m->d_comp.ptr, m->d_sorted.ptr, R);
+                       m->d_fbuf.ptr, m->d_rall.ptr, m->d_fpart.ptr, vals, vals_off, vals_cap, FD, m->d_comp.ptr, m->d_sorted.ptr, R,
+                       own ? m->d_own.ptr : nullptr, m->d_lqs.ptr);
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, m->stream, (int64_t)cg, m->d_fpart.ptr, m->d_tot.ptr);
 }
 
