@@ -448,29 +448,47 @@ __global__ __launch_bounds__(kTile) void k_zgen(MCArgs A, int64_t n, const uint3
 }
 
 // All-channel inverse of every kept point: f, a_j = alpha_j / ps_j / vegas_j
-// ([j][stride]) and the hypercube points r_j ([j][stride][D]).
+// ([j][stride]) and the hypercube points r_j ([j][stride][D]).  Each channel's
+// rows of the workgroup are staged in LDS (pitch D + 1 doubles, no bank
+// conflicts) and written out contiguously.
 template <int NT>
 __global__ __launch_bounds__(kTile) void k_geom(MCArgs A, int64_t kept, const uint32_t* __restrict__ comp,
                                                 const double* __restrict__ mom, int64_t nmax, double* __restrict__ rall,
                                                 double* __restrict__ abuf, double* __restrict__ fbuf, int64_t stride) {
     constexpr int D = 3 * (NT - 2) - 2;
-    const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;
-    if (k >= kept) return;
-    const uint32_t sp = comp[k];
+    constexpr int PR = D + 1;
+    __shared__ double s_r[kTile * PR];
+    const int64_t k0 = (int64_t)blockIdx.x * kTile;
+    const int64_t k = k0 + threadIdx.x;
+    const bool valid = k < kept;
+    const int nrow = (int)std::min<int64_t>(kTile, kept - k0);
     MomRF<2 * NT> P;
+    if (valid) {
+        const uint32_t sp = comp[k];
 #pragma unroll
-    for (int i = 0; i < NT; ++i) {
-        P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
-        P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
-        P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
-        P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+        for (int i = 0; i < NT; ++i) {
+            P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
+            P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
+            P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
+            P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+        }
+        fbuf[k] = integrand<NT>(A.f, P);
     }
-    fbuf[k] = integrand<NT>(A.f, P);
+    double* row = s_r + threadIdx.x * PR;
     for (int j = 0; j < A.C; ++j) {
-        double vw;
-        const double ps = generate_weight<NT>(A, A.progs[j], j, P, &vw, nullptr, 0, nullptr, nullptr, 0,
-                                              rall + ((int64_t)j * stride + k) * D);
-        abuf[(int64_t)j * stride + k] = A.alpha[j] / ps / vw;
+        if (valid) {
+            // every slot is written by the channel program; 0.5 stands in
+            // for any a degenerate point leaves out
+#pragma unroll
+            for (int q = 0; q < D; ++q) row[q] = 0.5;
+            double vw;
+            const double ps = generate_weight<NT>(A, A.progs[j], j, P, &vw, nullptr, 0, nullptr, nullptr, 0, row);
+            abuf[(int64_t)j * stride + k] = A.alpha[j] / ps / vw;
+        }
+        __syncthreads();
+        double* dst = rall + ((int64_t)j * stride + k0) * D;
+        for (int e = threadIdx.x; e < nrow * D; e += kTile) dst[e] = s_r[(e / D) * PR + e % D];
+        __syncthreads();
     }
 }
 
