@@ -372,6 +372,28 @@ def test_flow_mixture_unbiased():
     assert abs(m1 - m2) < 4 * np.hypot(e1, e2)
 
 
+@pytest.mark.parametrize("mix", [0.0, 0.4])
+def test_flow_path_specialised_context(mix):
+    """A context with the run-time specialised forward kernel gives the flow
+    path's attempts, statistics and values bit for bit (two identically
+    adapted contexts, one specialised)."""
+    _, _, _, g0 = adapted_context()
+    _, _, _, g1 = adapted_context()
+    g1.specialize(1)
+    fl, _ = make_flow(g0.ndims, 4, g0.nchannels, 0.05, seed=29)
+    fl.set_mixture(mix)
+    n = 60000
+    out = []
+    for g in (g0, g1):
+        vals = torch.empty(n, dtype=torch.float64, device="cuda")
+        st, na = g.flow_iterate(fl, abi.philox_uniforms(SEED + 60, stream=3), n, vals_ptr=vals.data_ptr(), vals_cap=n)
+        torch.cuda.synchronize()
+        out.append((na, [st.n, st.min, st.max, st.sum, st.sum2, st.n_finite], vals.cpu().numpy()))
+    assert out[0][0] == out[1][0]
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 def weight_spread(step, n, stream):
     """StatsData mean and error over all attempts (rejected ones add zeros),
     and the relative std and <w>/w_max of the n accepted values."""
