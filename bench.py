@@ -359,12 +359,14 @@ def main():
         flow = scan[lam_best]
         # algorithmic BF16 MFMA work of the conditioner networks per step:
         # flow-component attempts through the picked channel's inverse flow,
-        # every kept point through every channel's flow; per row and layer
-        # 2 (nA 64 + 2 64^2 + 64 nB (3K - 1)) flops
+        # every kept point through every other channel's flow and, unless it
+        # took the flow component (its log q comes from sampling), its own
+        # (kept points of the flow component estimated as the fraction
+        # 1 - lambda); per row and layer 2 (nA 64 + 2 64^2 + 64 nB (3K - 1)) flops
         per_row = sum(2 * (len(A) * 64 + 2 * 64 * 64 + 64 * (d - len(A)) * 47) for A in
                       [[i for i in range(d) if i not in set(Bl)] for Bl in fl.masks()])
         atts_step = n / flow["acceptance"]
-        rows_step = atts_step * (1.0 - lam_best) + n * C
+        rows_step = atts_step * (1.0 - lam_best) + n * (C - (1.0 - lam_best))
         mlp_tflops = rows_step * per_row / (flow["ms_per_step"] * 1e-3) / 1e12
         flow["mfma"] = dict(mlp_flops_per_row=per_row, rows_per_step=round(rows_step), achieved_tflops=round(mlp_tflops, 2),
                             peak_tflops=2500.0, frac=round(mlp_tflops / 2500.0, 5),

@@ -409,7 +409,7 @@ void FlowEngine::gemm_wgrad(int64_t n, int in, int out, const bf16* A, int lda, 
     hipLaunchKernelGGL(k_sum_parts, dim3(nblk((int64_t)in * out, 64)), dim3(256), 0, stream, (int64_t)in * out, nz, part.ptr, dW);
 }
 void FlowEngine::colsum(int64_t n, int m, const float* g, float* db) {
-    const int64_t chunk = 1024;
+    const int64_t chunk = 256;
     const int nz = (int)((n + chunk - 1) / chunk);
     part2.resize((size_t)nz * m);
     hipLaunchKernelGGL(k_colsum_part, dim3(nz, (m + 63) / 64), dim3(256), 0, stream, n, m, g, chunk, part2.ptr);
