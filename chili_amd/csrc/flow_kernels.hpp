@@ -40,8 +40,8 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(int M, int N, int K, const b
                                                     const bf16* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
                                                     const float* __restrict__ bias, int relu, bf16* __restrict__ Cb, int ldcb,
                                                     int kchunk, int64_t part_stride) {
-    __shared__ bf16 sA[kGT][kGK + 8];
-    __shared__ bf16 sB[kGT][kGK + 8];
+    __shared__ __align__(16) bf16 sA[kGT][kGK + 8];
+    __shared__ __align__(16) bf16 sB[kGT][kGK + 8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * kGT, n0 = blockIdx.x * kGT;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -68,6 +68,16 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(int M, int N, int K, const b
 #pragma unroll
                 for (int j = 0; j < 8; ++j) sA[r8 + j][c] = (k < kend && m + j < M) ? A[(int64_t)k * lda + m + j] : f2bf(0.f);
             }
+        } else if (!A_T && vecA) {
+            // m-major: each thread loads 8 consecutive k of one row
+            const int r = tid >> 2, c8 = (tid & 3) * 8;        // 64 rows x 4 groups of 8 k
+            const int m = m0 + r, k = k0 + c8;
+            if (m < M && k + 8 <= kend) {
+                *reinterpret_cast<uint4*>(&sA[r][c8]) = *reinterpret_cast<const uint4*>(A + (int64_t)m * lda + k);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sA[r][c8 + j] = (m < M && k + j < kend) ? A[(int64_t)m * lda + k + j] : f2bf(0.f);
+            }
         } else {
             for (int i = tid; i < kGT * kGK; i += kGThreads) {
                 int r, c;
@@ -89,6 +99,15 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(int M, int N, int K, const b
             } else {
 #pragma unroll
                 for (int j = 0; j < 8; ++j) sB[r8 + j][c] = (k < kend && n + j < N) ? B[(int64_t)k * ldb + n + j] : f2bf(0.f);
+            }
+        } else if (B_T && vecB) {
+            const int r = tid >> 2, c8 = (tid & 3) * 8;
+            const int n = n0 + r, k = k0 + c8;
+            if (n < N && k + 8 <= kend) {
+                *reinterpret_cast<uint4*>(&sB[r][c8]) = *reinterpret_cast<const uint4*>(B + (int64_t)n * ldb + k);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sB[r][c8 + j] = (n < N && k + j < kend) ? B[(int64_t)n * ldb + k + j] : f2bf(0.f);
             }
         } else {
             for (int i = tid; i < kGT * kGK; i += kGThreads) {

@@ -440,3 +440,32 @@ def test_trained_flow_gains_over_plain_vegas():
 def C_ptr(t):
     import ctypes
     return ctypes.c_void_p(t.data_ptr())
+
+
+def test_flow_path_errors():
+    """Argument errors of the flow entry points are reported, not run: no
+    calls, flows of another shape, a mixture on host uniforms, a bad mixture
+    weight."""
+    _, _, _, gpu = adapted_context(iters=1)
+    fl, _ = make_flow(gpu.ndims, 2, gpu.nchannels, 0.0)
+    uni = abi.philox_uniforms(SEED + 70, stream=1)
+    with pytest.raises(lib.ChiliError):
+        gpu.flow_iterate(fl, uni, 0)
+    other, _ = make_flow(gpu.ndims + 1, 2, gpu.nchannels, 0.0)
+    with pytest.raises(lib.ChiliError):
+        gpu.flow_iterate(other, uni, 1000)
+    fewer, _ = make_flow(gpu.ndims, 2, gpu.nchannels + 1, 0.0)
+    with pytest.raises(lib.ChiliError):
+        gpu.flow_iterate(fewer, uni, 1000)
+    fl.set_mixture(0.5)
+    hu = abi.host_uniforms(np.random.default_rng(2).random((gpu.ndims + 1, 5000)))
+    with pytest.raises(lib.ChiliError):
+        gpu.flow_iterate(fl, hu, 100)
+    for bad in (-0.1, 1.5):
+        with pytest.raises(lib.ChiliError):
+            fl.set_mixture(bad)
+    # the context still works after the errors
+    fl.set_mixture(0.0)
+    st, na = gpu.flow_iterate(fl, uni, 1000)
+    assert na >= 1000 and st.n == na
+
