@@ -142,17 +142,20 @@ __global__ __launch_bounds__(256) void k_sum_parts(int64_t cnt, int nz, const fl
 }
 
 // Weighted-variance loss over C flows; loss partial per block; dlq[j][i].
+// With a mixture weight mix the proposal is q' = (1 - mix) q + mix, and the
+// loss is the second moment of the weights under it.
 __global__ void k_loss(int64_t n, int C, const double* __restrict__ lq, const double* __restrict__ beta,
-                       const double* __restrict__ w2, double inv_n, double* __restrict__ dlq, double* __restrict__ lpart) {
+                       const double* __restrict__ w2, double inv_n, double mix, double* __restrict__ dlq, double* __restrict__ lpart) {
     __shared__ double s[kTh];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double li = 0;
     if (i < n) {
         double ratio = 0;
-        for (int j = 0; j < C; ++j) ratio += beta[(int64_t)j * n + i] * exp(lq[(int64_t)j * n + i]);
+        for (int j = 0; j < C; ++j) ratio += beta[(int64_t)j * n + i] * ((1.0 - mix) * exp(lq[(int64_t)j * n + i]) + mix);
         li = w2[i] / ratio;
         for (int j = 0; j < C; ++j)
-            dlq[(int64_t)j * n + i] = -inv_n * w2[i] * beta[(int64_t)j * n + i] * exp(lq[(int64_t)j * n + i]) / (ratio * ratio);
+            dlq[(int64_t)j * n + i] =
+                -inv_n * w2[i] * beta[(int64_t)j * n + i] * (1.0 - mix) * exp(lq[(int64_t)j * n + i]) / (ratio * ratio);
     }
     s[threadIdx.x] = li;
     __syncthreads();
@@ -620,7 +623,8 @@ double FlowEngine::loss_grad(int64_t nb, const double* r, const double* beta, co
     HIP_CHECK(hipMemcpyAsync(xwork.ptr, r, (size_t)nflows * nb * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk((int64_t)nflows * nb)), dim3(kTh), 0, stream, (int64_t)nflows * nb, 0.0, lqb.ptr);
     fused_pass(false, nflows, 0, nullptr, nb, nb, (int64_t)nflows * nb, xwork.ptr, lqb.ptr);
-    hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, dlqb.ptr, lpart.ptr);
+    hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, mix, dlqb.ptr,
+                       lpart.ptr);
     std::vector<double> lp(nbk);
     lpart.download(lp.data(), nbk, stream);
     // per flow (independent: its own slices of lqb / dlqb and its own

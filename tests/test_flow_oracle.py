@@ -95,7 +95,8 @@ def test_sample_inverts_density():
     assert float((lq_s - lq_d).abs().max()) < 1e-8
 
 
-def test_loss_gradient_finite_difference():
+@pytest.mark.parametrize("mix", [0.0, 0.4])
+def test_loss_gradient_finite_difference(mix):
     d = 4
     C = 2
     o = [FlowOracle(d, masks_for(d, 2, seed=s), mode="exact") for s in (1, 2)]
@@ -106,7 +107,7 @@ def test_loss_gradient_finite_difference():
     beta = g.random((C, n)) + 0.1
     beta /= beta.sum(0)
     w2 = g.random(n) ** 2
-    L, grads = loss_and_grads(o, ps, r, beta, w2)
+    L, grads = loss_and_grads(o, ps, r, beta, w2, mix)
     rng = np.random.default_rng(9)
     for j in range(C):
         idx = rng.choice(o[j].nparams, 12, replace=False)
@@ -117,10 +118,31 @@ def test_loss_gradient_finite_difference():
             pp[j][k] += h
             pm[j][k] -= h
             with torch.no_grad():
-                Lp = float(variance_loss(o, pp, torch.tensor(r), torch.tensor(beta), torch.tensor(w2)))
-                Lm = float(variance_loss(o, pm, torch.tensor(r), torch.tensor(beta), torch.tensor(w2)))
+                Lp = float(variance_loss(o, pp, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix))
+                Lm = float(variance_loss(o, pm, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix))
             fd = (Lp - Lm) / (2 * h)
             assert abs(fd - grads[j][k]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (j, k, fd, grads[j][k])
+
+
+def test_mixture_loss_limits():
+    # identity flows (q = 1): the loss is mean w2 / sum_j beta_j whatever the
+    # mixture weight; an all-uniform proposal (mix = 1) has zero gradient
+    d, C = 3, 2
+    o = [FlowOracle(d, masks_for(d, 2, seed=s), mode="exact") for s in (1, 2)]
+    g = np.random.default_rng(6)
+    n = 32
+    r = g.random((C, n, d))
+    beta = g.random((C, n)) + 0.1
+    w2 = g.random(n) ** 2
+    want = float(np.mean(w2 / beta.sum(0)))
+    zero = [np.zeros(oo.nparams) for oo in o]
+    for mix in (0.0, 0.3, 1.0):
+        L, _ = loss_and_grads(o, zero, r, beta, w2, mix)
+        assert abs(L - want) <= 1e-12 * want
+    ps = [random_params(oo, 0.2, 30 + j).numpy() for j, oo in enumerate(o)]
+    L1, g1 = loss_and_grads(o, ps, r, beta, w2, 1.0)
+    assert abs(L1 - want) <= 1e-12 * want
+    assert all(np.abs(gj).max() == 0.0 for gj in g1)
 
 
 def test_gpu_mode_close_to_exact():

@@ -131,11 +131,13 @@ def test_flow_normalised_2d():
     assert abs(q.mean() - 1) < 5e-5
 
 
-# C = 5: more flows than training lanes (round robin over four streams)
-@pytest.mark.parametrize("C", [2, 5])
-def test_flow_gradients_vs_oracle(C):
+# C = 5: more flows than training lanes (round robin over four streams);
+# mix: the loss of the defensive-mixture proposal
+@pytest.mark.parametrize("C,mix", [(2, 0.0), (5, 0.0), (2, 0.4)])
+def test_flow_gradients_vs_oracle(C, mix):
     d, L = 6, 4
     fl, ps = make_flow(d, L, C, 0.05, seed=8)
+    fl.set_mixture(mix)
     o = [FO.FlowOracle(d, fl.masks(), mode="gpu") for _ in range(C)]
     rng = np.random.default_rng(11)
     n = 2048
@@ -145,7 +147,7 @@ def test_flow_gradients_vs_oracle(C):
     w2 = rng.gamma(1.0, 1.0, n)
     rd, bd, wd = dev(r), dev(beta), dev(w2)
     loss = fl.loss_grad(n, rd.data_ptr(), bd.data_ptr(), wd.data_ptr())
-    Lo, go = FO.loss_and_grads(o, ps, r, beta, w2)
+    Lo, go = FO.loss_and_grads(o, ps, r, beta, w2, mix)
     assert abs(loss - Lo) <= 1e-4 * abs(Lo)
     for j in range(C):
         g = fl.grads(j).astype(np.float64)
