@@ -313,7 +313,9 @@ int chili_flow_get_grads(chili_flow* f, int flow, float* out);
  * (1 - lambda) q + lambda (the uniform density on the hypercube; an attempt
  * takes its uniforms as the point when its uniform number d+1 is below
  * lambda), which bounds every weight by 1/lambda times the plain weight.
- * 0 <= lambda <= 1; default 0.  Needs device-generated uniforms. */
+ * chili_flow_loss_grad / chili_flow_train then minimise the second moment of
+ * the weights under that proposal.  0 <= lambda <= 1; default 0.  Needs
+ * device-generated uniforms. */
 int chili_flow_set_mixture(chili_flow* f, double lambda);
 /* device pointers: r, z [n][ndims], lq [n] = log q(r) */
 int chili_flow_log_density(chili_flow* f, int flow, int64_t n, const double* r_dev, double* lq_dev, double* z_dev);
