@@ -176,6 +176,8 @@ def main():
                     help="defensive mixture weights of the uniform density to run (comma list; the best is reported)")
     ap.add_argument("--flow-train-mix", type=float, default=0.0,
                     help="mixture weight of the proposal the flows are trained (and validated) for")
+    ap.add_argument("--flow-moment", type=int, default=4,
+                    help="training loss: this moment of the weights under the proposal (2: variance)")
     ap.add_argument("--no-specialize", action="store_true",
                     help="keep the generic kernels (default: the autotuner compiles kernels with the channel programs as "
                          "constants and keeps them when a production step is faster)")
@@ -317,6 +319,7 @@ def main():
         _, nk, _ = gpu.flow_data(None, abi.philox_uniforms(seed, stream=500), cap, r.data_ptr(), beta.data_ptr(), w2.data_ptr(), cap)
         fl = lib.FlowGPU(d, W["flow_layers"], C, seed=seed)
         fl.set_mixture(args.flow_train_mix)
+        fl.set_moment(args.flow_moment)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         losses = []
@@ -380,6 +383,7 @@ def main():
                                                           for k, v in scan.items()}))
         flow.update(dict(selected_epoch=best[1], validation=val_hist))
         flow.update(dict(layers=W["flow_layers"], hidden="3x64", spline_bins=16, train_points=int(nk), train_mix=args.flow_train_mix,
+                         loss_moment=args.flow_moment,
                          epochs=args.flow_epochs,
                          batch=args.flow_batch, lr=args.flow_lr, train_s=round(t_train, 2),
                          data_s=round(t1 - t0, 3), loss_first=losses[0] if losses else None,

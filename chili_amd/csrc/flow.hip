@@ -142,20 +142,29 @@ __global__ __launch_bounds__(256) void k_sum_parts(int64_t cnt, int nz, const fl
 }
 
 // Weighted-variance loss over C flows; loss partial per block; dlq[j][i].
-// With a mixture weight mix the proposal is q' = (1 - mix) q + mix, and the
-// loss is the second moment of the weights under it.
+// Loss: the p-th moment of the weights under the new proposal, estimated on
+// the old one's samples, L = mean w^p / ratio^(p-1) with ratio =
+// sum_j beta_j q'_j (w^2 = w2); with a mixture weight mix the proposal is
+// q' = (1 - mix) q + mix.  p = 2 is the variance loss; larger p weighs the
+// largest weights more.
 __global__ void k_loss(int64_t n, int C, const double* __restrict__ lq, const double* __restrict__ beta,
-                       const double* __restrict__ w2, double inv_n, double mix, double* __restrict__ dlq, double* __restrict__ lpart) {
+                       const double* __restrict__ w2, double inv_n, double mix, int p, double* __restrict__ dlq,
+                       double* __restrict__ lpart) {
     __shared__ double s[kTh];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double li = 0;
     if (i < n) {
         double ratio = 0;
         for (int j = 0; j < C; ++j) ratio += beta[(int64_t)j * n + i] * ((1.0 - mix) * exp(lq[(int64_t)j * n + i]) + mix);
-        li = w2[i] / ratio;
+        double wp = w2[i];                       // w^p
+        for (int q = 4; q <= p; q += 2) wp *= w2[i];
+        if (p & 1) wp *= sqrt(w2[i]);
+        double rp = ratio;                       // ratio^(p-1)
+        for (int q = 3; q <= p; ++q) rp *= ratio;
+        li = wp / rp;
         for (int j = 0; j < C; ++j)
             dlq[(int64_t)j * n + i] =
-                -inv_n * w2[i] * beta[(int64_t)j * n + i] * (1.0 - mix) * exp(lq[(int64_t)j * n + i]) / (ratio * ratio);
+                -(p - 1) * inv_n * wp * beta[(int64_t)j * n + i] * (1.0 - mix) * exp(lq[(int64_t)j * n + i]) / (rp * ratio);
     }
     s[threadIdx.x] = li;
     __syncthreads();
@@ -623,8 +632,8 @@ double FlowEngine::loss_grad(int64_t nb, const double* r, const double* beta, co
     HIP_CHECK(hipMemcpyAsync(xwork.ptr, r, (size_t)nflows * nb * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk((int64_t)nflows * nb)), dim3(kTh), 0, stream, (int64_t)nflows * nb, 0.0, lqb.ptr);
     fused_pass(false, nflows, 0, nullptr, nb, nb, (int64_t)nflows * nb, xwork.ptr, lqb.ptr);
-    hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, mix, dlqb.ptr,
-                       lpart.ptr);
+    hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, mix, moment,
+                       dlqb.ptr, lpart.ptr);
     std::vector<double> lp(nbk);
     lpart.download(lp.data(), nbk, stream);
     // per flow (independent: its own slices of lqb / dlqb and its own
@@ -744,6 +753,14 @@ int chili_flow_set_mixture(chili_flow* f, double lambda) {
         if (!f) throw chili_error(CHILI_EINVAL, "null flow");
         if (!(lambda >= 0.0 && lambda <= 1.0)) throw chili_error(CHILI_EINVAL, "mixture weight must lie in [0, 1]");
         f->eng.mix = lambda;
+    });
+}
+
+int chili_flow_set_moment(chili_flow* f, int p) {
+    return guarded([&] {
+        if (!f) throw chili_error(CHILI_EINVAL, "null flow");
+        if (p < 2 || p > 8) throw chili_error(CHILI_ERANGE, "loss moment must lie in [2, 8]");
+        f->eng.moment = p;
     });
 }
 

@@ -55,6 +55,7 @@ struct FlowEngine {
     int d = 0, L = 0, H = 64, nflows = 0;
     float shift = 0;   // softplus offset: slope 1 at zero input
     double mix = 0;    // proposal q' = (1 - mix) q + mix (uniform density on the hypercube)
+    int moment = 2;    // training loss: the moment-th moment of the weights under q' (2: variance)
     size_t nparams = 0;
     std::vector<Layer> layers;
     std::vector<Flow> flows;

@@ -26,7 +26,7 @@ EXPORTS = [
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
     "chili_flow_train", "chili_gemm_bf16_test",
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
-    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_mc_generate", "chili_mc_evaluate", "chili_checkpoint_write", "chili_checkpoint_info", "chili_checkpoint_read", "chili_channels_kernel_source", "chili_channels_compile_check",
+    "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_flow_set_moment", "chili_mc_generate", "chili_mc_evaluate", "chili_checkpoint_write", "chili_checkpoint_info", "chili_checkpoint_read", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
 
 
@@ -92,6 +92,7 @@ def load():
     L.chili_flow_get_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_mixture.argtypes = [P, C.c_double]
+    L.chili_flow_set_moment.argtypes = [P, C.c_int]
     L.chili_mc_evaluate.argtypes = [P, C.c_int, VP, I64, I64, VP]
     L.chili_checkpoint_write.argtypes = [P, C.c_int, D, D, C.POINTER(abi.CheckpointState), C.POINTER(abi.Stats), C.c_char_p]
     L.chili_checkpoint_info.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -353,6 +354,10 @@ class FlowGPU:
         """Proposal (1 - lam) q + lam (uniform on the hypercube) in the
         multichannel flow path (chili_flow_set_mixture)."""
         check(load().chili_flow_set_mixture(self.h, float(lam)))
+
+    def set_moment(self, p):
+        """Training loss: the p-th moment of the weights (chili_flow_set_moment)."""
+        check(load().chili_flow_set_moment(self.h, int(p)))
 
     def set_params(self, flow, p):
         p = np.ascontiguousarray(p, dtype=np.float32)

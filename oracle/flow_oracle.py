@@ -220,20 +220,21 @@ class FlowOracle:
         return x, lq
 
 
-def variance_loss(oracles, params, r, beta, w2, mix=0.0):
-    """L = mean_i w2_i / sum_j beta_ji q'_j(r_ji), q' = (1 - mix) exp(lq_j) + mix
-    (the proposal blended with the uniform density); r [C][n][d], beta [C][n]."""
+def variance_loss(oracles, params, r, beta, w2, mix=0.0, moment=2):
+    """L = mean_i w2_i^(p/2) / ratio_i^(p-1), ratio_i = sum_j beta_ji q'_j(r_ji),
+    q' = (1 - mix) exp(lq_j) + mix (the proposal blended with the uniform
+    density), p = moment (2: the variance loss); r [C][n][d], beta [C][n]."""
     ratio = 0
     for j, (o, p) in enumerate(zip(oracles, params)):
         lq, _ = o.density(p, r[j])
         ratio = ratio + beta[j] * ((1.0 - mix) * torch.exp(lq) + mix)
-    return torch.mean(w2 / ratio)
+    return torch.mean(w2 ** (0.5 * moment) / ratio ** (moment - 1))
 
 
-def loss_and_grads(oracles, params_np, r, beta, w2, mix=0.0):
+def loss_and_grads(oracles, params_np, r, beta, w2, mix=0.0, moment=2):
     """numpy in/out: params_np list of fp32 arrays; returns (loss, [grads])."""
     ps = [torch.tensor(np.asarray(p, dtype=np.float64), requires_grad=True) for p in params_np]
     L = variance_loss(oracles, ps, torch.as_tensor(r, dtype=torch.float64), torch.as_tensor(beta, dtype=torch.float64),
-                      torch.as_tensor(w2, dtype=torch.float64), mix)
+                      torch.as_tensor(w2, dtype=torch.float64), mix, moment)
     L.backward()
     return float(L), [p.grad.numpy().copy() for p in ps]

@@ -95,8 +95,8 @@ def test_sample_inverts_density():
     assert float((lq_s - lq_d).abs().max()) < 1e-8
 
 
-@pytest.mark.parametrize("mix", [0.0, 0.4])
-def test_loss_gradient_finite_difference(mix):
+@pytest.mark.parametrize("mix,moment", [(0.0, 2), (0.4, 2), (0.0, 4), (0.3, 3)])
+def test_loss_gradient_finite_difference(mix, moment):
     d = 4
     C = 2
     o = [FlowOracle(d, masks_for(d, 2, seed=s), mode="exact") for s in (1, 2)]
@@ -107,7 +107,7 @@ def test_loss_gradient_finite_difference(mix):
     beta = g.random((C, n)) + 0.1
     beta /= beta.sum(0)
     w2 = g.random(n) ** 2
-    L, grads = loss_and_grads(o, ps, r, beta, w2, mix)
+    L, grads = loss_and_grads(o, ps, r, beta, w2, mix, moment)
     rng = np.random.default_rng(9)
     for j in range(C):
         idx = rng.choice(o[j].nparams, 12, replace=False)
@@ -118,8 +118,8 @@ def test_loss_gradient_finite_difference(mix):
             pp[j][k] += h
             pm[j][k] -= h
             with torch.no_grad():
-                Lp = float(variance_loss(o, pp, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix))
-                Lm = float(variance_loss(o, pm, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix))
+                Lp = float(variance_loss(o, pp, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix, moment))
+                Lm = float(variance_loss(o, pm, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix, moment))
             fd = (Lp - Lm) / (2 * h)
             assert abs(fd - grads[j][k]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (j, k, fd, grads[j][k])
 
@@ -139,6 +139,11 @@ def test_mixture_loss_limits():
     for mix in (0.0, 0.3, 1.0):
         L, _ = loss_and_grads(o, zero, r, beta, w2, mix)
         assert abs(L - want) <= 1e-12 * want
+    # the p-th moment: mean w^p / (sum_j beta_j)^(p-1) for q = 1
+    for p in (3, 4):
+        Lp, _ = loss_and_grads(o, zero, r, beta, w2, 0.2, p)
+        wp = float(np.mean(w2 ** (p / 2) / beta.sum(0) ** (p - 1)))
+        assert abs(Lp - wp) <= 1e-12 * wp
     ps = [random_params(oo, 0.2, 30 + j).numpy() for j, oo in enumerate(o)]
     L1, g1 = loss_and_grads(o, ps, r, beta, w2, 1.0)
     assert abs(L1 - want) <= 1e-12 * want
