@@ -176,8 +176,9 @@ def main():
                     help="defensive mixture weights of the uniform density to run (comma list; the best is reported)")
     ap.add_argument("--flow-train-mix", type=float, default=0.0,
                     help="mixture weight of the proposal the flows are trained (and validated) for")
-    ap.add_argument("--flow-moment", type=int, default=4,
-                    help="training loss: this moment of the weights under the proposal (2: variance)")
+    ap.add_argument("--flow-moment", type=int, default=16,
+                    help="training loss: this moment of the weights under the proposal (2: variance; higher moments "
+                         "control the largest weights, which set the unweighting efficiency)")
     ap.add_argument("--no-specialize", action="store_true",
                     help="keep the generic kernels (default: the autotuner compiles kernels with the channel programs as "
                          "constants and keeps them when a production step is faster)")
@@ -309,7 +310,7 @@ def main():
 
     # ---- flow leg ----
     flow = None
-    if not args.no_flow:
+    def flow_leg():
         C, d = gpu.nchannels, gpu.ndims
         cap = args.flow_train
         r = torch.empty(C * cap * d, dtype=torch.float64, device="cuda:0")
@@ -391,6 +392,16 @@ def main():
                          vs_plain_unweighted=round(flow["unweighted_per_s"] / unw_s, 4),
                          vs_plain_unweighted_q999=round(flow["unweighted_per_s_q999"] / plain["unweighted_per_s_q999"], 4)))
         del fl
+        return flow
+
+    if not args.no_flow:
+        # a failing flow leg (e.g. training that diverged) is reported in
+        # the bench line; the plain leg still stands
+        try:
+            flow = flow_leg()
+        except (lib.ChiliError, RuntimeError, ValueError, ZeroDivisionError) as e:
+            log(f"  flow leg failed: {e}")
+            flow = dict(error=str(e))
 
     # ---- roofline of k_main ----
     ora = O.Oracle(proc, gpu.nbins, f, count=True)
@@ -418,7 +429,7 @@ def main():
 
     # plain multichannel VEGAS is the fallback: the flow leg is selected only
     # when it yields more unweighted events/s
-    selected = "flow" if (flow and flow["unweighted_per_s"] > unw_s) else "plain"
+    selected = "flow" if (flow and "unweighted_per_s" in flow and flow["unweighted_per_s"] > unw_s) else "plain"
     value = flow["unweighted_per_s"] if selected == "flow" else unw_s
     ms_step = flow["ms_per_step"] if selected == "flow" else ms_step
     out = {

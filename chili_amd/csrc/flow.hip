@@ -175,6 +175,71 @@ __global__ void k_loss(int64_t n, int C, const double* __restrict__ lq, const do
     if (threadIdx.x == 0) lpart[blockIdx.x] = s[0];
 }
 
+// Higher moments (p > 2) in log space: the objective is J = (1/p) log L_p
+// (same minimiser; its gradient dL_p / (p L_p) stays bounded where L_p's own
+// would overflow FP32).  Per point lt_i = log w_i^p - (p - 1) log ratio_i;
+// with m = max_i lt_i and s_i = exp(lt_i - m), S = sum_i s_i:
+// J = (m + log(S / n)) / p and dJ/dlq_j(i) = -(p - 1) / p (s_i / S)
+// beta_ji (1 - mix) q_j / ratio_i.  Three passes: terms and block maxima;
+// block sums of s_i; gradients (every block combines the partials in the
+// same fixed order).
+__global__ void k_loss_terms(int64_t n, int C, const double* __restrict__ lq, const double* __restrict__ beta,
+                             const double* __restrict__ w2, double mix, int p, double* __restrict__ lt, double* __restrict__ rat,
+                             double* __restrict__ bmax) {
+    __shared__ double s[kTh];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = -INFINITY;
+    if (i < n) {
+        double ratio = 0;
+        for (int j = 0; j < C; ++j) ratio += beta[(int64_t)j * n + i] * ((1.0 - mix) * exp(lq[(int64_t)j * n + i]) + mix);
+        v = (w2[i] > 0.0) ? 0.5 * p * log(w2[i]) - (p - 1) * log(ratio) : -INFINITY;
+        lt[i] = v;
+        rat[i] = ratio;
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = kTh / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
+}
+
+__device__ __forceinline__ double loss_max(const double* __restrict__ bmax, int nbk) {
+    double m = -INFINITY;
+    for (int b = 0; b < nbk; ++b) m = fmax(m, bmax[b]);
+    return m;
+}
+
+__global__ void k_loss_sums(int64_t n, int nbk, const double* __restrict__ lt, const double* __restrict__ bmax,
+                            double* __restrict__ bsum) {
+    __shared__ double s[kTh];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double m = loss_max(bmax, nbk);
+    s[threadIdx.x] = (i < n && lt[i] > -INFINITY) ? exp(lt[i] - m) : 0.0;
+    __syncthreads();
+    for (int off = kTh / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+
+__global__ void k_loss_grad_log(int64_t n, int C, const double* __restrict__ lq, const double* __restrict__ beta, double mix, int p,
+                                const double* __restrict__ lt, const double* __restrict__ rat, const double* __restrict__ bmax,
+                                const double* __restrict__ bsum, int nbk, double* __restrict__ dlq, double* __restrict__ lpart) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double m = loss_max(bmax, nbk);
+    double S = 0;
+    for (int b = 0; b < nbk; ++b) S += bsum[b];
+    if (i < n) {
+        const double c = (S > 0.0 && lt[i] > -INFINITY) ? -(double)(p - 1) / p * exp(lt[i] - m) / S * (1.0 - mix) / rat[i] : 0.0;
+        for (int j = 0; j < C; ++j) dlq[(int64_t)j * n + i] = c * beta[(int64_t)j * n + i] * exp(lq[(int64_t)j * n + i]);
+    }
+    // the host sums lpart over blocks and divides by n
+    if (threadIdx.x == 0) lpart[blockIdx.x] = (blockIdx.x == 0) ? (m + log(S / (double)n)) / p * (double)n : 0.0;
+}
+
 __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
                        float lr, float b1, float b2, float eps, float c1, float c2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -632,8 +697,21 @@ double FlowEngine::loss_grad(int64_t nb, const double* r, const double* beta, co
     HIP_CHECK(hipMemcpyAsync(xwork.ptr, r, (size_t)nflows * nb * d * sizeof(double), hipMemcpyDeviceToDevice, stream));
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk((int64_t)nflows * nb)), dim3(kTh), 0, stream, (int64_t)nflows * nb, 0.0, lqb.ptr);
     fused_pass(false, nflows, 0, nullptr, nb, nb, (int64_t)nflows * nb, xwork.ptr, lqb.ptr);
-    hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, mix, moment,
-                       dlqb.ptr, lpart.ptr);
+    if (moment == 2) {
+        hipLaunchKernelGGL(k_loss, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, 1.0 / (double)nb, mix, moment,
+                           dlqb.ptr, lpart.ptr);
+    } else {
+        ltb.resize((size_t)2 * nb);
+        lred.resize((size_t)2 * nbk);
+        double* lt = ltb.ptr;
+        double* rat = ltb.ptr + nb;
+        double* bmax = lred.ptr;
+        double* bsum = lred.ptr + nbk;
+        hipLaunchKernelGGL(k_loss_terms, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, w2, mix, moment, lt, rat, bmax);
+        hipLaunchKernelGGL(k_loss_sums, dim3(nbk), dim3(kTh), 0, stream, nb, (int)nbk, lt, bmax, bsum);
+        hipLaunchKernelGGL(k_loss_grad_log, dim3(nbk), dim3(kTh), 0, stream, nb, nflows, lqb.ptr, beta, mix, moment, lt, rat, bmax, bsum,
+                           (int)nbk, dlqb.ptr, lpart.ptr);
+    }
     std::vector<double> lp(nbk);
     lpart.download(lp.data(), nbk, stream);
     // per flow (independent: its own slices of lqb / dlqb and its own
@@ -759,7 +837,7 @@ int chili_flow_set_mixture(chili_flow* f, double lambda) {
 int chili_flow_set_moment(chili_flow* f, int p) {
     return guarded([&] {
         if (!f) throw chili_error(CHILI_EINVAL, "null flow");
-        if (p < 2 || p > 8) throw chili_error(CHILI_ERANGE, "loss moment must lie in [2, 8]");
+        if (p < 2 || p > 64) throw chili_error(CHILI_ERANGE, "loss moment must lie in [2, 64]");
         f->eng.moment = p;
     });
 }

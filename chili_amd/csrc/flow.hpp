@@ -62,6 +62,7 @@ struct FlowEngine {
     Workspace ws;
     DevBuf<float> part, part2;
     DevBuf<double> lqb, dlqb, lpart;
+    DevBuf<double> ltb, lred;      // higher-moment loss: per-point terms and ratios, block maxima and sums
     // fused inference: swizzled weight blocks [flow][layer] and the work table
     DevBuf<bf16> fusedW;
     DevBuf<float> fusedB;

@@ -1224,8 +1224,10 @@ void iterate_loop(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n
     // the reference loops until ncalls points pass; stop instead of spinning
     // when the cut (or NaN values) leave essentially nothing
     const int64_t max_attempts = 100000 * ncalls + (int64_t(1) << 24);
+    int all_nan = 0;   // consecutive sub-batches whose every kept value was NaN
     while (need > 0) {
         if (att > max_attempts) throw chili_error(CHILI_ERANGE, "no attempts pass the cut: retry limit reached");
+        if (all_nan >= 8) throw chili_error(CHILI_ERANGE, "every value is NaN: retry limit reached");
         // attempts expected to give `need` points, with a margin of ~1% plus
         // five binomial standard deviations; a shortfall runs another round
         const double expect = (double)need / std::max(rate, rate_floor);
@@ -1242,6 +1244,7 @@ void iterate_loop(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n
         HIP_CHECK(hipStreamSynchronize(m->stream));
         const int64_t nan_new = (int64_t)(nanc - prev_nan);
         prev_nan = nanc;
+        all_nan = (kept > 0 && nan_new == kept) ? all_nan + 1 : 0;
         rejected += used - kept;
         need -= kept - nan_new;
         got += kept;

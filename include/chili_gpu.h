@@ -319,9 +319,11 @@ int chili_flow_get_grads(chili_flow* f, int flow, float* out);
 int chili_flow_set_mixture(chili_flow* f, double lambda);
 /* Training loss of chili_flow_loss_grad / chili_flow_train: the p-th moment
  * of the weights under the (mixture) proposal, estimated on the training
- * records, mean w^p / (sum_j beta_j q'_j)^(p-1); p = 2 (default) is the
- * variance, larger p weighs the largest weights more.  2 <= p <= 8.  No
- * counterpart in the reference (the learned proposal is new). */
+ * records, L_p = mean w^p / (sum_j beta_j q'_j)^(p-1); p = 2 (default) is
+ * the variance, larger p weighs the largest weights more.  For p > 2 the
+ * loss returned and differentiated is (1/p) log L_p (same minimiser, bounded
+ * gradients).  2 <= p <= 64.  No counterpart in the reference (the learned
+ * proposal is new). */
 int chili_flow_set_moment(chili_flow* f, int p);
 /* device pointers: r, z [n][ndims], lq [n] = log q(r) */
 int chili_flow_log_density(chili_flow* f, int flow, int64_t n, const double* r_dev, double* lq_dev, double* z_dev);

@@ -231,10 +231,18 @@ def variance_loss(oracles, params, r, beta, w2, mix=0.0, moment=2):
     return torch.mean(w2 ** (0.5 * moment) / ratio ** (moment - 1))
 
 
+def training_loss(oracles, params, r, beta, w2, mix=0.0, moment=2):
+    """The objective the device trains: L_2 for the variance loss, (1/p) log L_p
+    for a higher moment p (same minimiser, bounded gradients)."""
+    L = variance_loss(oracles, params, r, beta, w2, mix, moment)
+    return L if moment == 2 else torch.log(L) / moment
+
+
 def loss_and_grads(oracles, params_np, r, beta, w2, mix=0.0, moment=2):
-    """numpy in/out: params_np list of fp32 arrays; returns (loss, [grads])."""
+    """numpy in/out: params_np list of fp32 arrays; returns (loss, [grads])
+    of training_loss."""
     ps = [torch.tensor(np.asarray(p, dtype=np.float64), requires_grad=True) for p in params_np]
-    L = variance_loss(oracles, ps, torch.as_tensor(r, dtype=torch.float64), torch.as_tensor(beta, dtype=torch.float64),
+    L = training_loss(oracles, ps, torch.as_tensor(r, dtype=torch.float64), torch.as_tensor(beta, dtype=torch.float64),
                       torch.as_tensor(w2, dtype=torch.float64), mix, moment)
     L.backward()
     return float(L), [p.grad.numpy().copy() for p in ps]

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 import torch
 
-from oracle.flow_oracle import FlowOracle, loss_and_grads, variance_loss
+from oracle.flow_oracle import FlowOracle, loss_and_grads, training_loss, variance_loss
 
 
 def masks_for(d, L, seed=1):
@@ -118,8 +118,8 @@ def test_loss_gradient_finite_difference(mix, moment):
             pp[j][k] += h
             pm[j][k] -= h
             with torch.no_grad():
-                Lp = float(variance_loss(o, pp, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix, moment))
-                Lm = float(variance_loss(o, pm, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix, moment))
+                Lp = float(training_loss(o, pp, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix, moment))
+                Lm = float(training_loss(o, pm, torch.tensor(r), torch.tensor(beta), torch.tensor(w2), mix, moment))
             fd = (Lp - Lm) / (2 * h)
             assert abs(fd - grads[j][k]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (j, k, fd, grads[j][k])
 
@@ -139,11 +139,14 @@ def test_mixture_loss_limits():
     for mix in (0.0, 0.3, 1.0):
         L, _ = loss_and_grads(o, zero, r, beta, w2, mix)
         assert abs(L - want) <= 1e-12 * want
-    # the p-th moment: mean w^p / (sum_j beta_j)^(p-1) for q = 1
+    # the p-th moment: mean w^p / (sum_j beta_j)^(p-1) for q = 1 (trained as
+    # its log over p)
     for p in (3, 4):
-        Lp, _ = loss_and_grads(o, zero, r, beta, w2, 0.2, p)
         wp = float(np.mean(w2 ** (p / 2) / beta.sum(0) ** (p - 1)))
+        Lp = float(variance_loss(o, [torch.tensor(z) for z in zero], torch.tensor(r), torch.tensor(beta), torch.tensor(w2), 0.2, p))
         assert abs(Lp - wp) <= 1e-12 * wp
+        Jp, _ = loss_and_grads(o, zero, r, beta, w2, 0.2, p)
+        assert abs(Jp - np.log(wp) / p) <= 1e-12
     ps = [random_params(oo, 0.2, 30 + j).numpy() for j, oo in enumerate(o)]
     L1, g1 = loss_and_grads(o, ps, r, beta, w2, 1.0)
     assert abs(L1 - want) <= 1e-12 * want

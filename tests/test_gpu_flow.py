@@ -133,7 +133,7 @@ def test_flow_normalised_2d():
 
 # C = 5: more flows than training lanes (round robin over four streams);
 # mix: the loss of the defensive-mixture proposal; moment: the loss's order
-@pytest.mark.parametrize("C,mix,moment", [(2, 0.0, 2), (5, 0.0, 2), (2, 0.4, 2), (2, 0.0, 4), (3, 0.3, 3)])
+@pytest.mark.parametrize("C,mix,moment", [(2, 0.0, 2), (5, 0.0, 2), (2, 0.4, 2), (2, 0.0, 4), (3, 0.3, 3), (2, 0.0, 8)])
 def test_flow_gradients_vs_oracle(C, mix, moment):
     d, L = 6, 4
     fl, ps = make_flow(d, L, C, 0.05, seed=8)
@@ -149,7 +149,7 @@ def test_flow_gradients_vs_oracle(C, mix, moment):
     rd, bd, wd = dev(r), dev(beta), dev(w2)
     loss = fl.loss_grad(n, rd.data_ptr(), bd.data_ptr(), wd.data_ptr())
     Lo, go = FO.loss_and_grads(o, ps, r, beta, w2, mix, moment)
-    assert abs(loss - Lo) <= 1e-4 * abs(Lo)
+    assert abs(loss - Lo) <= 1e-4 * max(1.0, abs(Lo))
     for j in range(C):
         g = fl.grads(j).astype(np.float64)
         rel = np.linalg.norm(g - go[j]) / np.linalg.norm(go[j])
@@ -467,7 +467,7 @@ def test_flow_path_errors():
     for bad in (-0.1, 1.5):
         with pytest.raises(lib.ChiliError):
             fl.set_mixture(bad)
-    for bad in (1, 9):
+    for bad in (1, 65):
         with pytest.raises(lib.ChiliError):
             fl.set_moment(bad)
     # the context still works after the errors
