@@ -439,9 +439,14 @@ def main():
         "config": {"workload": args.workload, "description": W["desc"], "channels": gpu.nchannels, "dims": gpu.ndims,
                    "bins": gpu.nbins, "points_per_step": n, "selected_leg": selected, "kernel_variant": variant,
                    "specialize_compile_s": compile_s},
-        "points_per_s": round(pts_s, 1), "eps_unw": eps, "eps_unw_q999": eps999, "w_max": wmax, "w_mean": wmean,
-        "attempts_per_step": atts / args.steps,
+        # the selected leg's rate and efficiency (both legs in full below)
+        **(dict(points_per_s=flow["points_per_s"], eps_unw=flow["eps_unw"], eps_unw_q999=flow["eps_unw_q999"],
+                w_max=flow["w_max"], w_mean=flow["w_mean"], attempts_per_step=n / flow["acceptance"])
+           if selected == "flow" else
+           dict(points_per_s=round(pts_s, 1), eps_unw=eps, eps_unw_q999=eps999, w_max=wmax, w_mean=wmean,
+                attempts_per_step=atts / args.steps)),
         "adaptation": ad, "roofline": roof, "cpu_baseline": cpu, "plain": plain, "flow": flow,
+        # the CPU baseline runs the plain algorithm: its points/s against the plain leg's
         "gpu_vs_cpu": round(pts_s / cpu["points_per_s"], 1) if cpu else None,
     }
     print(json.dumps(out))
