@@ -176,6 +176,8 @@ def main():
                     help="defensive mixture weights of the uniform density to run (comma list; the best is reported)")
     ap.add_argument("--flow-train-mix", type=float, default=0.0,
                     help="mixture weight of the proposal the flows are trained (and validated) for")
+    ap.add_argument("--flow-rounds", type=int, default=1,
+                    help="training rounds; rounds after the first train on records drawn by the best flows so far")
     ap.add_argument("--flow-moment", type=int, default=16,
                     help="training loss: this moment of the weights under the proposal (2: variance; higher moments "
                          "control the largest weights, which set the unweighting efficiency)")
@@ -316,13 +318,9 @@ def main():
         r = torch.empty(C * cap * d, dtype=torch.float64, device="cuda:0")
         beta = torch.empty(C * cap, dtype=torch.float64, device="cuda:0")
         w2 = torch.empty(cap, dtype=torch.float64, device="cuda:0")
-        t0 = time.perf_counter()
-        _, nk, _ = gpu.flow_data(None, abi.philox_uniforms(seed, stream=500), cap, r.data_ptr(), beta.data_ptr(), w2.data_ptr(), cap)
         fl = lib.FlowGPU(d, W["flow_layers"], C, seed=seed)
         fl.set_mixture(args.flow_train_mix)
         fl.set_moment(args.flow_moment)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
         losses = []
         done = 0
         # model selection: after every --flow-val-every epochs the flows'
@@ -331,25 +329,40 @@ def main():
         nv = min(n, 1 << 22)
         best = None
         val_hist = []
-        while done < args.flow_epochs:
-            e = min(args.flow_val_every, args.flow_epochs - done)
-            losses += list(fl.train(nk, r.data_ptr(), beta.data_ptr(), w2.data_ptr(), args.flow_batch, e, args.flow_lr,
-                                    seed=seed + done))
-            done += e
-            gpu.flow_iterate(fl, abi.philox_uniforms(seed, stream=700 + done), nv, vals_ptr=ctypes.c_void_p(vals.data_ptr()),
-                             vals_cap=nv)
-            v = vals[:nv]
-            eps_v = float(v.mean() / v.max())
-            val_hist.append(dict(epoch=done, loss=round(losses[-1], 3), eps_unw=round(eps_v, 5),
-                                 w_rel_std=round(float(v.std() / v.mean()), 4)))
-            if best is None or eps_v > best[0]:
-                best = (eps_v, done, [fl.params(j) for j in range(C)])
-            log(f"  flow training: epoch {done}/{args.flow_epochs}, loss {losses[-1]:.5e}, validation eps {eps_v:.4f} "
-                f"({time.perf_counter() - t1:.1f} s)")
+        t_data = 0.0
+        t1 = time.perf_counter()
+        total = args.flow_epochs * args.flow_rounds
+        for rnd in range(args.flow_rounds):
+            # training records: drawn by plain VEGAS first, then (later
+            # rounds) by the best flows so far
+            t0 = time.perf_counter()
+            if rnd > 0:
+                for j in range(C):
+                    fl.set_params(j, best[2][j])
+            _, nk, _ = gpu.flow_data(fl if rnd > 0 else None, abi.philox_uniforms(seed, stream=500 + rnd), cap, r.data_ptr(),
+                                     beta.data_ptr(), w2.data_ptr(), cap)
+            torch.cuda.synchronize()
+            t_data += time.perf_counter() - t0
+            end = done + args.flow_epochs
+            while done < end:
+                e = min(args.flow_val_every, end - done)
+                losses += list(fl.train(nk, r.data_ptr(), beta.data_ptr(), w2.data_ptr(), args.flow_batch, e, args.flow_lr,
+                                        seed=seed + done))
+                done += e
+                gpu.flow_iterate(fl, abi.philox_uniforms(seed, stream=700 + done), nv, vals_ptr=ctypes.c_void_p(vals.data_ptr()),
+                                 vals_cap=nv)
+                v = vals[:nv]
+                eps_v = float(v.mean() / v.max())
+                val_hist.append(dict(epoch=done, loss=round(losses[-1], 3), eps_unw=round(eps_v, 5),
+                                     w_rel_std=round(float(v.std() / v.mean()), 4)))
+                if best is None or eps_v > best[0]:
+                    best = (eps_v, done, [fl.params(j) for j in range(C)])
+                log(f"  flow training: epoch {done}/{total}, loss {losses[-1]:.5e}, validation eps {eps_v:.4f} "
+                    f"({time.perf_counter() - t1:.1f} s)")
         for j in range(C):
             fl.set_params(j, best[2][j])
         log(f"  flow parameters of epoch {best[1]} kept (validation eps {best[0]:.4f})")
-        t_train = time.perf_counter() - t1
+        t_train = time.perf_counter() - t1 - t_data
         del r, beta, w2
 
         def flow_step(uni, vp, cap_):
@@ -387,7 +400,7 @@ def main():
                          loss_moment=args.flow_moment,
                          epochs=args.flow_epochs,
                          batch=args.flow_batch, lr=args.flow_lr, train_s=round(t_train, 2),
-                         data_s=round(t1 - t0, 3), loss_first=losses[0] if losses else None,
+                         data_s=round(t_data, 3), rounds=args.flow_rounds, loss_first=losses[0] if losses else None,
                          loss_last=losses[-1] if losses else None,
                          vs_plain_unweighted=round(flow["unweighted_per_s"] / unw_s, 4),
                          vs_plain_unweighted_q999=round(flow["unweighted_per_s_q999"] / plain["unweighted_per_s_q999"], 4)))
