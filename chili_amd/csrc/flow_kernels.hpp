@@ -228,22 +228,68 @@ __device__ __forceinline__ SplineBin spline_bin(const U& u, float shift, double 
     // exact: v >= X  <=>  vd >= X for every float X, so that the bin of v
     // agrees with the FP64 transform (forward and inverse round-trip)
     const float vd = f32_down(v);
-    float E = 0.f, E0 = 0.f, E1 = 1.f, P0 = 0.f, P1 = 0.f;
+    float E0 = 0.f, E1 = 1.f, P0 = 0.f, P1 = 0.f;
     float ud0 = 0.f, ud1 = 0.f;          // derivative logits at the bin's left / right knot
     int k = 0;
+    // bin k = the last i with i == 0 or vd >= E_i (the knots E_i of the
+    // searched axis do not decrease, so those i form a prefix); the
+    // selections below are selects, not branches
+    if constexpr (K == 16) {
+        // two levels: the group of four bins from E_4, E_8, E_12, then the
+        // bin within it from the group's own knots
+        float Ek[K + 1];
+        Ek[0] = 0.f;
+        Ek[K] = 1.f;
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-        // searched axis: knot i is E, knot i + 1 is En
-        const float En = (i == K - 1) ? 1.f : fmaf(as, by_y ? SH[i + 1] : SW[i + 1], knot_offset(i + 1));
-        const bool in = (i == 0) || (vd >= E);
-        k = in ? i : k;                  // selects, not branches
-        E0 = in ? E : E0;
-        E1 = in ? En : E1;
-        P0 = in ? (by_y ? SW[i] : SH[i]) : P0;
-        P1 = in ? (by_y ? SW[i + 1] : SH[i + 1]) : P1;
-        if (i >= 1) ud0 = in ? u[2 * K + i - 1] : ud0;
-        if (i <= K - 2) ud1 = in ? u[2 * K + i] : ud1;
-        E = En;
+        for (int i = 1; i < K; ++i) Ek[i] = fmaf(as, by_y ? SH[i] : SW[i], knot_offset(i));
+        const bool b1 = vd >= Ek[8];
+        const bool b0 = vd >= (b1 ? Ek[12] : Ek[4]);
+        auto sel = [&](float x0, float x1, float x2, float x3) { return b1 ? (b0 ? x3 : x2) : (b0 ? x1 : x0); };
+        // derivative logit slot j, clamped to the last real one (the clamped
+        // values only ever serve bins whose outer derivative is fixed to 1)
+        auto ulog = [&](int j) { return u[j < 3 * K - 2 ? j : 3 * K - 2]; };
+        float Eg[5], Pg[5], Ug[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            Eg[i] = sel(Ek[i], Ek[4 + i], Ek[8 + i], Ek[12 + i]);
+            Pg[i] = by_y ? sel(SW[i], SW[4 + i], SW[8 + i], SW[12 + i]) : sel(SH[i], SH[4 + i], SH[8 + i], SH[12 + i]);
+            Ug[i] = sel(ulog(2 * K + i - 1), ulog(2 * K + 3 + i), ulog(2 * K + 7 + i), ulog(2 * K + 11 + i));
+        }
+        int kl = 0;
+        E0 = Eg[0];
+        E1 = Eg[1];
+        P0 = Pg[0];
+        P1 = Pg[1];
+        ud0 = Ug[0];
+        ud1 = Ug[1];
+#pragma unroll
+        for (int i = 1; i < 4; ++i) {
+            const bool in = vd >= Eg[i];
+            kl = in ? i : kl;
+            E0 = in ? Eg[i] : E0;
+            E1 = in ? Eg[i + 1] : E1;
+            P0 = in ? Pg[i] : P0;
+            P1 = in ? Pg[i + 1] : P1;
+            ud0 = in ? Ug[i] : ud0;
+            ud1 = in ? Ug[i + 1] : ud1;
+        }
+        k = 4 * (2 * (int)b1 + (int)b0) + kl;
+    } else {
+        float E = 0.f;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            // searched axis: knot i is E, knot i + 1 is En
+            const float En = (i == K - 1) ? 1.f : fmaf(as, by_y ? SH[i + 1] : SW[i + 1], knot_offset(i + 1));
+            const bool in = (i == 0) || (vd >= E);
+            k = in ? i : k;
+            E0 = in ? E : E0;
+            E1 = in ? En : E1;
+            P0 = in ? (by_y ? SW[i] : SH[i]) : P0;
+            P1 = in ? (by_y ? SW[i + 1] : SH[i + 1]) : P1;
+            if (i >= 1) ud0 = in ? u[2 * K + i - 1] : ud0;
+            if (i <= K - 2) ud1 = in ? u[2 * K + i] : ud1;
+            E = En;
+        }
     }
     // the other axis' knots k and k + 1 from their prefix sums
     const float O0 = (k == 0) ? 0.f : fmaf(ao, P0, knot_offset(k));
