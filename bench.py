@@ -431,6 +431,8 @@ def main():
                 frac=round(achieved / FP64_PEAK_TFLOPS, 5), traffic=measured_traffic(args.workload, variant, pts_launch),
                 kernel="k_main (all-channel inverse maps + VEGAS densities + multichannel weight)",
                 compute="FP64 VALU; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s",
+                scope="the multichannel weight kernel of the reference's path (plain leg); the flow leg's BF16 rate is "
+                      "flow.mfma, its coupling kernel is profiled in profiles/k_couple_flowbench_d16_l8.md",
                 flop_eq_per_point=round(f_main, 1), points_per_launch=round(pts_launch, 1),
                 avg_launch_ms=round(avg_ms, 4), launches=tm["launches"],
                 forward_kernel_ms_per_step=round(tm["forward_ms"] / (args.steps + args.warmup), 4))
