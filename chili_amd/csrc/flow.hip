@@ -965,35 +965,4 @@ int chili_flow_train(chili_flow* f, int64_t n, const double* r_dev, const double
     });
 }
 
-// MFMA GEMM probe for tests: C [M x N] fp32 = A [M x K] (bf16 bits, row
-// major) x B^T with B [N x K] (bf16 bits); host buffers.
-int chili_gemm_bf16_test(int M, int N, int K, const uint16_t* A, const uint16_t* B, float* C, int transA, int transB) {
-    return guarded([&] {
-        if (M <= 0 || N <= 0 || K <= 0 || M > 4096 || N > 4096 || K > 1 << 20) throw chili_error(CHILI_ERANGE, "sizes out of range");
-        select_device(0);
-        DevBuf<uint16_t> a, b;
-        DevBuf<float> c;
-        std::vector<uint16_t> av(A, A + (size_t)M * K), bv(B, B + (size_t)N * K);
-        a.upload(av, 0);
-        b.upload(bv, 0);
-        c.resize((size_t)M * N);
-        dim3 g((N + kGT - 1) / kGT, (M + kGT - 1) / kGT, 1);
-        const bf16* ap = (const bf16*)a.ptr;
-        const bf16* bp = (const bf16*)b.ptr;
-        // A stored [M][K] (transA = 0) or [K][M]; B stored [N][K] (transB = 1) or [K][N]
-        const int lda = transA ? M : K, ldb = transB ? K : N;
-        if (!transA && transB)
-            hipLaunchKernelGGL((k_gemm<false, true>), g, dim3(kGThreads), 0, 0, M, N, K, ap, lda, bp, ldb, c.ptr, N, (const float*)nullptr, 0,
-                               (bf16*)nullptr, 0, K, (int64_t)0);
-        else if (transA && !transB)
-            hipLaunchKernelGGL((k_gemm<true, false>), g, dim3(kGThreads), 0, 0, M, N, K, ap, lda, bp, ldb, c.ptr, N, (const float*)nullptr, 0,
-                               (bf16*)nullptr, 0, K, (int64_t)0);
-        else
-            throw chili_error(CHILI_EINVAL, "unsupported operand layout");
-        HIP_CHECK(hipGetLastError());
-        c.download(C, (size_t)M * N, 0);
-        HIP_CHECK(hipDeviceSynchronize());
-    });
-}
-
 }  // extern "C"

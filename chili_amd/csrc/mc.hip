@@ -795,7 +795,9 @@ struct chili_mc {
     double main_ms = 0, fwd_ms = 0;
     int64_t main_launches = 0, main_points = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> fev_live;
+    hipEvent_t producer_ev = nullptr;      // chili_mc_evaluate: the caller's stream -> the context's
     ~chili_mc() {
+        if (producer_ev) hipEventDestroy(producer_ev);
         for (auto& e : ev_live) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto& e : fev_live) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto& e : ev_pool) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
@@ -1973,6 +1975,9 @@ int chili_mc_set_grids(chili_mc* m, const double* grids, int nbins) {
     return guarded([&] {
         if (!m || !grids) throw chili_error(CHILI_EINVAL, "null argument");
         if (nbins < 2 || nbins > 256) throw chili_error(CHILI_ERANGE, "nbins must be in [2, 256]");
+        // validate before any state changes: a rejected grid leaves the
+        // context (bin count, buffers sized by it) as it was
+        check_edges(grids, (size_t)m->C * m->d, nbins);
         select_device(m->device);
         if (nbins != m->B) {
             m->B = nbins;
@@ -1980,7 +1985,6 @@ int chili_mc_set_grids(chili_mc* m, const double* grids, int nbins) {
             m->d_tot.resize(kStatW + m->C + (size_t)m->C * m->d * m->B);
             m->nmax = 0;   // record sizes changed
         }
-        check_edges(grids, (size_t)m->C * m->d, nbins);
         m->grids.assign(grids, grids + (size_t)m->C * m->d * (m->B + 1));
         upload_state(m);
     });
@@ -2064,13 +2068,19 @@ int chili_mc_generate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64
 // VEGAS map of that channel, every channel's inverse map and the
 // multichannel weight as in chili_mc_iterate (ld >= d + 1, randoms in
 // [0, 1)); out = f * W, 0 when the cut fails.  No retries: one value per row.
-int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t n, int64_t ld, double* out_dev) {
+int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t n, int64_t ld, double* out_dev,
+                      void* producer_stream) {
     return guarded([&] {
         if (!m || !rans_dev || !out_dev) throw chili_error(CHILI_EINVAL, "null argument");
         if (n <= 0 || n > (int64_t(1) << 26)) throw chili_error(CHILI_ERANGE, "n must be in [1, 2^26]");
         if (channel < -1 || channel >= m->C) throw chili_error(CHILI_EINVAL, "channel out of range");
         if (ld < m->d + (channel < 0 ? 1 : 0)) throw chili_error(CHILI_EINVAL, "row stride shorter than the randoms needed");
         select_device(m->device);
+        // the randoms (and the output's allocation) are ordered on the
+        // caller's stream: the context's non-blocking stream waits for it
+        if (!m->producer_ev) HIP_CHECK(hipEventCreateWithFlags(&m->producer_ev, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(m->producer_ev, (hipStream_t)producer_stream));
+        HIP_CHECK(hipStreamWaitEvent(m->stream, m->producer_ev, 0));
         if (channel >= 0) {
             MCArgs A = make_args(m, nullptr, 0, 0, 0, 0);
             const unsigned g = (unsigned)((n + kTile - 1) / kTile);

@@ -259,15 +259,17 @@ int chili_vegas_set_grid(chili_vegas* v, const double* edges, int nbins) {
     return guarded([&] {
         if (!v || !edges) throw chili_error(CHILI_EINVAL, "null argument");
         if (nbins < 2 || nbins > 256) throw chili_error(CHILI_ERANGE, "nbins must be in [2, 256]");
+        // validate before any state changes: a rejected grid leaves the
+        // context as it was
+        for (int k = 0; k < v->d; ++k)
+            for (int b = 0; b < nbins; ++b)
+                if (!(edges[(size_t)k * (nbins + 1) + b] <= edges[(size_t)k * (nbins + 1) + b + 1]))
+                    throw chili_error(CHILI_EINVAL, "grid edges must be non-decreasing");
         select_device(v->device);
         v->B = nbins;
         int p = 1;
         while (p < nbins) p <<= 1;
         v->P2 = p;
-        for (int k = 0; k < v->d; ++k)
-            for (int b = 0; b < nbins; ++b)
-                if (!(edges[(size_t)k * (nbins + 1) + b] <= edges[(size_t)k * (nbins + 1) + b + 1]))
-                    throw chili_error(CHILI_EINVAL, "grid edges must be non-decreasing");
         v->edges.assign(edges, edges + (size_t)v->d * (nbins + 1));
         v->cap_tiles = 0;
         vupload(v);

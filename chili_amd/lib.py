@@ -24,7 +24,7 @@ EXPORTS = [
     "chili_mc_flow_iterate", "chili_mc_flow_data", "chili_mc_flow_points",
     "chili_flow_create", "chili_flow_destroy", "chili_flow_info", "chili_flow_mask", "chili_flow_get_params",
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
-    "chili_flow_train", "chili_gemm_bf16_test",
+    "chili_flow_train",
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
     "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_flow_set_moment", "chili_mc_generate", "chili_mc_evaluate", "chili_checkpoint_write", "chili_checkpoint_info", "chili_checkpoint_read", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
@@ -93,7 +93,7 @@ def load():
     L.chili_flow_set_params.argtypes = [P, C.c_int, F]
     L.chili_flow_set_mixture.argtypes = [P, C.c_double]
     L.chili_flow_set_moment.argtypes = [P, C.c_int]
-    L.chili_mc_evaluate.argtypes = [P, C.c_int, VP, I64, I64, VP]
+    L.chili_mc_evaluate.argtypes = [P, C.c_int, VP, I64, I64, VP, VP]
     L.chili_checkpoint_write.argtypes = [P, C.c_int, D, D, C.POINTER(abi.CheckpointState), C.POINTER(abi.Stats), C.c_char_p]
     L.chili_checkpoint_info.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                         C.POINTER(C.c_int64)]
@@ -105,8 +105,6 @@ def load():
     L.chili_flow_sample.argtypes = [P, C.c_int, I64, VP, VP]
     L.chili_flow_loss_grad.argtypes = [P, I64, VP, VP, VP, D]
     L.chili_flow_train.argtypes = [P, I64, VP, VP, VP, I64, C.c_int, C.c_double, U64, D]
-    L.chili_gemm_bf16_test.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), F, C.c_int,
-                                       C.c_int]
     L.chili_mc_specialize.argtypes = [P, C.c_int, D]
     L.chili_channels_kernel_source.argtypes = [P, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)]
     L.chili_channels_compile_check.argtypes = [P, D]
@@ -245,8 +243,11 @@ class MultiChannelGPU:
         if t.shape[1] < self.ndims + (1 if channel < 0 else 0):
             raise ChiliError(f"rans needs {self.ndims + (1 if channel < 0 else 0)} columns")
         out = torch.empty(t.shape[0], dtype=torch.float64, device=t.device)
+        # torch.from_dlpack orders a foreign producer on torch's current
+        # stream; the library waits for that stream before reading
+        stream = torch.cuda.current_stream(t.device).cuda_stream
         check(load().chili_mc_evaluate(self.h, int(channel), C.c_void_p(t.data_ptr()), t.shape[0], t.stride(0),
-                                       C.c_void_p(out.data_ptr())))
+                                       C.c_void_p(out.data_ptr()), C.c_void_p(stream)))
         return out
 
     def generate(self, uni, nevents, wmax, ev_ptr, ew_ptr, cap, a0=0):
@@ -442,17 +443,6 @@ class BatchGPU:
         check(load().chili_batch_tune(self.h, s.ctypes.data_as(C.POINTER(C.c_uint64)), st.ctypes.data_as(C.POINTER(C.c_uint32)),
                                       nc.ctypes.data_as(C.POINTER(C.c_int64)), _d(best)))
         return dict(self.shape(), forward_ms=best[0], weight_ms=best[1])
-
-
-def gemm_bf16_test(A_bits, B_bits, M, N, K, transA=False, transB=True):
-    """MFMA GEMM probe: A [M][K] (or [K][M] if transA) and B [N][K] (or
-    [K][N]) as bf16 bit patterns (uint16); returns C [M][N] fp32."""
-    a = np.ascontiguousarray(A_bits, dtype=np.uint16)
-    b = np.ascontiguousarray(B_bits, dtype=np.uint16)
-    out = np.zeros((M, N), np.float32)
-    check(load().chili_gemm_bf16_test(M, N, K, a.ctypes.data_as(C.POINTER(C.c_uint16)), b.ctypes.data_as(C.POINTER(C.c_uint16)),
-                                      out.ctypes.data_as(C.POINTER(C.c_float)), 1 if transA else 0, 1 if transB else 0))
-    return out
 
 
 class VegasGPU:

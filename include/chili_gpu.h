@@ -200,8 +200,12 @@ int chili_mc_generate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64
  * (ld >= d), out = f * GenerateWeight, 0 when PreProcess fails -- the
  * reference's Evaluate.  channel = -1 (multichannel): column d picks the
  * channel, then VEGAS map, mapper, cut and the multichannel weight as in
- * chili_mc_iterate (ld >= d + 1, randoms in [0, 1)); one value per row. */
-int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t n, int64_t ld, double* out_dev);
+ * chili_mc_iterate (ld >= d + 1, randoms in [0, 1)); one value per row.
+ * producer_stream: the hipStream_t (NULL: the null stream) on which rans and
+ * out were produced / allocated; the library's stream waits for it before
+ * reading, and the call returns when out is written. */
+int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t n, int64_t ld, double* out_dev,
+                      void* producer_stream);
 /* Per-attempt records of attempts [a0, a0+n) (no retry loop): the batch FFI
  * view (include/Interfaces/nanobind.hh:23-50) of MultiChannel::GeneratePoint
  * / GenerateWeight (MultiChannel.hh:141-165).  Host outputs, each optional:
@@ -342,10 +346,6 @@ int chili_mc_flow_data(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, u
 /* per-attempt records; lq [C][n], r [d][n] host outputs */
 int chili_mc_flow_points(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t n, int32_t* status,
                          int32_t* pick, double* w, double* f, double* val, double* lq, double* r);
-/* MFMA GEMM probe (test of the BF16 fragment layout; host buffers):
- * C [M][N] = A B^T (A [M][K], B [N][K]) or A^T B (A [K][M], B [K][N]). */
-int chili_gemm_bf16_test(int M, int N, int K, const uint16_t* A, const uint16_t* B, float* C, int transA, int transB);
-
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,8 @@
 flow's density / sampling / gradients against the CPU restatement
 (oracle/flow_oracle.py), and the multichannel path with flows against the
 plain path and the oracle."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -26,20 +28,38 @@ def bf16_bits(x):
     return t.view(torch.int16).numpy().view(np.uint16), t.to(torch.float64).numpy()
 
 
+def gemm_probe(A_bits, B_bits, M, N, K, transA=False, transB=True):
+    """The product's BF16 MFMA GEMM kernel on host buffers, through the
+    test-only probe library (tests/probes)."""
+    import ctypes as C
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "probes", "_build", "libchili_probe.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (make -C tests/probes)")
+    L = C.CDLL(path)
+    L.chili_probe_gemm_bf16.argtypes = [C.c_int] * 3 + [C.POINTER(C.c_uint16)] * 2 + [C.POINTER(C.c_float), C.c_int, C.c_int]
+    a = np.ascontiguousarray(A_bits, dtype=np.uint16)
+    b = np.ascontiguousarray(B_bits, dtype=np.uint16)
+    c = np.zeros((M, N), dtype=np.float32)
+    rc = L.chili_probe_gemm_bf16(M, N, K, a.ctypes.data_as(C.POINTER(C.c_uint16)), b.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                 c.ctypes.data_as(C.POINTER(C.c_float)), int(transA), int(transB))
+    assert rc == 0, f"probe rc {rc}"
+    return c.astype(np.float64)
+
+
 @pytest.mark.parametrize("shape", [(64, 64, 64), (96, 70, 80), (200, 235, 64), (5, 64, 8)])
 def test_gemm_mfma_layout(shape):
     M, N, K = shape
     rng = np.random.default_rng(1)
     a_bits, a = bf16_bits(rng.normal(size=(M, K)))
     b_bits, b = bf16_bits(rng.normal(size=(N, K)))
-    c = lib.gemm_bf16_test(a_bits, b_bits, M, N, K, transA=False, transB=True)
+    c = gemm_probe(a_bits, b_bits, M, N, K, transA=False, transB=True)
     ref = a @ b.T
     err = np.abs(c - ref) / (np.abs(a) @ np.abs(b).T)
     assert err.max() < 1e-6, f"A B^T: max scaled error {err.max():.2e}"
     # transposed-A / plain-B form (weight gradients): A stored [K][M], B [K][N]
     at_bits, at = bf16_bits(rng.normal(size=(K, M)))
     bn_bits, bn = bf16_bits(rng.normal(size=(K, N)))
-    c2 = lib.gemm_bf16_test(at_bits, bn_bits, M, N, K, transA=True, transB=False)
+    c2 = gemm_probe(at_bits, bn_bits, M, N, K, transA=True, transB=False)
     ref2 = at.T @ bn
     err2 = np.abs(c2 - ref2) / (np.abs(at.T) @ np.abs(bn))
     assert err2.max() < 1e-6, f"A^T B: max scaled error {err2.max():.2e}"

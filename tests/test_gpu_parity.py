@@ -359,18 +359,53 @@ def test_last_bin_reachable():
 
 
 def test_vegas_parity():
+    """Plain VEGAS (Vegas::operator(), Vegas.cc:18-52), three iterations in
+    which each side adapts its grid with its own code (the product's
+    chili_grid_adapt on the GPU's histogram, the oracle's AdaptiveMap::Adapt
+    on its own): statistics, histograms and the adapted grids agree."""
     f = P.integrand_eemumu()
     B = 50
     gpu = lib.VegasGPU(2, B, f)
-    edges = O.uniform_grid(2, B)
+    edges_o = O.uniform_grid(2, B)
     uni = abi.philox_uniforms(SEED + 0)
     n = 20000
     for it in range(3):
-        so, ho, _ = O.vegas_iterate(edges, f, uni, n, a0=it * n)
+        so, ho, _ = O.vegas_iterate(edges_o, f, uni, n, a0=it * n)
         sg, hg = gpu.iterate(uni, n, a0=it * n)
         assert stats_tuple(sg)[0] == n
         assert_rel(stats_tuple(sg)[1:3], stats_tuple(so)[1:3], 1e-12, what="min/max")
         assert_rel(stats_tuple(sg)[3:5], stats_tuple(so)[3:5], 1e-12, what="sums")
         assert_rel(hg, ho, 1e-12, scale=np.abs(ho).max(), what="vegas histogram")
-        edges = O.grid_adapt(edges, ho)
-        gpu.set_grid(edges)
+        edges_o = O.grid_adapt(edges_o, ho)
+        edges_g = lib.grid_adapt(gpu.grid(), hg)
+        assert_rel(edges_g, edges_o, 1e-12, scale=1.0, what=f"adapted grid, iteration {it}")
+        gpu.set_grid(edges_g)
+
+
+def test_rejected_grid_leaves_context_unchanged():
+    """A grid with another bin count and a non-monotone edge is rejected
+    before the context changes: the next iteration still uses the old grid
+    (and bin count) and matches the oracle."""
+    proc, f = CASES["gg_3g"]()
+    ch = lib.Channels(proc)
+    B = 16
+    gpu = lib.MultiChannelGPU(ch, B, f)
+    ora = O.Oracle(proc, B, f)
+    bad = np.tile(np.linspace(0.0, 1.0, 2 * B + 1), (gpu.nchannels, gpu.ndims, 1))
+    bad[0, 0, 5] = 0.9
+    with pytest.raises(lib.ChiliError):
+        gpu.set_grids(bad)
+    assert gpu.nbins == B and gpu.grids().shape[-1] == B + 1
+    uni = abi.philox_uniforms(SEED + 50, stream=1)
+    sg, Tg, hg, nag = gpu.iterate(uni, 3000)
+    rec = gpu.points(uni, 0, nag)
+    so, To, ho, nao = ora.iterate(uni, 3000, mom_in=rec["mom"])
+    assert nag == nao and hg.shape == ho.shape
+    assert_rel(np.array([sg.sum, sg.sum2]), np.array([so.sum, so.sum2]), 1e-12, what="sums")
+    assert_rel(hg, ho, 1e-12, scale=np.abs(ho).max(), what="histograms")
+    vg = lib.VegasGPU(2, 50, P.integrand_eemumu())
+    badv = np.tile(np.linspace(0.0, 1.0, 101), (2, 1))
+    badv[1, 7] = -1.0
+    with pytest.raises(lib.ChiliError):
+        vg.set_grid(badv)
+    assert vg.grid().shape == (2, 51) and vg.nbins == 50

@@ -26,8 +26,11 @@ sign or a transposed index fails:
   JetCluster (kT cut) ...... independent textbook inclusive kT (400 random
                              final states, two radii), closed forms (dR vs R,
                              unwrapped phi)
-  unweighting loop ......... (GPU tests) the device stage against it on the
-                             same streams
+  unweighting loop ......... its definition on the per-attempt records and
+                             the Philox uniforms (keep rule, first n kept,
+                             event weights, StatsData)
+  ConstructChannels order .. std::set / std::map semantics (tests/test_channel_order.py)
+  W conditioning ........... one-ulp momentum changes on the oracle alone
 """
 import math
 import os
@@ -450,6 +453,69 @@ def test_retry_loop_accounting():
         for k in range(o.ndims):
             H[rec["pick"][i], k, rec["bins"][i, k]] += v2[i]
     assert np.allclose(hist, H, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("wmax_q", [0.5, 0.99])
+def test_unweighting_loop_from_records(wmax_q):
+    """The oracle's unweighting loop (MultiChannel::operator() with a
+    PostProcess that keeps a point of value v with probability
+    min(1, |v| / wmax), MultiChannel.hh:196-251) against its definition
+    applied to the per-attempt records: attempt a is kept when it passes the
+    cut and u_{d+1}(a) < |v| / wmax (u from the attempt's Philox stream),
+    every other attempt adds a zero to StatsData, the loop ends at the n-th
+    kept attempt, the events are the kept attempts' momenta in order and
+    their weights sign(v) max(1, |v| / wmax)."""
+    proc, f = P.gluon_process(3, smax=2), P.integrand_propprod()
+    o = O.Oracle(proc, 16, f)
+    seed, stream, a0, n = 91, 4, 5, 300
+    uni = abi.philox_uniforms(seed, stream)
+    probe = o.points(uni, a0, 4000)
+    ok = probe["status"] == 1
+    wmax = float(np.quantile(np.abs(probe["val"][ok]), wmax_q))
+    st, na, ev, ew = o.generate(uni, n, wmax, a0=a0)
+    rec = o.points(uni, a0, na)
+    u = np.array([O.philox_uniform(seed, stream, a0 + i, o.ndims + 1) for i in range(na)])
+    keep = (rec["status"] == 1) & (u < np.abs(rec["val"]) / wmax)
+    assert keep.sum() == n and keep[-1], "the loop ends at the n-th kept attempt"
+    kept = np.nonzero(keep)[0]
+    assert np.array_equal(ev, rec["mom"][kept])
+    v = rec["val"][kept]
+    assert np.array_equal(ew, np.where(np.abs(v) > wmax, v / wmax, np.sign(v)))
+    vals = np.where(keep, rec["val"], 0.0)
+    assert st.n == na and st.n_finite == np.count_nonzero(vals)
+    assert math.isclose(st.sum, vals.sum(), rel_tol=1e-12) and math.isclose(st.sum2, (vals ** 2).sum(), rel_tol=1e-12)
+    assert st.min == vals.min() and st.max == vals.max()
+    if wmax_q < 0.9:
+        assert (np.abs(ew) > 1).any()      # overweight events keep |v| / wmax
+
+
+def test_weight_conditioning_one_ulp():
+    """The multichannel weight of the reference algorithm is ill-conditioned
+    in the momenta: WgtDecays takes sqrt(|m^2|) of the massless children
+    (Channel.cc:167-168), so a last-bit change of the momenta (what two libm
+    implementations of the forward map's pow / sin / cos produce) moves W by
+    far more than 1e-12 at some points.  Shown on the oracle alone: each
+    momentum component of the accepted points moved by at most one ulp gives
+    a W spread of the same order as the one measured between the GPU's and the
+    oracle's independent forward maps (<= 1e-6), while identical momenta give
+    identical weights."""
+    for proc, f in ((P.gluon_process(4, smax=3), P.integrand_propprod()), (P.tt_process(smax=5), P.integrand_resonant())):
+        o = O.Oracle(proc, 64, f)
+        uni = abi.philox_uniforms(0xC0FFEE01, 3)
+        n = 3000
+        rec = o.points(uni, 17, n)
+        same = o.points(uni, 17, n, mom_in=rec["mom"])
+        ok = (rec["status"] == 1) & (rec["w"] != 0)
+        assert np.array_equal(same["w"][ok], rec["w"][ok])
+        rng = np.random.default_rng(5)
+        m = rec["mom"]
+        step = rng.integers(-1, 2, m.shape)
+        mp = np.where(step > 0, np.nextafter(m, np.inf), np.where(step < 0, np.nextafter(m, -np.inf), m))
+        pert = o.points(uni, 17, n, mom_in=mp)
+        both = ok & (pert["status"] == 1)
+        d = np.abs(pert["w"][both] - rec["w"][both]) / np.abs(rec["w"][both])
+        assert d.max() > 1e-9, d.max()          # far above the 1e-12 of identical momenta
+        assert d.max() < 1e-6, d.max()
 
 
 # ---------------------------------------------------------------------------
