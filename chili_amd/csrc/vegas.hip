@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -182,6 +184,23 @@ __global__ __launch_bounds__(256) void k_vreduce(int ntile, int DB, const double
     tot[6 + h] = s;
 }
 
+// Allocation-free iteration (chili_vegas_iterate_static): the running
+// totals and the per-workgroup partials live in this file-scope device array
+// (one copy per GPU, held until the process ends); every call on a GPU shares
+// it, so calls that overlap on one GPU race.  Layout: totals [0, kStaticTot),
+// partials from kStaticTot on, ntile x rec doubles per launch chunk.
+constexpr int64_t kStaticDoubles = int64_t(1) << 19;      // 4 MiB
+constexpr int64_t kStaticTot = 1024;                      // >= 6 + d B
+static __device__ __align__(256) double scratch[kStaticDoubles];
+
+// totals to StatsData's start values (no memset: callers may capture the
+// launch in a graph)
+__global__ void k_vtot_init(int64_t n, double* __restrict__ tot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tot[i] = (i == 1) ? std::numeric_limits<double>::max() : (i == 2) ? std::numeric_limits<double>::min() : 0.0;
+}
+
 }  // namespace
 
 struct chili_vegas {
@@ -203,15 +222,29 @@ void vupload(chili_vegas* v) {
     v->d_edges.upload(v->edges, v->stream);
     v->d_search.upload(s, v->stream);
 }
+// One launch over n points from attempt A.a0: partials per workgroup into
+// part, then added tile by tile, in tile order, to the totals in tot.
 template <int D>
-void vlaunch(chili_vegas* v, const VArgs& A, int64_t n, double* vals, int64_t vals_cap) {
+void vlaunch(chili_vegas* v, const VArgs& A, int64_t n, double* vals, int64_t vals_cap, double* part, double* tot) {
     const int grid = (int)((n + kVTile - 1) / kVTile);
     const int64_t rec = 6 + (int64_t)D * v->B;
     const size_t lds = sizeof(double) * (D * v->B + 32) + sizeof(short) * D * kVTile;
-    hipLaunchKernelGGL(k_vegas<D>, dim3(grid), dim3(kVTile), lds, v->stream, A, n, v->d_part.ptr, rec, vals, vals_cap);
+    hipLaunchKernelGGL(k_vegas<D>, dim3(grid), dim3(kVTile), lds, v->stream, A, n, part, rec, vals, vals_cap);
     const int64_t nout = 1 + (int64_t)D * v->B;
-    hipLaunchKernelGGL(k_vreduce, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, v->stream, grid, D * v->B, v->d_part.ptr, rec,
-                       v->d_tot.ptr);
+    hipLaunchKernelGGL(k_vreduce, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, v->stream, grid, D * v->B, part, rec, tot);
+}
+
+// Device address of this GPU's scratch array, looked up once per device.
+double* static_scratch(int device) {
+    static std::mutex mu;
+    static std::map<int, double*> ptrs;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptrs.find(device);
+    if (it != ptrs.end()) return it->second;
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptrs[device] = (double*)p;
+    return (double*)p;
 }
 }  // namespace
 
@@ -323,13 +356,60 @@ int chili_vegas_iterate(chili_vegas* v, const chili_uniforms* uni, uint64_t a0, 
         tot[1] = std::numeric_limits<double>::max();
         tot[2] = std::numeric_limits<double>::min();
         v->d_tot.upload(tot, v->stream);
-        if (v->d == 2) vlaunch<2>(v, A, ncalls, vals_dev, vals_cap);
+        if (v->d == 2) vlaunch<2>(v, A, ncalls, vals_dev, vals_cap, v->d_part.ptr, v->d_tot.ptr);
         else throw chili_error(CHILI_ERANGE, "unsupported dimension");
         HIP_CHECK(hipGetLastError());
         v->d_tot.download(tot.data(), tot.size(), v->stream);
         HIP_CHECK(hipStreamSynchronize(v->stream));
         stats->n = tot[0]; stats->min = tot[1]; stats->max = tot[2]; stats->sum = tot[3]; stats->sum2 = tot[4]; stats->n_finite = tot[5];
         if (hist) std::copy(tot.begin() + 6, tot.end(), hist);
+    });
+}
+
+// chili_vegas_iterate without any device allocation: device-generated
+// uniforms only; totals and partials in this GPU's static scratch array, the
+// points processed in chunks of at most (kStaticDoubles - kStaticTot) / rec
+// workgroups (the totals are summed in the same tile order as one launch, so
+// the results are chili_vegas_iterate's bit for bit).
+int chili_vegas_iterate_static(chili_vegas* v, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, chili_stats* stats,
+                               double* hist, double* vals_dev, int64_t vals_cap) {
+    return guarded([&] {
+        if (!v || !uni || !stats) throw chili_error(CHILI_EINVAL, "null argument");
+        if (uni->u) throw chili_error(CHILI_EINVAL, "chili_vegas_iterate_static takes device-generated (Philox) uniforms only");
+        if (ncalls <= 0 || ncalls > (int64_t(1) << 31)) throw chili_error(CHILI_ERANGE, "ncalls must be in [1, 2^31]");
+        const int64_t rec = 6 + (int64_t)v->d * v->B;
+        if (rec > kStaticTot) throw chili_error(CHILI_ERANGE, "dims x bins too large for the static scratch (6 + d B <= 1024)");
+        select_device(v->device);
+        double* base = static_scratch(v->device);
+        double* tot = base;
+        double* part = base + kStaticTot;
+        const int64_t chunk = ((kStaticDoubles - kStaticTot) / rec) * kVTile;   // points per launch
+        VArgs A{};
+        A.edges = v->d_edges.ptr;
+        A.search = v->d_search.ptr;
+        A.f = v->d_f.ptr;
+        A.seed = uni->seed;
+        A.stream = uni->stream;
+        A.d = v->d;
+        A.B = v->B;
+        A.P2 = v->P2;
+        hipLaunchKernelGGL(k_vtot_init, dim3(4), dim3(256), 0, v->stream, rec, tot);
+        for (int64_t off = 0; off < ncalls; off += chunk) {
+            const int64_t n = std::min(chunk, ncalls - off);
+            A.a0 = a0 + (uint64_t)off;
+            A.ua0 = A.a0;
+            double* vp = (vals_dev && off < vals_cap) ? vals_dev + off : nullptr;
+            const int64_t vc = vp ? vals_cap - off : 0;
+            if (v->d == 2) vlaunch<2>(v, A, n, vp, vc, part, tot);
+            else throw chili_error(CHILI_ERANGE, "unsupported dimension");
+        }
+        HIP_CHECK(hipGetLastError());
+        double head[6];
+        HIP_CHECK(hipMemcpyAsync(head, tot, sizeof head, hipMemcpyDeviceToHost, v->stream));
+        if (hist) HIP_CHECK(hipMemcpyAsync(hist, tot + 6, sizeof(double) * (size_t)(rec - 6), hipMemcpyDeviceToHost, v->stream));
+        HIP_CHECK(hipStreamSynchronize(v->stream));
+        stats->n = head[0]; stats->min = head[1]; stats->max = head[2]; stats->sum = head[3]; stats->sum2 = head[4];
+        stats->n_finite = head[5];
     });
 }
 

@@ -20,6 +20,7 @@ EXPORTS = [
     "chili_mc_create", "chili_mc_destroy", "chili_mc_info", "chili_mc_set_grids", "chili_mc_get_grids",
     "chili_mc_set_alpha", "chili_mc_get_alpha", "chili_mc_iterate", "chili_mc_points", "chili_mc_timing",
     "chili_vegas_create", "chili_vegas_destroy", "chili_vegas_set_grid", "chili_vegas_get_grid", "chili_vegas_iterate",
+    "chili_vegas_iterate_static",
     "chili_grid_adapt", "chili_grid_split", "chili_alpha_adapt", "chili_max_difference",
     "chili_mc_flow_iterate", "chili_mc_flow_data", "chili_mc_flow_points",
     "chili_flow_create", "chili_flow_destroy", "chili_flow_info", "chili_flow_mask", "chili_flow_get_params",
@@ -75,6 +76,7 @@ def load():
     L.chili_vegas_set_grid.argtypes = [P, D, C.c_int]
     L.chili_vegas_get_grid.argtypes = [P, D, C.POINTER(C.c_int)]
     L.chili_vegas_iterate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), D, C.c_void_p, I64]
+    L.chili_vegas_iterate_static.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), D, C.c_void_p, I64]
     L.chili_grid_adapt.argtypes = [C.c_int, C.c_int, D, C.c_double, D]
     L.chili_grid_split.argtypes = [C.c_int, C.c_int, D, C.c_int, D]
     L.chili_alpha_adapt.argtypes = [C.c_int, D, D, C.c_double, C.c_double]
@@ -477,6 +479,14 @@ class VegasGPU:
         st = abi.Stats()
         hist = np.zeros((self.ndims, self.nbins))
         check(load().chili_vegas_iterate(self.h, C.byref(uni), a0, ncalls, C.byref(st), _d(hist), vals_ptr, vals_cap))
+        return st, hist
+
+    def iterate_static(self, uni, ncalls, a0=0, vals_ptr=None, vals_cap=0):
+        """chili_vegas_iterate_static: the same iteration with no device
+        allocation (static scratch; Philox uniforms only)."""
+        st = abi.Stats()
+        hist = np.zeros((self.ndims, self.nbins))
+        check(load().chili_vegas_iterate_static(self.h, C.byref(uni), a0, ncalls, C.byref(st), _d(hist), vals_ptr, vals_cap))
         return st, hist
 
 

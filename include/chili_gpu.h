@@ -257,6 +257,17 @@ int chili_vegas_get_grid(const chili_vegas* v, double* edges, int* nbins);
  * stats and the f^2 histogram hist [d][nbins] for AdaptiveMap::Adapt. */
 int chili_vegas_iterate(chili_vegas* v, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, chili_stats* stats, double* hist,
                         double* vals_dev, int64_t vals_cap);
+/* chili_vegas_iterate without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory during the call
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results (bit for bit); device-generated
+ * uniforms only (uni->u must be NULL).  Its totals and per-workgroup
+ * partials live in a static 4 MiB device array, one per GPU, held until the
+ * process ends and shared by every call on that GPU: calls that overlap on
+ * one GPU race.  Limit: 6 + ndims x nbins <= 1024 (always true for the
+ * two-dimensional integrand); any ncalls up to 2^31 (processed in chunks). */
+int chili_vegas_iterate_static(chili_vegas* v, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, chili_stats* stats,
+                               double* hist, double* vals_dev, int64_t vals_cap);
 
 /* ---- host-side adaptation (small, serial; the reference's own steps) ---- */
 /* AdaptiveMap::Adapt (src/Integrator/AdaptiveMap.cc:67-125), in place on edges [dims][bins+1]. */
