@@ -92,6 +92,27 @@ class Stats(C.Structure):
         return ((self.sum2 / self.n - m * m) / (self.n - 1)) ** 0.5
 
 
+class MCParams(C.Structure):
+    """chili_mc_params: MultiChannelParams (include/Channel/MultiChannel.hh:29-49)."""
+    _fields_ = [("ncalls", C.c_int64), ("niterations", C.c_int64), ("rtol", C.c_double), ("nrefine", C.c_int64),
+                ("beta", C.c_double), ("min_alpha", C.c_double), ("refine_size", C.c_double), ("max_bins", C.c_int64),
+                ("iteration", C.c_int64), ("should_optimize", C.c_int32), ("pad_", C.c_int32), ("vegas_alpha", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
+class MCSummary(C.Structure):
+    """chili_mc_summary: pooled StatsData, iteration count, min_diff."""
+    _fields_ = [("sum", Stats), ("nresults", C.c_int64), ("min_diff", C.c_double)]
+
+
+class VegasParams(C.Structure):
+    """chili_vegas_params: VegasParams (include/Integrator/Vegas.hh:30-38) + refinement counter."""
+    _fields_ = [("ncalls", C.c_int64), ("nrefine", C.c_int64), ("rtol", C.c_double), ("atol", C.c_double),
+                ("alpha", C.c_double), ("niterations", C.c_int64), ("irefine", C.c_int64)]
+
+
 class CheckpointState(C.Structure):
     """chili_checkpoint_state: MultiChannelParams, min_diff, result count."""
     _fields_ = [("ncalls", C.c_uint64), ("niterations", C.c_uint64), ("rtol", C.c_double), ("nrefine", C.c_uint64),

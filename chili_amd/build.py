@@ -13,7 +13,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchili_gpu.so")
 ARCH = "gfx950"
 
-SOURCES = ["mc.hip", "vegas.hip", "flow.hip", "channels.cpp", "adapt.cpp", "common.cpp", "jit.cpp", "checkpoint.cpp"]
+SOURCES = ["mc.hip", "vegas.hip", "flow.hip", "channels.cpp", "adapt.cpp", "common.cpp", "jit.cpp", "checkpoint.cpp", "optimize.cpp"]
 
 # Device headers compiled again at run time (jit.cpp): embedded as text.
 JIT_HEADERS = [("mc_device.hpp", "mc_device.hpp"), ("mc_kernels.hpp", "mc_kernels.hpp"), ("phys.hpp", "phys.hpp"),

@@ -413,4 +413,71 @@ int chili_vegas_iterate_static(chili_vegas* v, const chili_uniforms* uni, uint64
     });
 }
 
+void chili_vegas_params_default(chili_vegas_params* p) {
+    if (!p) return;
+    *p = chili_vegas_params{};
+    p->ncalls = 10000;          // VegasParams defaults, Vegas.hh:35-36
+    p->nrefine = 5;
+    p->rtol = 1e-4;
+    p->atol = 1e-4;
+    p->alpha = 1.0;
+    p->niterations = 10;
+    p->irefine = 0;
+}
+
+// Vegas::Optimize (src/Integrator/Vegas.cc:54-77) with Vegas::operator()'s
+// Adapt (Vegas.cc:49) and Refine (Vegas.cc:79-82); iteration k draws Philox
+// stream uni->stream + k.
+int chili_vegas_optimize(chili_vegas* v, const chili_uniforms* uni, chili_vegas_params* p, chili_stats* sum, int64_t* nresults,
+                         chili_stats* results, int64_t results_cap, int64_t max_steps) {
+    return guarded([&] {
+        if (!v || !uni || !p || !sum || !nresults) throw chili_error(CHILI_EINVAL, "null argument");
+        if (uni->u) throw chili_error(CHILI_EINVAL, "chili_vegas_optimize draws Philox uniforms (uni->u must be NULL)");
+        if (max_steps < 1) throw chili_error(CHILI_EINVAL, "max_steps must be >= 1 (Vegas::Optimize need not end)");
+        double abs_err = std::numeric_limits<double>::max(), rel_err = std::numeric_limits<double>::max();
+        int64_t steps = 0;
+        std::vector<double> hist, edges;
+        while (((abs_err > p->atol && rel_err > p->rtol) || *nresults < p->niterations) && steps < max_steps) {
+            const int B = v->B, d = v->d;
+            if (p->irefine + 1 == p->nrefine && 2 * B > 256)
+                throw chili_error(CHILI_ERANGE, "refinement would split the grid beyond 256 bins");
+            const int64_t k = *nresults;
+            chili_uniforms u = *uni;
+            u.stream = uni->stream + (uint32_t)k;
+            chili_stats st{};
+            hist.assign((size_t)d * B, 0.0);
+            int rc = chili_vegas_iterate(v, &u, 0, p->ncalls, &st, hist.data(), nullptr, 0);
+            if (rc != CHILI_OK) throw chili_error(rc, chili_last_error());
+            // grid.Adapt(alpha, train_data), unconditional (Vegas.cc:49)
+            edges = v->edges;
+            rc = chili_grid_adapt(d, B, edges.data(), p->alpha, hist.data());
+            if (rc != CHILI_OK) throw chili_error(rc, chili_last_error());
+            rc = chili_vegas_set_grid(v, edges.data(), B);
+            if (rc != CHILI_OK) throw chili_error(rc, chili_last_error());
+            // summary (Vegas.cc:50-51): StatsData::operator+=(StatsData)
+            sum->n += st.n;
+            sum->n_finite += st.n_finite;
+            sum->sum += st.sum;
+            sum->sum2 += st.sum2;
+            sum->min = std::min(sum->min, st.min);
+            sum->max = std::max(sum->max, st.max);
+            if (results && k < results_cap) results[k] = st;
+            *nresults = k + 1;
+            const double mean = sum->sum / sum->n;
+            abs_err = std::sqrt((sum->sum2 / sum->n - mean * mean) / (sum->n - 1));
+            rel_err = abs_err / std::abs(mean);
+            ++steps;
+            if (++p->irefine == p->nrefine) {      // Refine: Split in two, ncalls *= 2
+                std::vector<double> split((size_t)d * (2 * B + 1));
+                rc = chili_grid_split(d, B, v->edges.data(), 2, split.data());
+                if (rc != CHILI_OK) throw chili_error(rc, chili_last_error());
+                rc = chili_vegas_set_grid(v, split.data(), 2 * B);
+                if (rc != CHILI_OK) throw chili_error(rc, chili_last_error());
+                p->ncalls *= 2;
+                p->irefine = 0;
+            }
+        }
+    });
+}
+
 }  // extern "C"

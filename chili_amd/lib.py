@@ -20,7 +20,8 @@ EXPORTS = [
     "chili_mc_create", "chili_mc_destroy", "chili_mc_info", "chili_mc_set_grids", "chili_mc_get_grids",
     "chili_mc_set_alpha", "chili_mc_get_alpha", "chili_mc_iterate", "chili_mc_points", "chili_mc_timing",
     "chili_vegas_create", "chili_vegas_destroy", "chili_vegas_set_grid", "chili_vegas_get_grid", "chili_vegas_iterate",
-    "chili_vegas_iterate_static",
+    "chili_vegas_iterate_static", "chili_mc_optimize", "chili_vegas_optimize", "chili_mc_params_default",
+    "chili_mc_summary_init", "chili_vegas_params_default",
     "chili_grid_adapt", "chili_grid_split", "chili_alpha_adapt", "chili_max_difference",
     "chili_mc_flow_iterate", "chili_mc_flow_data", "chili_mc_flow_points",
     "chili_flow_create", "chili_flow_destroy", "chili_flow_info", "chili_flow_mask", "chili_flow_get_params",
@@ -76,6 +77,16 @@ def load():
     L.chili_vegas_set_grid.argtypes = [P, D, C.c_int]
     L.chili_vegas_get_grid.argtypes = [P, D, C.POINTER(C.c_int)]
     L.chili_vegas_iterate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), D, C.c_void_p, I64]
+    L.chili_mc_optimize.argtypes = [P, C.POINTER(abi.Uniforms), C.POINTER(abi.MCParams), C.POINTER(abi.MCSummary), D,
+                                    C.POINTER(abi.Stats), I64, I64]
+    L.chili_vegas_optimize.argtypes = [P, C.POINTER(abi.Uniforms), C.POINTER(abi.VegasParams), C.POINTER(abi.Stats),
+                                       C.POINTER(C.c_int64), C.POINTER(abi.Stats), I64, I64]
+    L.chili_mc_params_default.argtypes = [C.POINTER(abi.MCParams)]
+    L.chili_mc_params_default.restype = None
+    L.chili_mc_summary_init.argtypes = [C.POINTER(abi.MCSummary)]
+    L.chili_mc_summary_init.restype = None
+    L.chili_vegas_params_default.argtypes = [C.POINTER(abi.VegasParams)]
+    L.chili_vegas_params_default.restype = None
     L.chili_vegas_iterate_static.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), D, C.c_void_p, I64]
     L.chili_grid_adapt.argtypes = [C.c_int, C.c_int, D, C.c_double, D]
     L.chili_grid_split.argtypes = [C.c_int, C.c_int, D, C.c_int, D]
@@ -231,6 +242,22 @@ class MultiChannelGPU:
         check(load().chili_mc_iterate(self.h, C.byref(uni), a0, ncalls, 1 if train else 0, C.byref(st), _d(T), _d(hist),
                                       C.byref(na), vals_ptr, vals_cap))
         return st, T, hist, na.value
+
+    def optimize(self, uni, params, summary, best, max_steps=0):
+        """chili_mc_optimize (MultiChannel::Optimize with Train and
+        RefineChannels): params (abi.MCParams), summary (abi.MCSummary) and
+        best ([C] float64, the reference's best_weights) are updated in place;
+        returns the StatsData of this call's iterations."""
+        if best.dtype != np.float64 or not best.flags.c_contiguous or best.shape != (self.nchannels,):
+            raise ChiliError("best must be a contiguous float64 array of nchannels entries")
+        cap = int(params.niterations)
+        res = (abi.Stats * max(cap, 1))()
+        k0 = summary.nresults
+        check(load().chili_mc_optimize(self.h, C.byref(uni), C.byref(params), C.byref(summary), _d(best), res, cap, max_steps))
+        nc, nd, nb, nt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(load().chili_mc_info(self.h, C.byref(nc), C.byref(nd), C.byref(nb), C.byref(nt)))
+        self.nbins = nb.value
+        return [res[k] for k in range(k0, min(summary.nresults, cap))]
 
     def evaluate(self, rans, channel=0):
         """Batch op of the reference's ChiliCPU.evaluate / CallChili on the
@@ -481,6 +508,20 @@ class VegasGPU:
         check(load().chili_vegas_iterate(self.h, C.byref(uni), a0, ncalls, C.byref(st), _d(hist), vals_ptr, vals_cap))
         return st, hist
 
+    def optimize(self, uni, params, total, nresults, max_steps):
+        """chili_vegas_optimize (Vegas::Optimize with Adapt and Refine):
+        params (abi.VegasParams) and total (abi.Stats) updated in place;
+        returns ([StatsData of this call's iterations], new nresults)."""
+        n = C.c_int64(nresults)
+        cap = nresults + max_steps
+        res = (abi.Stats * max(cap, 1))()
+        check(load().chili_vegas_optimize(self.h, C.byref(uni), C.byref(params), C.byref(total), C.byref(n), res, cap, max_steps))
+        nb = C.c_int()
+        e = np.zeros(self.ndims * 257 * 2)
+        check(load().chili_vegas_get_grid(self.h, _d(e), C.byref(nb)))
+        self.nbins = nb.value
+        return [res[k] for k in range(nresults, n.value)], n.value
+
     def iterate_static(self, uni, ncalls, a0=0, vals_ptr=None, vals_cap=0):
         """chili_vegas_iterate_static: the same iteration with no device
         allocation (static scratch; Philox uniforms only)."""
@@ -488,6 +529,30 @@ class VegasGPU:
         hist = np.zeros((self.ndims, self.nbins))
         check(load().chili_vegas_iterate_static(self.h, C.byref(uni), a0, ncalls, C.byref(st), _d(hist), vals_ptr, vals_cap))
         return st, hist
+
+
+def mc_params(**kw):
+    """chili_mc_params with the reference's defaults (MultiChannelParams), overridden by kw."""
+    p = abi.MCParams()
+    load().chili_mc_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def mc_summary():
+    s = abi.MCSummary()
+    load().chili_mc_summary_init(C.byref(s))
+    return s
+
+
+def vegas_params(**kw):
+    """chili_vegas_params with the reference's defaults (VegasParams), overridden by kw."""
+    p = abi.VegasParams()
+    load().chili_vegas_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 
 
 def grid_adapt(edges, data, alpha=1.0):

@@ -24,6 +24,9 @@
  *   chili_vegas_*             <- chili::Vegas::operator()/Optimize  src/Integrator/Vegas.cc:18-69
  *   chili_grid_adapt/_split   <- chili::AdaptiveMap::Adapt/Split   src/Integrator/AdaptiveMap.cc:67-160
  *   chili_alpha_adapt         <- chili::MultiChannel::Adapt         src/Channel/MultiChannel.cc:109-136
+ *   chili_mc_optimize         <- chili::MultiChannel::Optimize + Train + RefineChannels
+ *                                include/Channel/MultiChannel.hh:122-130,182-194,253-268
+ *   chili_vegas_optimize      <- chili::Vegas::Optimize + Refine   src/Integrator/Vegas.cc:54-82
  */
 #ifndef CHILI_GPU_H
 #define CHILI_GPU_H
@@ -268,6 +271,73 @@ int chili_vegas_iterate(chili_vegas* v, const chili_uniforms* uni, uint64_t a0, 
  * two-dimensional integrand); any ncalls up to 2^31 (processed in chunks). */
 int chili_vegas_iterate_static(chili_vegas* v, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, chili_stats* stats,
                                double* hist, double* vals_dev, int64_t vals_cap);
+
+/* ---- optimisation drivers: the reference's Optimize loops over the device
+ *      iterations (host control, device work) ---- */
+/* MultiChannelParams (include/Channel/MultiChannel.hh:29-49) plus the VEGAS
+ * damping of the channels' grids (VegasParams::alpha, Vegas.hh:30-38). */
+typedef struct chili_mc_params {
+    int64_t ncalls;          /* accepted points per iteration (10000) */
+    int64_t niterations;     /* iteration cap of Optimize (10) */
+    double  rtol;            /* stop when the pooled relative error <= rtol (1e-5) */
+    int64_t nrefine;         /* refine when ++iteration == nrefine (1; 0 never) */
+    double  beta;            /* alpha update exponent (0.25) */
+    double  min_alpha;       /* alpha clamp (1e-5) */
+    double  refine_size;     /* ncalls growth per refinement (1.5) */
+    int64_t max_bins;        /* grids below this many bins are split in two (200) */
+    int64_t iteration;       /* MultiChannelParams::iteration, the refinement counter (0) */
+    int32_t should_optimize; /* train alpha and grids after each iteration (1) */
+    int32_t pad_;
+    double  vegas_alpha;     /* AdaptiveMap::Adapt damping of the channels' grids (1.0) */
+} chili_mc_params;
+/* MultiChannel's summary state (MultiChannel.hh:132-138, MultiChannel.cc:69-77,
+ * 138-154): the pooled StatsData of every iteration, their count and the
+ * smallest max|T_i - T_j| seen (the best weights' criterion). */
+typedef struct chili_mc_summary {
+    chili_stats sum;         /* starts as StatsData{}: min = DBL_MAX, max = DBL_MIN */
+    int64_t nresults;
+    double  min_diff;        /* starts at +inf */
+} chili_mc_summary;
+void chili_mc_params_default(chili_mc_params* p);
+void chili_mc_summary_init(chili_mc_summary* s);
+/* MultiChannel::Optimize (MultiChannel.hh:253-268): while the pooled relative
+ * error > rtol and nresults < niterations: one iteration (chili_mc_iterate,
+ * training when should_optimize), then Train -- MultiChannel::Adapt of alpha,
+ * AdaptiveMap::Adapt of every channel grid whose histogram is not all zero
+ * (Integrand::Train, Integrand.hh:97-104), MaxDifference keeping the new
+ * alpha in best_weights when it improves min_diff -- then UpdateResults, and
+ * RefineChannels when ++iteration == nrefine (iteration = 0, ncalls *=
+ * refine_size truncated, grids with fewer than max_bins bins split in two,
+ * MultiChannel.hh:122-130).  Iteration k (= summary->nresults) draws Philox
+ * uniforms from stream uni->stream + k, attempts from 0 (device uniforms
+ * only).  params and summary are updated in place; best_weights [C] (in/out)
+ * holds the reference's best_weights; results [results_cap] (optional)
+ * receives iteration k's StatsData at index k.  max_steps > 0 bounds the
+ * iterations of this call.  A refinement beyond 256 bins is refused
+ * (CHILI_ERANGE) before the iteration that would need it. */
+int chili_mc_optimize(chili_mc* m, const chili_uniforms* uni, chili_mc_params* params, chili_mc_summary* summary,
+                      double* best_weights, chili_stats* results, int64_t results_cap, int64_t max_steps);
+/* VegasParams (include/Integrator/Vegas.hh:30-38) and Vegas::Optimize's
+ * refinement counter. */
+typedef struct chili_vegas_params {
+    int64_t ncalls;          /* points per iteration (10000) */
+    int64_t nrefine;         /* Refine when ++irefine == nrefine (5) */
+    double  rtol, atol;      /* 1e-4, 1e-4 */
+    double  alpha;           /* AdaptiveMap::Adapt damping (1.0) */
+    int64_t niterations;     /* 10 */
+    int64_t irefine;         /* refinement counter (0) */
+} chili_vegas_params;
+void chili_vegas_params_default(chili_vegas_params* p);
+/* Vegas::Optimize (src/Integrator/Vegas.cc:54-77): while (abs err > atol and
+ * rel err > rtol) or nresults < niterations: one Vegas::operator() pass
+ * (chili_vegas_iterate) and AdaptiveMap::Adapt of the grid, the pass's
+ * StatsData pooled into *sum and stored at results[k]; Refine (Split in two,
+ * ncalls *= 2, Vegas.cc:79-82) when ++irefine == nrefine.  Iteration k draws
+ * stream uni->stream + k, attempts from 0.  sum / nresults / params are
+ * updated in place.  max_steps >= 1 bounds this call (the reference loop
+ * need not end). */
+int chili_vegas_optimize(chili_vegas* v, const chili_uniforms* uni, chili_vegas_params* params, chili_stats* sum, int64_t* nresults,
+                         chili_stats* results, int64_t results_cap, int64_t max_steps);
 
 /* ---- host-side adaptation (small, serial; the reference's own steps) ---- */
 /* AdaptiveMap::Adapt (src/Integrator/AdaptiveMap.cc:67-125), in place on edges [dims][bins+1]. */

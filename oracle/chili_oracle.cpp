@@ -1116,6 +1116,7 @@ int oracle_set_grids(void* h, const double* grids, int nbins) {
     }
     return 0;
 }
+int oracle_get_nbins(void* h) { return (int)((OProc*)h)->grids[0].bins; }
 int oracle_get_grids(void* h, double* grids) {
     OProc* o = (OProc*)h;
     size_t off = 0;
@@ -1426,6 +1427,111 @@ int oracle_vegas_iterate(int dims, int bins, const double* edges, const chili_in
     }
     res.out(stats);
     if (hist) std::copy(hh.begin(), hh.end(), hist);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Optimisation loops.  MultiChannel::Optimize (MultiChannel.hh:253-268) and
+// its parts: operator() with training (:196-251), Train (:182-194 ->
+// MultiChannel::Adapt MultiChannel.cc:109-136, Integrand::Train
+// Integrand.hh:97-104, MaxDifference MultiChannel.cc:138-154), UpdateResults
+// (MultiChannel.cc:69-77), RefineChannels (MultiChannel.hh:122-130 ->
+// Vegas::Refine Vegas.cc:79-82 -> AdaptiveMap::Split).  Iteration k draws
+// the uniform stream U->stream + k from attempt 0 (the reference draws one
+// unseeded stream); mom_in replaces the forward map of that iteration.
+// ---------------------------------------------------------------------------
+static Stats stats_in(const chili_stats& c) {
+    Stats s; s.n = c.n; s.mn = c.min; s.mx = c.max; s.sum = c.sum; s.sum2 = c.sum2; s.nf = c.n_finite; return s;
+}
+int oracle_mc_optimize_step(void* h, const chili_uniforms* U, const double* mom_in, int64_t n_in, chili_mc_params* p,
+                            chili_mc_summary* s, double* best, chili_stats* st_out) {
+    OProc* o = (OProc*)h;
+    const size_t C = o->maps.size(), d = o->maps[0].ndims();
+    chili_uniforms u = *U;
+    u.stream = U->stream + (uint32_t)s->nresults;
+    std::vector<double> T(C), hist;
+    const size_t B = o->grids[0].bins;
+    hist.assign(C * d * B, 0.0);
+    chili_stats st{};
+    int64_t na = 0;
+    const bool optimize = p->should_optimize != 0;
+    int rc = oracle_mc_iterate2(h, &u, 0, p->ncalls, optimize ? 1 : 0, mom_in, n_in, &st, T.data(), hist.data(), &na);
+    if (rc != 0) return rc;
+    if (optimize) {                                                                                    // Train
+        oracle_alpha_adapt((int)C, o->alpha.data(), T.data(), p->beta, p->min_alpha);
+        for (size_t c = 0; c < C; ++c) {
+            const double* hc = hist.data() + c * d * o->grids[c].bins;
+            bool zero = true;
+            for (size_t i = 0; i < d * o->grids[c].bins; ++i) zero = zero && hc[i] == 0;
+            if (zero) continue;
+            grid_adapt(o->grids[c], p->vegas_alpha, hc);
+        }
+        double mx = 0;                                                                                 // MaxDifference
+        for (size_t i = 0; i + 1 < C; ++i)
+            for (size_t j = i + 1; j < C; ++j) mx = std::max(mx, std::abs(T[i] - T[j]));
+        if (mx < s->min_diff) { std::copy(o->alpha.begin(), o->alpha.end(), best); s->min_diff = mx; }
+    }
+    Stats sum = stats_in(s->sum);                                                                      // UpdateResults
+    sum.merge(stats_in(st));
+    sum.out(&s->sum);
+    s->nresults += 1;
+    if (st_out) *st_out = st;
+    if (++p->iteration == p->nrefine) {                                                                // RefineChannels
+        p->iteration = 0;
+        p->ncalls = (int64_t)((double)p->ncalls * p->refine_size);
+        for (auto& g : o->grids)
+            if ((int64_t)g.bins < p->max_bins) grid_split(g, 2);
+    }
+    return 0;
+}
+int oracle_mc_optimize(void* h, const chili_uniforms* U, chili_mc_params* p, chili_mc_summary* s, double* best,
+                       chili_stats* results, int64_t cap, int64_t max_steps) {
+    double rel = std::numeric_limits<double>::max();
+    int64_t steps = 0;
+    while (rel > p->rtol && s->nresults < p->niterations && (max_steps <= 0 || steps < max_steps)) {
+        chili_stats st{};
+        const int64_t k = s->nresults;
+        int rc = oracle_mc_optimize_step(h, U, nullptr, 0, p, s, best, &st);
+        if (rc != 0) return rc;
+        if (results && k < cap) results[k] = st;
+        Stats sum = stats_in(s->sum);
+        rel = sum.err() / std::abs(sum.mean());
+        ++steps;
+    }
+    return 0;
+}
+// Vegas::Optimize (Vegas.cc:54-77) with operator()'s Adapt (:49) and Refine
+// (:79-82).  edges holds dims x (bins+1) and has room for edges_cap doubles.
+int oracle_vegas_optimize(int dims, int* bins, double* edges, int64_t edges_cap, const chili_integrand* f, const chili_uniforms* U,
+                          chili_vegas_params* p, chili_stats* sum, int64_t* nres, chili_stats* results, int64_t cap,
+                          int64_t max_steps) {
+    double abs_err = std::numeric_limits<double>::max(), rel_err = std::numeric_limits<double>::max();
+    int64_t steps = 0;
+    while (((abs_err > p->atol && rel_err > p->rtol) || *nres < p->niterations) && steps < max_steps) {
+        Grid g; g.dims = (size_t)dims; g.bins = (size_t)*bins; g.h.assign(edges, edges + dims * (*bins + 1));
+        chili_uniforms u = *U;
+        u.stream = U->stream + (uint32_t)*nres;
+        chili_stats st{};
+        std::vector<double> hist((size_t)dims * *bins);
+        oracle_vegas_iterate(dims, *bins, edges, f, &u, 0, p->ncalls, &st, hist.data(), nullptr);
+        grid_adapt(g, p->alpha, hist.data());
+        Stats acc = stats_in(*sum);
+        acc.merge(stats_in(st));
+        acc.out(sum);
+        if (results && *nres < cap) results[*nres] = st;
+        *nres += 1;
+        abs_err = acc.err();
+        rel_err = abs_err / std::abs(acc.mean());
+        ++steps;
+        if (++p->irefine == p->nrefine) {
+            grid_split(g, 2);
+            p->ncalls *= 2;
+            p->irefine = 0;
+        }
+        if ((int64_t)g.h.size() > edges_cap) return -2;
+        std::copy(g.h.begin(), g.h.end(), edges);
+        *bins = (int)g.bins;
+    }
     return 0;
 }
 

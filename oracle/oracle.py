@@ -40,6 +40,14 @@ def lib(count=False):
         L.oracle_mc_iterate2.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_int, D, I64, C.POINTER(abi.Stats), D, D, C.POINTER(C.c_int64)]
         L.oracle_mc_iterate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_int, C.POINTER(abi.Stats), D, D, C.POINTER(C.c_int64)]
         L.oracle_alpha_adapt.argtypes = [C.c_int, D, D, C.c_double, C.c_double]
+        L.oracle_get_nbins.argtypes = [P]
+        L.oracle_mc_optimize_step.argtypes = [P, C.POINTER(abi.Uniforms), D, I64, C.POINTER(abi.MCParams),
+                                              C.POINTER(abi.MCSummary), D, C.POINTER(abi.Stats)]
+        L.oracle_mc_optimize.argtypes = [P, C.POINTER(abi.Uniforms), C.POINTER(abi.MCParams), C.POINTER(abi.MCSummary), D,
+                                         C.POINTER(abi.Stats), I64, I64]
+        L.oracle_vegas_optimize.argtypes = [C.c_int, C.POINTER(C.c_int), D, I64, C.POINTER(abi.Integrand), C.POINTER(abi.Uniforms),
+                                            C.POINTER(abi.VegasParams), C.POINTER(abi.Stats), C.POINTER(C.c_int64),
+                                            C.POINTER(abi.Stats), I64, I64]
         L.oracle_max_difference.restype = C.c_double
         L.oracle_max_difference.argtypes = [C.c_int, D]
         L.oracle_grid_adapt.argtypes = [C.c_int, C.c_int, D, C.c_double, D]
@@ -188,6 +196,49 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle_mc_iterate rc={rc}")
         return st, T, hist, na.value
+
+    def optimize_step(self, uni, params, summary, best, mom_in=None):
+        """One pass of MultiChannel::Optimize's loop body (operator(), Train,
+        UpdateResults, RefineChannels) on stream uni.stream + nresults;
+        params / summary (abi.MCParams / abi.MCSummary) and best (numpy [C])
+        are updated in place; returns the iteration's StatsData."""
+        st = abi.Stats()
+        mi = None if mom_in is None else np.ascontiguousarray(mom_in, dtype=np.float64)
+        rc = self.L.oracle_mc_optimize_step(self.h, C.byref(uni), _d(mi) if mi is not None else None,
+                                            0 if mi is None else mi.shape[0], C.byref(params), C.byref(summary), _d(best),
+                                            C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"oracle_mc_optimize_step rc={rc}")
+        self.nbins = self.L.oracle_get_nbins(self.h)
+        return st
+
+    def optimize(self, uni, params, summary, best, max_steps=0):
+        """MultiChannel::Optimize; returns the StatsData of this call's iterations."""
+        cap = int(params.niterations)
+        res = (abi.Stats * max(cap, 1))()
+        k0 = summary.nresults
+        rc = self.L.oracle_mc_optimize(self.h, C.byref(uni), C.byref(params), C.byref(summary), _d(best), res, cap, max_steps)
+        if rc != 0:
+            raise RuntimeError(f"oracle_mc_optimize rc={rc}")
+        self.nbins = self.L.oracle_get_nbins(self.h)
+        return [res[k] for k in range(k0, summary.nresults)]
+
+
+def vegas_optimize(edges, integrand, uni, params, total, nres, max_steps):
+    """Vegas::Optimize on a grid [dims][bins+1]; params (abi.VegasParams),
+    total (abi.Stats) updated in place; returns (edges, [StatsData], nres)."""
+    dims, bp1 = edges.shape
+    cap_edges = dims * (bp1 - 1) * 2 ** 12 + dims
+    e = np.zeros(cap_edges)
+    e[:edges.size] = edges.ravel()
+    bins = C.c_int(bp1 - 1)
+    n = C.c_int64(nres)
+    res = (abi.Stats * (nres + max_steps + 1))()
+    rc = lib().oracle_vegas_optimize(dims, C.byref(bins), _d(e), cap_edges, C.byref(integrand), C.byref(uni), C.byref(params),
+                                     C.byref(total), C.byref(n), res, nres + max_steps + 1, max_steps)
+    if rc != 0:
+        raise RuntimeError(f"oracle_vegas_optimize rc={rc}")
+    return e[:dims * (bins.value + 1)].reshape(dims, bins.value + 1).copy(), [res[k] for k in range(nres, n.value)], n.value
 
 
 def alpha_adapt(alpha, train, beta=0.25, min_alpha=1e-5):

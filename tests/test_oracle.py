@@ -26,6 +26,9 @@ sign or a transposed index fails:
   JetCluster (kT cut) ...... independent textbook inclusive kT (400 random
                              final states, two radii), closed forms (dR vs R,
                              unwrapped phi)
+  Optimize loops ........... composition of the pinned parts (iterate, Adapt,
+                             grid Adapt, MaxDifference, merge, Split) and the
+                             stop / refinement rules
   unweighting loop ......... its definition on the per-attempt records and
                              the Philox uniforms (keep rule, first n kept,
                              event weights, StatsData)
@@ -516,6 +519,120 @@ def test_weight_conditioning_one_ulp():
         d = np.abs(pert["w"][both] - rec["w"][both]) / np.abs(rec["w"][both])
         assert d.max() > 1e-9, d.max()          # far above the 1e-12 of identical momenta
         assert d.max() < 1e-6, d.max()
+
+
+def test_optimize_step_is_its_parts():
+    """The oracle's MultiChannel::Optimize loop body (operator() with
+    training, Train, UpdateResults, RefineChannels, MultiChannel.hh:122-130,
+    182-268) equals its separately pinned parts composed by hand: the retry
+    loop's iterate, MultiChannel::Adapt, AdaptiveMap::Adapt of every channel
+    with a non-zero histogram, MaxDifference (best weights = the new alpha
+    when it improves), StatsData merge, and the refinement (ncalls x 1.5
+    truncated, grids below max_bins split in two)."""
+    from chili_amd import lib as L
+    proc, f = P.gluon_process(3, smax=2), P.integrand_propprod()
+    a = O.Oracle(proc, 8, f)
+    b = O.Oracle(proc, 8, f)
+    params = L.mc_params(ncalls=3001, nrefine=2, max_bins=16)
+    summ = L.mc_summary()
+    best = np.full(a.nchannels, -1.0)
+    uni = abi.philox_uniforms(123, 40)
+    pooled = None
+    min_diff, best_ref, ncalls, it, bins = np.inf, best.copy(), 3001, 0, 8
+    for k in range(5):
+        st = a.optimize_step(uni, params, summ, best)
+        # by hand on b
+        alpha, grids = b.alpha(), b.grids()
+        sb, T, hist, _ = b.iterate(abi.philox_uniforms(123, 40 + k), ncalls)
+        assert stats_eq(st, sb)
+        alpha = O.alpha_adapt(alpha, T)
+        for c in range(b.nchannels):
+            if np.any(hist[c] != 0):
+                grids[c] = O.grid_adapt(grids[c], hist[c], 1.0)
+        md = O.max_difference(T)
+        if md < min_diff:
+            min_diff, best_ref = md, alpha.copy()
+        it += 1
+        if it == 2:
+            it, ncalls = 0, int(ncalls * 1.5)
+            if bins < 16:
+                grids = np.stack([O.grid_split(g, 2) for g in grids])
+                bins *= 2
+        b.set_grids(grids)
+        b.set_alpha(alpha)
+        pooled = sb.as_tuple() if pooled is None else merge(pooled, sb.as_tuple())
+        assert np.array_equal(a.alpha(), b.alpha()) and np.array_equal(a.grids(), b.grids())
+        assert np.array_equal(best, best_ref) and summ.min_diff == min_diff
+        assert (params.ncalls, params.iteration, a.nbins, summ.nresults) == (ncalls, it, bins, k + 1)
+        assert summ.sum.as_tuple() == pooled
+    assert bins == 16 and ncalls == int(int(3001 * 1.5) * 1.5)
+
+
+def stats_eq(x, y):
+    return x.as_tuple() == y.as_tuple()
+
+
+def merge(a, b):
+    """StatsData::operator+=(StatsData), Statistics.hh:106-115."""
+    return (a[0] + b[0], min(a[1], b[1]), max(a[2], b[2]), a[3] + b[3], a[4] + b[4], a[5] + b[5])
+
+
+def test_optimize_stop_rules():
+    """The loop ends at niterations, or as soon as the pooled relative error
+    is <= rtol (MultiChannel.hh:256); a should_optimize = false run leaves
+    alpha and grids untouched; nrefine = 0 never refines."""
+    from chili_amd import lib as L
+    proc, f = P.gluon_process(3, smax=2), P.integrand_propprod()
+    o = O.Oracle(proc, 8, f)
+    p = L.mc_params(ncalls=2000, niterations=4, rtol=0.0, nrefine=0)
+    s = L.mc_summary()
+    best = np.zeros(o.nchannels)
+    res = o.optimize(abi.philox_uniforms(5, 0), p, s, best)
+    assert len(res) == 4 and s.nresults == 4 and p.ncalls == 2000 and o.nbins == 8
+    pooled = res[0].as_tuple()
+    for r in res[1:]:
+        pooled = merge(pooled, r.as_tuple())
+    assert s.sum.as_tuple() == pooled
+    rel = abi.Stats(*pooled).error() / abs(abi.Stats(*pooled).mean())
+    # rtol just above the error after two iterations: stops after two
+    o2 = O.Oracle(proc, 8, f)
+    p2 = L.mc_params(ncalls=2000, niterations=10, nrefine=0)
+    s2 = L.mc_summary()
+    two = o2.optimize(abi.philox_uniforms(5, 0), p2, s2, np.zeros(o.nchannels), max_steps=2)
+    r2 = s2.sum.error() / abs(s2.sum.mean())
+    o3 = O.Oracle(proc, 8, f)
+    p3 = L.mc_params(ncalls=2000, niterations=10, nrefine=0, rtol=r2 * (1 + 1e-9))
+    s3 = L.mc_summary()
+    o3.optimize(abi.philox_uniforms(5, 0), p3, s3, np.zeros(o.nchannels))
+    assert s3.nresults == 2 and rel > 0
+    # frozen: no training
+    o4 = O.Oracle(proc, 8, f)
+    g0, a0 = o4.grids(), o4.alpha()
+    p4 = L.mc_params(ncalls=2000, niterations=2, should_optimize=0, nrefine=0)
+    o4.optimize(abi.philox_uniforms(5, 0), p4, L.mc_summary(), np.zeros(o.nchannels))
+    assert np.array_equal(o4.grids(), g0) and np.array_equal(o4.alpha(), a0)
+
+
+def test_vegas_optimize_schedule():
+    """Vegas::Optimize (Vegas.cc:54-82): at least niterations passes, the grid
+    adapted after every pass (AdaptiveMap::Adapt, pinned above), split in two
+    and ncalls doubled every nrefine passes; each pass equals the oracle's
+    single Vegas pass on the adapted grid."""
+    from chili_amd import lib as L
+    f = P.integrand_eemumu()
+    edges = O.uniform_grid(2, 50)
+    p = L.vegas_params(ncalls=5000, nrefine=2, niterations=5, rtol=0.0, atol=0.0)
+    tot = abi.Stats(0, np.finfo(float).max, np.finfo(float).tiny, 0, 0, 0)
+    e, res, n = O.vegas_optimize(edges, f, abi.philox_uniforms(9, 0), p, tot, 0, 5)
+    assert n == 5 and len(res) == 5 and e.shape == (2, 201) and p.ncalls == 20000 and p.irefine == 1
+    g, nc = O.uniform_grid(2, 50), 5000
+    for k in range(5):
+        st, hist, _ = O.vegas_iterate(g, f, abi.philox_uniforms(9, k), nc)
+        assert stats_eq(st, res[k])
+        g = O.grid_adapt(g, hist, 1.0)
+        if k % 2 == 1:
+            g, nc = O.grid_split(g, 2), nc * 2
+    assert np.array_equal(g, e)
 
 
 # ---------------------------------------------------------------------------
