@@ -1,11 +1,11 @@
 #!/usr/bin/env python3
 """Unweighted events/s at 1% relative error on one MI355X.
 
-Procedure (per workload):
-  1. adaptation: MultiChannel::Optimize semantics (include/Channel/
-     MultiChannel.hh:253-268) -- iterate with training, adapt alpha and every
-     channel's VEGAS grid -- for the workload's iteration count, recording
-     when the pooled relative error first reaches 1%;
+Procedure (the 2->6 resonant workload by default):
+  1. adaptation: the product's MultiChannel::Optimize (chili_mc_optimize:
+     iterate with training, alpha and grid adaptation, MaxDifference; the
+     workload's iteration count, fixed bin count), recording when the pooled
+     relative error first reaches 1%;
   2. freeze grids and alpha;
   3. `--warmup` untimed then `--steps` timed production steps, one step = one
      batch of accepted points through the whole hot path (pick, bucket,
@@ -14,16 +14,25 @@ Procedure (per workload):
   4. eps_unw = <w> / w_max over the production sample (also with w_max = the
      99.9% quantile), unweighted events/s = accepted points/s x eps_unw;
   5. flow leg: a coupling-layer spline flow per channel is trained on points
-     and weights the frozen plain proposal generates (weighted-variance loss,
-     Adam), then the same production steps run through the flow-assisted
-     path.  Plain multichannel VEGAS is the fallback: `value` reports the leg
-     with more unweighted events/s, and both legs are reported.
-The roofline entry prices the dominant kernel (k_main, the all-channel
+     and weights the frozen plain proposal generates, the training epoch is
+     chosen on a validation sample and the defensive-mixture weight on a
+     selection sample (their own Philox streams), then the production steps
+     run through the flow-assisted path.  Plain multichannel VEGAS is the
+     fallback: the leg with more unweighted events/s on the selection sample
+     is reported as `value`; both legs' production figures are reported.
+The roofline entry prices the dominant plain kernel (k_main, the all-channel
 inverse/weight kernel) by its algorithmic FP64 flop-equivalents per point
 (N_arith + 20 N_trans, counted by the op-counting CPU oracle) over its
+HIP-event duration; roofline_flow prices the flow leg's coupling-layer
+kernel (k_couple) by its algorithmic BF16 conditioner flops over its
 HIP-event duration.  The CPU baseline times the oracle's scalar restatement
 of the reference loop on the host on a bounded sample with the same frozen
-state.
+state.  Short legs for the other workloads (plain VEGAS e+e- -> mu+mu-, the
+2->4 multichannel run, 32 batched jobs) are reported under `configs`.
+
+    python bench.py [--steps K --warmup W]                 the default line
+    python bench.py --workload gg_4g                       the 2->4 workload
+    python bench.py --kernel couple --dims 16 --layers 8   k_couple alone (profiling)
 """
 import argparse
 import ctypes
@@ -39,7 +48,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "unweighted events/s at 1% rel. err: plain VEGAS vs flow, 1 MI355X"
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
 SEED = 0xC0FFEE00
+# Philox stream ranges (each sample its own): adaptation 0.., kernel-variant
+# timing 900.., production 2000.., unweighting 3000.., flow training records
+# 500.., flow validation 700.., selection samples 4000..
+S_PROD, S_SEL = 2000, 4000
 
 
 def workloads():
@@ -60,56 +74,46 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def adapt(gpu, ora_lib, W, uni_seed):
-    """Optimize: iterate + train (alpha, grids), pooled StatsData."""
+def adapt(gpu, W, seed, iters=None, ncalls=None):
+    """MultiChannel::Optimize through the product (chili_mc_optimize), one
+    iteration per call so the time to 1% is recorded; the workload's bin
+    count is fixed (nrefine = 0) and rtol = 0 runs every iteration."""
     from chili_amd import abi, lib
-    pooled = np.zeros(6)
-    pooled[1], pooled[2] = np.finfo(np.float64).max, np.finfo(np.float64).tiny
+    iters = iters or W["iters"]
+    params = lib.mc_params(ncalls=ncalls or W["batch"], niterations=iters, rtol=0.0, nrefine=0)
+    summ = lib.mc_summary()
+    best = gpu.alpha().copy()
     t0 = time.perf_counter()
-    t_1pct = None
-    hist_rel = []
-    best_alpha, min_diff = gpu.alpha(), np.inf
-    for it in range(W["iters"]):
-        uni = abi.philox_uniforms(uni_seed, stream=it)
-        st, T, hist, na = gpu.iterate(uni, W["batch"], train=True)
-        if np.isnan(T).any():
+    t_1pct, hist_rel = None, []
+    uni = abi.philox_uniforms(seed, stream=0)
+    for it in range(iters):
+        st = gpu.optimize(uni, params, summ, best, max_steps=1)[0]
+        if np.isnan(gpu.alpha()).any():
             raise RuntimeError("alpha training became NaN")
-        # MultiChannel::Train: alpha adapt, per-channel grid adapt, MaxDifference
-        alpha = lib.alpha_adapt(gpu.alpha(), T)
-        grids = gpu.grids()
-        for c in range(gpu.nchannels):
-            if np.any(hist[c] != 0):
-                grids[c] = lib.grid_adapt(grids[c], hist[c], 1.0)
-        gpu.set_grids(grids)
-        gpu.set_alpha(alpha)
-        md = lib.max_difference(T)
-        if md < min_diff:
-            best_alpha, min_diff = alpha.copy(), md
-        pooled[0] += st.n; pooled[3] += st.sum; pooled[4] += st.sum2; pooled[5] += st.n_finite
-        mean = pooled[3] / pooled[0]
-        err = np.sqrt((pooled[4] / pooled[0] - mean * mean) / (pooled[0] - 1))
-        rel = err / abs(mean)
+        rel = summ.sum.error() / abs(summ.sum.mean())
         hist_rel.append(rel)
         el = time.perf_counter() - t0
         if t_1pct is None and rel <= 0.01:
             t_1pct = el
-        log(f"  iter {it}: I={st.sum / st.n:.6e} +- {st.error():.2e}  pooled {mean:.6e} rel {rel * 100:.3f}%  "
-            f"acc {W['batch']}/{na} ({W['batch'] / na * 100:.1f}%)  {el:.2f}s")
-    return dict(pooled_mean=float(mean), pooled_rel_err=float(rel), time_adapt_s=time.perf_counter() - t0,
-                time_to_1pct_s=t_1pct, rel_err_history=[float(x) for x in hist_rel])
+        log(f"  iter {it}: I={st.sum / st.n:.6e} +- {st.error():.2e}  pooled {summ.sum.mean():.6e} rel {rel * 100:.3f}%  "
+            f"acc {params.ncalls}/{int(st.n)} ({params.ncalls / st.n * 100:.1f}%)  {el:.2f}s")
+    return dict(pooled_mean=float(summ.sum.mean()), pooled_rel_err=float(hist_rel[-1]), time_adapt_s=time.perf_counter() - t0,
+                time_to_1pct_s=t_1pct, rel_err_history=[float(x) for x in hist_rel], driver="chili_mc_optimize",
+                params=dict(ncalls=int(params.ncalls), niterations=iters, nrefine=0, beta=params.beta, min_alpha=params.min_alpha))
 
 
 def measured_traffic(workload, variant, pts_launch):
-    """HBM bytes per k_main launch at this launch size, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), scaled by the
-    counters' reading of a copy kernel of known size; None if absent."""
+    """HBM bytes per k_main launch at this launch size, projected from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/),
+    scaled by the counters' reading of a copy kernel of known size; None if
+    absent."""
     kind = "specialized" if variant in ("specialized", "specialized-weight") else "generic"
     path = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}_{kind}.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as fh:
         t = json.load(fh)
-    return round(t["bytes_per_point"] * pts_launch, 1)
+    return round(t["bytes_per_point"] * pts_launch, 1), t["bytes_per_point"]
 
 
 def _cpu_worker(job):
@@ -156,6 +160,174 @@ def cpu_baseline(proc, f, grids, alpha, seed, seconds, eps, workload):
                        f"production sample's eps_unw")
 
 
+def couple_flops_per_row(masks, d, K=16, H=64):
+    """Algorithmic BF16 flops of one row through every coupling layer's
+    conditioner: 2 (nA H + 2 H^2 + H nB (3K - 1)) per layer."""
+    return sum(2 * ((d - len(B)) * H + 2 * H * H + H * len(B) * (3 * K - 1)) for B in masks)
+
+
+# ---------------------------------------------------------------------------
+# legs of the other workloads (short; reported under "configs")
+# ---------------------------------------------------------------------------
+def leg_vegas_eemumu():
+    """Plain VEGAS e+e- -> mu+mu- (2 dims x 50 bins): Vegas::Optimize through
+    the product (chili_vegas_optimize: 10 iterations of 1e4 points, refinement
+    every 5 -> 200 bins, ncalls 4e4), then 1e5 production points, timed."""
+    import torch
+    from chili_amd import abi, lib
+    from chili_amd import processes as P
+    f = P.integrand_eemumu()
+    v = lib.VegasGPU(2, 50, f)
+    p = lib.vegas_params(ncalls=10000, nrefine=5, niterations=10)
+    tot = abi.Stats(0, np.finfo(float).max, np.finfo(float).tiny, 0, 0, 0)
+    t0 = time.perf_counter()
+    res, n = v.optimize(abi.philox_uniforms(SEED, stream=0), p, tot, 0, 10)
+    t_opt = time.perf_counter() - t0
+    v.iterate(abi.philox_uniforms(SEED, stream=50), 100000)         # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 20
+    for k in range(reps):
+        st, _ = v.iterate(abi.philox_uniforms(SEED, stream=51 + k), 100000)
+    dt = (time.perf_counter() - t0) / reps
+    return dict(workload="ee_mumu: e+e- -> mu+mu- plain VEGAS, 2 dims, 50 bins refined to 200, 10 x 1e4 (x2 per refinement) "
+                         "then 1e5 points", optimize_iterations=n, bins_after=v.nbins, ncalls_after=int(p.ncalls),
+                optimize_s=round(t_opt, 4), integral=float(tot.sum / tot.n), rel_err=float(tot.error() / abs(tot.sum / tot.n)),
+                production_points=100000, production_ms=round(dt * 1e3, 4), points_per_s=round(100000 / dt, 1),
+                note="a 1e5-point iteration is launch-latency bound (two kernels)")
+
+
+def leg_multichannel(name, steps=5):
+    """A workload's plain leg, short: adaptation through chili_mc_optimize,
+    then `steps` timed production iterations (values in HBM)."""
+    import torch
+    from chili_amd import abi, lib
+    W = workloads()[name]
+    gpu = lib.MultiChannelGPU(lib.Channels(W["proc"]()), W["bins"], W["f"](), device=0)
+    seed = SEED + W["cfg"]
+    ad = adapt(gpu, W, seed)
+    n = W["batch"]
+    vals = torch.empty(n * steps, dtype=torch.float64, device="cuda:0")
+    gpu.iterate(abi.philox_uniforms(seed, stream=1000), n, train=False)
+    torch.cuda.synchronize()
+    gpu.timing(enable=True, reset=True)
+    t0 = time.perf_counter()
+    atts = 0
+    for s in range(steps):
+        _, _, _, na = gpu.iterate(abi.philox_uniforms(seed, stream=S_PROD + s), n, train=False,
+                                  vals_ptr=ctypes.c_void_p(vals.data_ptr() + 8 * n * s), vals_cap=n)
+        atts += na
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tm = gpu.timing(enable=False)
+    w = vals
+    wmean, wmax = float(w.mean()), float(w.max())
+    return dict(workload=W["desc"], channels=gpu.nchannels, dims=gpu.ndims, adaptation_s=round(ad["time_adapt_s"], 3),
+                time_to_1pct_s=ad["time_to_1pct_s"], points_per_s=round(n * steps / dt, 1), ms_per_step=round(dt / steps * 1e3, 3),
+                eps_unw=wmean / wmax, unweighted_per_s=round(n * steps / dt * wmean / wmax, 1), acceptance=n * steps / atts,
+                k_main_avg_ms=round(tm["main_ms"] / max(tm["launches"], 1), 4),
+                k_main_points_per_launch=round(tm["points"] / max(tm["launches"], 1), 1))
+
+
+def leg_batched(iters=5, adapt_iters=3):
+    """32 batched jobs (2->2..2->5, log-uniform accepted points): adaptation
+    through the batch, then one iteration of all jobs timed three ways (loop
+    of single-job iterations, one eager batched pass, the hipGraph replay
+    after the launch-shape autotuner)."""
+    import torch
+    from chili_amd import abi, lib
+    from chili_amd import processes as P
+    spec = P.batched_jobs()
+    chans = [lib.Channels(p) for p, _, _ in spec]
+    jobs = [lib.MultiChannelGPU(ch, 64, f) for ch, (_, f, _) in zip(chans, spec)]
+    ncalls = [n for _, _, n in spec]
+    J = len(jobs)
+    b = lib.BatchGPU(jobs)
+    seeds = [P.BATCH_SEED + j for j in range(J)]
+    stream = 0
+    for it in range(adapt_iters):
+        _, _, Ts, hs = b.iterate(seeds, [stream] * J, ncalls, train=True, graph=False)
+        stream += 1
+        for gpu, T, h in zip(jobs, Ts, hs):
+            gpu.set_alpha(lib.alpha_adapt(gpu.alpha(), T))
+            g = gpu.grids()
+            for c in range(gpu.nchannels):
+                if np.any(h[c] != 0):
+                    g[c] = lib.grid_adapt(g[c], h[c], 1.0)
+            gpu.set_grids(g)
+    total = int(sum(ncalls))
+
+    def timed(fn):
+        nonlocal stream
+        fn(stream)
+        stream += 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            fn(stream)
+            stream += 1
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / iters
+
+    def single_jobs(s):
+        for gpu, sd, n in zip(jobs, seeds, ncalls):
+            gpu.iterate(abi.philox_uniforms(sd, stream=s), n, train=False)
+
+    t_single = timed(single_jobs)
+    t_eager = timed(lambda s: b.iterate(seeds, [s] * J, ncalls, train=False, graph=False))
+    tune = b.tune(seeds, [stream] * J, ncalls)
+    stream += 1
+    t_graph = timed(lambda s: b.iterate(seeds, [s] * J, ncalls, train=False, graph=True))
+    sh = b.shape()
+    return dict(workload="32 jobs, 2->2..2->5 all-gluon (2/5/15/52 channels, 4/7/10/13 dims, 64 bins), accepted points per "
+                         "job log-uniform [1e4, 1e6] from seed 0xC0FFEE02, propagator products with pT/|y|/dR cuts",
+                jobs=J, accepted_points=total, single_job_loop_ms=round(t_single * 1e3, 3),
+                batched_eager_ms=round(t_eager * 1e3, 3), batched_graph_ms=round(t_graph * 1e3, 3),
+                points_per_s=round(total / t_graph, 1), points_per_s_single=round(total / t_single, 1),
+                speedup_vs_single=round(t_single / t_graph, 3), tuned_shape=tune, topups=sh["topups"],
+                captures=sh["captures"], replays=sh["replays"])
+
+
+def kernel_couple(args):
+    """The coupling-layer kernel alone (the profiles' workload): log q of
+    `rows` random hypercube points (density direction) and as many samples
+    (inverse direction) through one flow of random parameters; prints rows/s
+    and the algorithmic BF16 rate, with the kernel's own HIP-event time."""
+    import torch
+    from chili_amd import lib
+    d, L, n = args.dims, args.layers, args.rows
+    fl = lib.FlowGPU(d, L, 1, seed=5)
+    rng = np.random.default_rng(1)
+    p = fl.params(0)
+    fl.set_params(0, (p + rng.normal(0, 0.02, p.shape)).astype(np.float32))
+    flops = couple_flops_per_row(fl.masks(), d)
+    r = torch.rand(n, d, dtype=torch.float64, device="cuda")
+    lq = torch.empty(n, dtype=torch.float64, device="cuda")
+    z = torch.empty_like(r)
+    out = {}
+    for direction in ("density", "sample"):
+        def run():
+            if direction == "density":
+                fl.log_density(0, n, r.data_ptr(), lq.data_ptr(), z.data_ptr())
+            else:
+                z.copy_(r)
+                fl.sample(0, n, z.data_ptr(), lq.data_ptr())
+        run()
+        torch.cuda.synchronize()
+        fl.timing(enable=True, reset=True)
+        t0 = time.perf_counter()
+        for _ in range(args.reps):
+            run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.reps
+        tm = fl.timing(enable=False, reset=True)
+        avg_us = tm["ms"] / max(tm["launches"], 1) * 1e3
+        out[direction] = dict(rows_per_s=n / dt, ms_per_pass=dt * 1e3, k_couple_avg_us=round(avg_us, 2),
+                              k_couple_tflops=round(tm["flops"] / (tm["ms"] * 1e-3) / 1e12, 2),
+                              k_couple_frac_of_bf16_peak=round(tm["flops"] / (tm["ms"] * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4))
+    print(json.dumps(dict(kernel="k_couple", dims=d, layers=L, rows=n, flops_per_row=flops, **out)))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,15 +337,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flow", action="store_true", help="skip the flow leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the short legs of the other workloads")
+    ap.add_argument("--sel-steps", type=int, default=3, help="steps of each selection sample (mixture weight, leg)")
     ap.add_argument("--flow-train", type=int, default=1 << 22, help="training points (method-generated)")
-    ap.add_argument("--flow-epochs", type=int, default=30, help="training epochs (the full schedule is 200)")
+    ap.add_argument("--flow-epochs", type=int, default=30, help="training epochs (the config's schedule is 200)")
     ap.add_argument("--flow-batch", type=int, default=1 << 16)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-val-every", type=int, default=3, help="epochs between validation passes (model selection)")
     ap.add_argument("--gen-events", type=int, default=1 << 23,
                     help="unweighted events to generate against the production w_max (0: skip)")
     ap.add_argument("--flow-mix", default="0,0.3,0.5,0.7",
-                    help="defensive mixture weights of the uniform density to run (comma list; the best is reported)")
+                    help="defensive mixture weights to try on the selection sample (comma list)")
     ap.add_argument("--flow-train-mix", type=float, default=0.0,
                     help="mixture weight of the proposal the flows are trained (and validated) for")
     ap.add_argument("--flow-rounds", type=int, default=1,
@@ -187,6 +361,11 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: adapt, run two production steps and a 1 GiB device copy (the "
                          "calibration kernel of known traffic), print nothing else")
+    ap.add_argument("--kernel", choices=["couple"], help="time one kernel alone (for profiles) and exit")
+    ap.add_argument("--dims", type=int, default=16)
+    ap.add_argument("--layers", type=int, default=8)
+    ap.add_argument("--rows", type=int, default=1 << 22)
+    ap.add_argument("--reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (each GPU would run independent "
@@ -197,21 +376,25 @@ def main():
     from chili_amd import abi, lib
     from oracle import oracle as O
 
-    W = workloads()[args.workload]
     if lib.device_count() < 1:
         raise SystemExit("no HIP device")
     torch.cuda.init()
+    if args.kernel == "couple":
+        kernel_couple(args)
+        return
+
+    W = workloads()[args.workload]
     proc, f = W["proc"](), W["f"]()
     ch = lib.Channels(proc)
     gpu = lib.MultiChannelGPU(ch, W["bins"], f, device=0)
     log(f"workload {args.workload}: {gpu.nchannels} channels, {gpu.ndims} dims, {gpu.nbins} bins")
     seed = SEED + W["cfg"]
-    ad = adapt(gpu, None, W, seed)
+    ad = adapt(gpu, W, seed)
     if args.traffic_probe:
         if not args.no_specialize:
             gpu.specialize(True)
         for s in range(2):
-            gpu.iterate(abi.philox_uniforms(seed, stream=2000 + s), W["batch"], train=False)
+            gpu.iterate(abi.philox_uniforms(seed, stream=S_PROD + s), W["batch"], train=False)
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
         b = torch.empty_like(a)
         b.copy_(a)
@@ -221,29 +404,35 @@ def main():
 
     # ---- production with frozen state ----
     n = W["batch"]
-    vals = torch.empty(n * args.steps, dtype=torch.float64, device="cuda:0")
+    vals = torch.empty(n * max(args.steps, args.sel_steps), dtype=torch.float64, device="cuda:0")
 
-    def production(step_fn, tag):
-        for s in range(args.warmup):
+    def production(step_fn, tag, stream0=S_PROD, steps=None, warmup=None):
+        steps = args.steps if steps is None else steps
+        warmup = args.warmup if warmup is None else warmup
+        for s in range(warmup):
             step_fn(abi.philox_uniforms(seed, stream=1000 + s), None, 0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         atts = 0
-        for s in range(args.steps):
-            atts += step_fn(abi.philox_uniforms(seed, stream=2000 + s), ctypes.c_void_p(vals.data_ptr() + 8 * n * s), n)
+        for s in range(steps):
+            atts += step_fn(abi.philox_uniforms(seed, stream=stream0 + s), ctypes.c_void_p(vals.data_ptr() + 8 * n * s), n)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        w = vals[: n * args.steps]
+        w = vals[: n * steps]
         wmax, wmean = float(w.max()), float(w.mean())
         q999 = float(torch.sort(w).values[int(0.999 * (w.numel() - 1))])
-        pts_s = n * args.steps / dt
+        per_step = w.view(steps, n)
+        eps_steps = (per_step.mean(dim=1) / per_step.max(dim=1).values).cpu().numpy()
+        pts_s = n * steps / dt
         res = dict(points_per_s=round(pts_s, 1), eps_unw=wmean / wmax, eps_unw_q999=wmean / q999,
                    unweighted_per_s=round(pts_s * wmean / wmax, 1), unweighted_per_s_q999=round(pts_s * wmean / q999, 1),
-                   ms_per_step=round(dt / args.steps * 1e3, 3), w_max=wmax, w_mean=wmean,
-                   w_rel_std=float(w.std()) / wmean, acceptance=n * args.steps / atts,
-                   integral=wmean * n * args.steps / atts)
-        log(f"  [{tag}] {pts_s:.3e} points/s, eps {res['eps_unw']:.4f} (q999 {res['eps_unw_q999']:.4f}), "
-            f"{res['unweighted_per_s']:.3e} unweighted/s, rel std {res['w_rel_std']:.3f}, {res['ms_per_step']:.2f} ms/step")
+                   ms_per_step=round(dt / steps * 1e3, 3), w_max=wmax, w_mean=wmean,
+                   w_rel_std=float(w.std()) / wmean, acceptance=n * steps / atts,
+                   integral=wmean * n * steps / atts, steps=steps, first_stream=stream0,
+                   eps_unw_per_step=[round(float(e), 5) for e in eps_steps])
+        log(f"  [{tag}] {pts_s:.3e} points/s, eps {res['eps_unw']:.4f} (q999 {res['eps_unw_q999']:.4f}, per step "
+            f"{eps_steps.min():.4f}..{eps_steps.max():.4f}), {res['unweighted_per_s']:.3e} unweighted/s, rel std "
+            f"{res['w_rel_std']:.3f}, {res['ms_per_step']:.2f} ms/step")
         return res
 
     def plain_step(uni, vp, cap):
@@ -279,13 +468,13 @@ def main():
             log(f"  specialisation unavailable: {e}")
         log(f"  kernel variant: {variant} (step s: {t_variant}, compile {compile_s} s)")
 
+    plain_sel = production(plain_step, "plain, selection sample", stream0=S_SEL, steps=args.sel_steps, warmup=0)
     gpu.timing(enable=True, reset=True)
     plain = production(plain_step, "plain")
     tm = gpu.timing(enable=False)
     pts_s, eps, eps999 = plain["points_per_s"], plain["eps_unw"], plain["eps_unw_q999"]
     wmax, wmean = plain["w_max"], plain["w_mean"]
     unw_s = plain["unweighted_per_s"]
-    ms_step = plain["ms_per_step"]
     atts = n * args.steps / plain["acceptance"]
 
     # ---- unweighted events actually generated (device unweighting stage
@@ -311,7 +500,8 @@ def main():
         del ev, ew
 
     # ---- flow leg ----
-    flow = None
+    flow, flow_sel, roof_flow = None, None, None
+
     def flow_leg():
         C, d = gpu.nchannels, gpu.ndims
         cap = args.flow_train
@@ -323,9 +513,9 @@ def main():
         fl.set_moment(args.flow_moment)
         losses = []
         done = 0
-        # model selection: after every --flow-val-every epochs the flows'
-        # unweighting efficiency <w>/w_max is measured on an independent
-        # sample (its own Philox streams); the best parameters are kept
+        # epoch selection: after every --flow-val-every epochs the flows'
+        # unweighting efficiency <w>/w_max is measured on a validation sample
+        # (its own Philox streams); the best parameters are kept
         nv = min(n, 1 << 22)
         best = None
         val_hist = []
@@ -370,48 +560,55 @@ def main():
             return na
 
         # proposal (1 - lam) q + lam: the uniform component bounds every
-        # weight by 1/lam times the plain one; the best lam is reported
+        # weight by 1/lam times the plain one; lam is chosen on the selection
+        # sample, then the production sample is timed with it
         scan = {}
         for lam in [float(v) for v in args.flow_mix.split(",")]:
             fl.set_mixture(lam)
-            scan[lam] = production(flow_step, f"flow mix {lam:g}")
+            scan[lam] = production(flow_step, f"flow mix {lam:g}, selection sample", stream0=S_SEL, steps=args.sel_steps,
+                                   warmup=1)
         lam_best = max(scan, key=lambda k: scan[k]["unweighted_per_s"])
-        flow = scan[lam_best]
-        # algorithmic BF16 MFMA work of the conditioner networks per step:
-        # flow-component attempts through the picked channel's inverse flow,
-        # every kept point through every other channel's flow and, unless it
-        # took the flow component (its log q comes from sampling), its own
-        # (kept points of the flow component estimated as the fraction
-        # 1 - lambda); per row and layer 2 (nA 64 + 2 64^2 + 64 nB (3K - 1)) flops
-        per_row = sum(2 * (len(A) * 64 + 2 * 64 * 64 + 64 * (d - len(A)) * 47) for A in
-                      [[i for i in range(d) if i not in set(Bl)] for Bl in fl.masks()])
-        atts_step = n / flow["acceptance"]
-        rows_step = atts_step * (1.0 - lam_best) + n * (C - (1.0 - lam_best))
-        mlp_tflops = rows_step * per_row / (flow["ms_per_step"] * 1e-3) / 1e12
-        flow["mfma"] = dict(mlp_flops_per_row=per_row, rows_per_step=round(rows_step), achieved_tflops=round(mlp_tflops, 2),
-                            peak_tflops=2500.0, frac=round(mlp_tflops / 2500.0, 5),
+        fl.set_mixture(lam_best)
+        fl.timing(enable=True, reset=True)
+        prod = production(flow_step, f"flow mix {lam_best:g}")
+        tf = fl.timing(enable=False, reset=True)
+        # the step's algorithmic BF16 MFMA work (k_couple's conditioners)
+        # against the dense peak, and the coupling kernel's own rate
+        per_row = couple_flops_per_row(fl.masks(), d)
+        step_tflops = tf["flops"] / (args.steps + args.warmup) / (prod["ms_per_step"] * 1e-3) / 1e12
+        prod["mfma"] = dict(mlp_flops_per_row=per_row, rows_per_step=round(tf["rows"] / (args.steps + args.warmup)),
+                            achieved_tflops=round(step_tflops, 2), peak_tflops=BF16_PEAK_TFLOPS,
+                            frac=round(step_tflops / BF16_PEAK_TFLOPS, 5),
                             note="whole flow step (spline transforms, VEGAS and phase-space maps included) against the "
-                                 "dense BF16 MFMA peak")
-        flow.update(dict(mixture=lam_best, mixture_scan={f"{k:g}": dict(unweighted_per_s=v["unweighted_per_s"], eps_unw=v["eps_unw"],
-                                                                          w_rel_std=round(v["w_rel_std"], 4))
-                                                          for k, v in scan.items()}))
-        flow.update(dict(selected_epoch=best[1], validation=val_hist))
-        flow.update(dict(layers=W["flow_layers"], hidden="3x64", spline_bins=16, train_points=int(nk), train_mix=args.flow_train_mix,
-                         loss_moment=args.flow_moment,
-                         epochs=args.flow_epochs,
-                         batch=args.flow_batch, lr=args.flow_lr, train_s=round(t_train, 2),
-                         data_s=round(t_data, 3), rounds=args.flow_rounds, loss_first=losses[0] if losses else None,
-                         loss_last=losses[-1] if losses else None,
-                         vs_plain_unweighted=round(flow["unweighted_per_s"] / unw_s, 4),
-                         vs_plain_unweighted_q999=round(flow["unweighted_per_s_q999"] / plain["unweighted_per_s_q999"], 4)))
+                                 "dense BF16 MFMA peak; rows counted on the device")
+        kc_ms = tf["ms"] / max(tf["launches"], 1)
+        kc_tf = tf["flops"] / (tf["ms"] * 1e-3) / 1e12
+        roof = dict(bound="mfma", achieved=round(kc_tf, 2), peak=BF16_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(kc_tf / BF16_PEAK_TFLOPS, 5), traffic=None,
+                    kernel="k_couple (fused coupling layer: BF16 MFMA conditioner MLP + FP32/FP64 spline)",
+                    launches=tf["launches"], avg_launch_ms=round(kc_ms, 4),
+                    rows_per_launch=round(tf["rows"] * W["flow_layers"] / max(tf["launches"], 1), 1),
+                    flops_per_row_layer=round(per_row / W["flow_layers"], 1),
+                    note="algorithmic conditioner flops of every timed launch (rows counted on the device) over the "
+                         "launches' HIP-event time, production steps incl. warm-up; traffic not measured (MFMA/VALU bound)")
+        prod.update(dict(mixture=lam_best,
+                         mixture_selection={f"{k:g}": dict(unweighted_per_s=v["unweighted_per_s"], eps_unw=v["eps_unw"],
+                                                           w_rel_std=round(v["w_rel_std"], 4), acceptance=round(v["acceptance"], 4))
+                                            for k, v in scan.items()},
+                         selected_epoch=best[1], validation=val_hist,
+                         train_points=int(nk), train_s=round(t_train, 2), data_s=round(t_data, 3),
+                         loss_first=losses[0] if losses else None, loss_last=losses[-1] if losses else None,
+                         vs_plain_unweighted=round(prod["unweighted_per_s"] / unw_s, 4),
+                         vs_plain_unweighted_q999=round(prod["unweighted_per_s_q999"] / plain["unweighted_per_s_q999"], 4),
+                         vs_plain_acceptance=round(prod["acceptance"] / plain["acceptance"], 4)))
         del fl
-        return flow
+        return prod, scan[lam_best], roof
 
     if not args.no_flow:
         # a failing flow leg (e.g. training that diverged) is reported in
         # the bench line; the plain leg still stands
         try:
-            flow = flow_leg()
+            flow, flow_sel, roof_flow = flow_leg()
         except (lib.ChiliError, RuntimeError, ValueError, ZeroDivisionError) as e:
             log(f"  flow leg failed: {e}")
             flow = dict(error=str(e))
@@ -423,19 +620,35 @@ def main():
     O.lib(True).oracle_count_reset()
     cnt = ora.count_points(abi.philox_uniforms(seed, stream=99), 0, 4000)
     f_main = (cnt["wgt_arith"] + 20 * cnt["wgt_trans"]) / max(cnt["accepted"], 1)
-    f_fwd = (cnt["fwd_arith"] + 20 * cnt["fwd_trans"]) / cnt["attempts"]
     avg_ms = tm["main_ms"] / max(tm["launches"], 1)
     pts_launch = tm["points"] / max(tm["launches"], 1)
     achieved = f_main * pts_launch / (avg_ms * 1e-3) / 1e12
-    roof = dict(bound="mfma", achieved=round(achieved, 4), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / FP64_PEAK_TFLOPS, 5), traffic=measured_traffic(args.workload, variant, pts_launch),
+    traffic, bpp = measured_traffic(args.workload, variant, pts_launch)
+    roof = dict(bound="fp64-valu", achieved=round(achieved, 4), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=round(achieved / FP64_PEAK_TFLOPS, 5), traffic=traffic,
+                traffic_source=("projected: bytes per point from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE run "
+                                f"(profiles/k_main_traffic_{args.workload}_*.md, {bpp} B/point) x this run's points per launch"
+                                if traffic is not None else None),
                 kernel="k_main (all-channel inverse maps + VEGAS densities + multichannel weight)",
-                compute="FP64 VALU; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s",
-                scope="the multichannel weight kernel of the reference's path (plain leg); the flow leg's BF16 rate is "
-                      "flow.mfma, its coupling kernel is profiled in profiles/k_couple_flowbench_d16_l8.md",
-                flop_eq_per_point=round(f_main, 1), points_per_launch=round(pts_launch, 1),
-                avg_launch_ms=round(avg_ms, 4), launches=tm["launches"],
+                compute="FP64 vector ALU (no matrix work); MI355X's FP64 vector peak 78.6 TFLOP/s",
+                scope="the multichannel weight kernel of the plain leg; the flow leg's coupling kernel is roofline_flow",
+                flop_eq_per_point=round(f_main, 1), flop_eq_count="N_arith + 20 N_trans, op-counting oracle, 4000 attempts",
+                points_per_launch=round(pts_launch, 1), avg_launch_ms=round(avg_ms, 4), launches=tm["launches"],
                 forward_kernel_ms_per_step=round(tm["forward_ms"] / (args.steps + args.warmup), 4))
+
+    # ---- short legs of the other workloads ----
+    configs = {}
+    if not args.no_configs:
+        for key, fn in (("ee_mumu", leg_vegas_eemumu), ("gg_4g_plain", lambda: leg_multichannel("gg_4g")),
+                        ("batched_32_jobs", leg_batched)):
+            if key == "gg_4g_plain" and args.workload == "gg_4g":
+                continue
+            try:
+                configs[key] = fn()
+                log(f"  [{key}] {json.dumps(configs[key])[:300]}")
+            except (lib.ChiliError, RuntimeError, ValueError) as e:
+                configs[key] = dict(error=str(e))
+                log(f"  [{key}] failed: {e}")
 
     # ---- CPU baseline: the scalar oracle on the host cores, bounded sample ----
     cpu = None
@@ -443,24 +656,42 @@ def main():
         cpu = cpu_baseline(proc, f, gpu.grids(), gpu.alpha(), seed, args.cpu_seconds, eps, args.workload)
 
     # plain multichannel VEGAS is the fallback: the flow leg is selected only
-    # when it yields more unweighted events/s
-    selected = "flow" if (flow and "unweighted_per_s" in flow and flow["unweighted_per_s"] > unw_s) else "plain"
+    # when it yields more unweighted events/s on the selection sample
+    flow_ok = bool(flow and "unweighted_per_s" in flow)
+    selected = "flow" if (flow_ok and flow_sel["unweighted_per_s"] > plain_sel["unweighted_per_s"]) else "plain"
     value = flow["unweighted_per_s"] if selected == "flow" else unw_s
-    ms_step = flow["ms_per_step"] if selected == "flow" else ms_step
+    ms_step = flow["ms_per_step"] if selected == "flow" else plain["ms_per_step"]
+    recipe = None
+    if flow_ok:
+        recipe = dict(layers=W["flow_layers"], hidden="3x64 ReLU", spline="rational-quadratic, 16 bins",
+                      train_points=flow["train_points"], batch=args.flow_batch, lr=args.flow_lr, epochs=args.flow_epochs,
+                      rounds=args.flow_rounds, loss=f"(1/p) log mean w^p, p = {args.flow_moment}",
+                      train_mixture=args.flow_train_mix,
+                      deviations_from_config=[f"{args.flow_epochs} epochs with epoch selection on a validation sample "
+                                              "(config: 200)",
+                                              f"moment-{args.flow_moment} loss (config: weighted variance, p = 2)"] +
+                                             (["defensive mixture lambda chosen on a selection sample"]
+                                              if len(args.flow_mix.split(",")) > 1 else []))
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "events/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (Philox4x32-10 uniforms)",
         "config": {"workload": args.workload, "description": W["desc"], "channels": gpu.nchannels, "dims": gpu.ndims,
-                   "bins": gpu.nbins, "points_per_step": n, "selected_leg": selected, "kernel_variant": variant,
-                   "specialize_compile_s": compile_s},
+                   "bins": gpu.nbins, "points_per_step": n, "selected_leg": selected,
+                   "selection": dict(rule="leg and mixture weight with more unweighted events/s on a selection sample "
+                                          f"({args.sel_steps} steps, Philox streams {S_SEL}..); value = that leg's production "
+                                          f"sample (streams {S_PROD}..)",
+                                     plain_selection_unweighted_per_s=plain_sel["unweighted_per_s"],
+                                     flow_selection_unweighted_per_s=flow_sel["unweighted_per_s"] if flow_sel else None),
+                   "kernel_variant": variant, "specialize_compile_s": compile_s, "flow_recipe": recipe},
         # the selected leg's rate and efficiency (both legs in full below)
         **(dict(points_per_s=flow["points_per_s"], eps_unw=flow["eps_unw"], eps_unw_q999=flow["eps_unw_q999"],
                 w_max=flow["w_max"], w_mean=flow["w_mean"], attempts_per_step=n / flow["acceptance"])
            if selected == "flow" else
            dict(points_per_s=round(pts_s, 1), eps_unw=eps, eps_unw_q999=eps999, w_max=wmax, w_mean=wmean,
                 attempts_per_step=atts / args.steps)),
-        "adaptation": ad, "roofline": roof, "cpu_baseline": cpu, "plain": plain, "flow": flow,
+        "adaptation": ad, "roofline": roof, "roofline_flow": roof_flow, "cpu_baseline": cpu, "plain": plain, "flow": flow,
+        "configs": configs,
         # the CPU baseline runs the plain algorithm: its points/s against the plain leg's
         "gpu_vs_cpu": round(pts_s / cpu["points_per_s"], 1) if cpu else None,
     }

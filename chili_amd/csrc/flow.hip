@@ -306,9 +306,10 @@ __global__ void k_swizzle(const float* __restrict__ p, int64_t pW1, int64_t pb1,
 // or j * stride) and number count[j] (seg[j+1] - seg[j], or ucount); items[]
 // prefix-sums ceil(count / chunk).
 __global__ void k_items(int nfl, const int* __restrict__ seg, const int* __restrict__ seg_end, int64_t ucount, int64_t stride,
-                        int64_t chunk, int64_t* start, int64_t* count, int* items) {
+                        int64_t chunk, int64_t* start, int64_t* count, int* items, int64_t* rows_out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int acc = 0;
+    int64_t rows = 0;
     for (int j = 0; j < nfl; ++j) {
         const int64_t s0 = seg ? (int64_t)seg[j] : (int64_t)j * stride;
         const int64_t c = seg ? (int64_t)(seg_end ? seg_end[j] : seg[j + 1]) - seg[j] : ucount;
@@ -316,8 +317,10 @@ __global__ void k_items(int nfl, const int* __restrict__ seg, const int* __restr
         count[j] = c;
         items[j] = acc;
         acc += (int)((c + chunk - 1) / chunk);
+        rows += c;
     }
     items[nfl] = acc;
+    if (rows_out) *rows_out = rows;
 }
 
 // out[i] = in[perm[i]] rows of width w (fp64)
@@ -626,8 +629,14 @@ void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, in
     const int64_t rows_exp = (rows_hint > 0) ? std::min(rows_hint, rows_bound) : rows_bound;
     int64_t chunk = ((rows_exp / 1024 + 255) / 256) * 256;
     chunk = std::max<int64_t>(256, std::min<int64_t>(kChunk, chunk));
+    int slot = -1;
+    if (timing) {
+        if (tslots == kTimingSlots) harvest_timing();
+        if (!d_prows.ptr) d_prows.resize(kTimingSlots);
+        slot = tslots++;
+    }
     hipLaunchKernelGGL(k_items, dim3(1), dim3(1), 0, stream, nfl, seg, seg_end, ucount, stride, chunk, d_start.ptr, d_count.ptr,
-                       d_items.ptr);
+                       d_items.ptr, slot >= 0 ? d_prows.ptr + slot : (int64_t*)nullptr);
     const unsigned grid = (unsigned)(rows_bound / chunk + nfl + 1);
     for (int q = 0; q < L; ++q) {
         const int l = inverse ? L - 1 - q : q;
@@ -659,8 +668,59 @@ void FlowEngine::fused_pass(bool inverse, int nfl, int flow0, const int* seg, in
             a.B[i] = i < ly.nB ? ly.B[i] : 0;
         }
         a.shift = shift;
-        launch_couple(inverse, FL.npair, dim3(grid), lds, a);
+        if (slot >= 0) {
+            CoupleEv e{};
+            if (!tpool.empty()) {
+                e.a = tpool.back().first;
+                e.b = tpool.back().second;
+                tpool.pop_back();
+            } else {
+                HIP_CHECK(hipEventCreate(&e.a));
+                HIP_CHECK(hipEventCreate(&e.b));
+            }
+            e.slot = slot;
+            // algorithmic BF16 work of the conditioner per row: nA -> 64 -> 64
+            // -> 64 -> nB (3K - 1) spline parameters, 2 flops per multiply-add
+            e.flops_row = 2.0 * ((double)ly.nA * H + 2.0 * H * H + (double)H * ly.nB * (3 * kSplineBins - 1));
+            HIP_CHECK(hipEventRecord(e.a, stream));
+            launch_couple(inverse, FL.npair, dim3(grid), lds, a);
+            HIP_CHECK(hipEventRecord(e.b, stream));
+            tev.push_back(e);
+        } else {
+            launch_couple(inverse, FL.npair, dim3(grid), lds, a);
+        }
     }
+}
+
+// Sums the timed launches (synchronises the device first: timed passes may
+// have run on several streams).
+void FlowEngine::harvest_timing() {
+    if (tev.empty() && tslots == 0) return;
+    HIP_CHECK(hipDeviceSynchronize());
+    std::vector<int64_t> rows((size_t)tslots);
+    if (tslots) HIP_CHECK(hipMemcpy(rows.data(), d_prows.ptr, sizeof(int64_t) * (size_t)tslots, hipMemcpyDeviceToHost));
+    std::vector<char> counted((size_t)tslots, 0);
+    for (auto& e : tev) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+        couple_ms += ms;
+        couple_launches += 1;
+        couple_flops += e.flops_row * (double)rows[(size_t)e.slot];
+        if (!counted[(size_t)e.slot]) {
+            counted[(size_t)e.slot] = 1;
+            couple_rows += rows[(size_t)e.slot];
+        }
+        tpool.emplace_back(e.a, e.b);
+    }
+    tev.clear();
+    tslots = 0;
+}
+
+void FlowEngine::release_timing() {
+    for (auto& e : tev) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    for (auto& e : tpool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    tev.clear();
+    tpool.clear();
 }
 
 // log q_f(r) of n points (device [n][d]); z (optional) receives the image.
@@ -811,6 +871,7 @@ int chili_flow_destroy(chili_flow* f) {
         hipStream_t s = f->eng.stream;
         if (s) hipStreamSynchronize(s);
         f->eng.release_lanes();
+        f->eng.release_timing();
         delete f;
         if (s) hipStreamDestroy(s);
     });
@@ -839,6 +900,28 @@ int chili_flow_set_moment(chili_flow* f, int p) {
         if (!f) throw chili_error(CHILI_EINVAL, "null flow");
         if (p < 2 || p > 64) throw chili_error(CHILI_ERANGE, "loss moment must lie in [2, 64]");
         f->eng.moment = p;
+    });
+}
+
+// k_couple timing: enable 1 on / 0 off, reset clears the totals; outputs
+// (each may be NULL): summed launch milliseconds, launch count, rows of the
+// timed passes (each row passes every layer) and algorithmic BF16 flops.
+int chili_flow_timing(chili_flow* f, int enable, int reset, double* ms, int64_t* launches, int64_t* rows, double* flops) {
+    return guarded([&] {
+        if (!f) throw chili_error(CHILI_EINVAL, "null flow");
+        select_device(f->eng.device);
+        f->eng.harvest_timing();
+        if (ms) *ms = f->eng.couple_ms;
+        if (launches) *launches = f->eng.couple_launches;
+        if (rows) *rows = f->eng.couple_rows;
+        if (flops) *flops = f->eng.couple_flops;
+        if (reset) {
+            f->eng.couple_ms = 0;
+            f->eng.couple_flops = 0;
+            f->eng.couple_launches = 0;
+            f->eng.couple_rows = 0;
+        }
+        f->eng.timing = enable != 0;
     });
 }
 

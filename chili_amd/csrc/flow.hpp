@@ -75,6 +75,20 @@ struct FlowEngine {
     static constexpr int kTrainLanes = 4;
     std::vector<std::unique_ptr<Lane>> lanes;
     hipEvent_t ev_ready = nullptr;
+    // k_couple timing (chili_flow_timing): HIP events around every launch,
+    // on the stream it runs on; the rows of each timed fused pass are summed
+    // on the device (no host synchronisation) into d_prows[slot]
+    struct CoupleEv { hipEvent_t a, b; int slot; double flops_row; };
+    bool timing = false;
+    std::vector<CoupleEv> tev;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tpool;
+    DevBuf<int64_t> d_prows;
+    int tslots = 0;
+    static constexpr int kTimingSlots = 4096;
+    double couple_ms = 0, couple_flops = 0;
+    int64_t couple_launches = 0, couple_rows = 0;
+    void harvest_timing();
+    void release_timing();
 
     int ws_lda() const { int m = 0; for (auto& l : layers) m = m > l.ldA ? m : l.ldA; return m; }
     int ws_out() const { int m = 0; for (auto& l : layers) m = m > l.out ? m : l.out; return m; }

@@ -26,7 +26,7 @@ EXPORTS = [
     "chili_mc_flow_iterate", "chili_mc_flow_data", "chili_mc_flow_points",
     "chili_flow_create", "chili_flow_destroy", "chili_flow_info", "chili_flow_mask", "chili_flow_get_params",
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
-    "chili_flow_train",
+    "chili_flow_train", "chili_flow_timing",
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
     "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_flow_set_moment", "chili_mc_generate", "chili_mc_evaluate", "chili_checkpoint_write", "chili_checkpoint_info", "chili_checkpoint_read", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
@@ -114,6 +114,7 @@ def load():
     L.chili_mc_generate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.c_double, VP, VP, I64, C.POINTER(abi.Stats),
                                     C.POINTER(C.c_int64)]
     L.chili_flow_get_grads.argtypes = [P, C.c_int, F]
+    L.chili_flow_timing.argtypes = [P, C.c_int, C.c_int, D, C.POINTER(C.c_int64), C.POINTER(C.c_int64), D]
     L.chili_flow_log_density.argtypes = [P, C.c_int, I64, VP, VP, VP]
     L.chili_flow_sample.argtypes = [P, C.c_int, I64, VP, VP]
     L.chili_flow_loss_grad.argtypes = [P, I64, VP, VP, VP, D]
@@ -379,6 +380,14 @@ class FlowGPU:
         p = np.zeros(self.nparams, np.float32)
         check(load().chili_flow_get_params(self.h, flow, p.ctypes.data_as(C.POINTER(C.c_float))))
         return p
+
+    def timing(self, enable=True, reset=False):
+        """k_couple launch timing (chili_flow_timing): totals so far."""
+        ms, fl = C.c_double(), C.c_double()
+        nl, nr = C.c_int64(), C.c_int64()
+        check(load().chili_flow_timing(self.h, 1 if enable else 0, 1 if reset else 0, C.byref(ms), C.byref(nl), C.byref(nr),
+                                       C.byref(fl)))
+        return dict(ms=ms.value, launches=nl.value, rows=nr.value, flops=fl.value)
 
     def set_mixture(self, lam):
         """Proposal (1 - lam) q + lam (uniform on the hypercube) in the

@@ -214,3 +214,22 @@ def eemumu_sigma(s):
     abschi2 = kap * kap * s * s / den
     a0 = 1.0 + 2.0 * ve * ve * rechi + (ae * ae + ve * ve) ** 2 * abschi2
     return np.pi * EE_ALPHA ** 2 / (2.0 * s) * a0 * (8.0 / 3.0) * GEV2PB
+
+
+# ---- batched workload: 32 independent jobs ----------------------------------
+BATCH_SEED = 0xC0FFEE02
+
+
+def batched_jobs(seed=BATCH_SEED, njobs=32):
+    """The batched workload's jobs: job j is 2 -> 2 + j % 4 all-gluon
+    (every s-/t-channel topology, smax = n_out - 1: 2/5/15/52 channels,
+    4/7/10/13 VEGAS dimensions), propagator-product integrand with pT / |y| /
+    dR cuts, accepted points per iteration floor(10^(4 + 2 u)) with u from
+    the seed (log-uniform in [1e4, 1e6]).  Returns [(process, integrand,
+    ncalls)]."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(njobs):
+        n_out = 2 + j % 4
+        jobs.append((gluon_process(n_out, smax=n_out - 1), integrand_propprod(), int(10 ** (4 + 2 * rng.random()))))
+    return jobs

@@ -410,6 +410,14 @@ int chili_flow_set_mixture(chili_flow* f, double lambda);
  * gradients).  2 <= p <= 64.  No counterpart in the reference (the learned
  * proposal is new). */
 int chili_flow_set_moment(chili_flow* f, int p);
+/* Timing of the coupling-layer kernel (no reference counterpart;
+ * measurement only): with enable != 0 every k_couple launch of this flow set
+ * is bracketed by HIP events on its stream and the rows of each pass are
+ * counted on the device.  Outputs (each may be NULL): summed milliseconds,
+ * launches, rows of the timed passes and their algorithmic BF16 flops
+ * (2 (nA H + 2 H^2 + H nB (3K - 1)) per row and layer); reset != 0 clears the
+ * totals after reading them. */
+int chili_flow_timing(chili_flow* f, int enable, int reset, double* ms, int64_t* launches, int64_t* rows, double* flops);
 /* device pointers: r, z [n][ndims], lq [n] = log q(r) */
 int chili_flow_log_density(chili_flow* f, int flow, int64_t n, const double* r_dev, double* lq_dev, double* z_dev);
 int chili_flow_sample(chili_flow* f, int flow, int64_t n, double* zr_dev, double* lq_dev);
