@@ -796,6 +796,10 @@ struct chili_mc {
     int64_t main_launches = 0, main_points = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> fev_live;
     hipEvent_t producer_ev = nullptr;      // chili_mc_evaluate: the caller's stream -> the context's
+    // host integrand / PreProcess callback (the reference's std::function path)
+    chili_host_integrand_fn host_fn = nullptr;
+    void* host_user = nullptr;
+    DevBuf<double> d_fhost;                // callback values by sorted position
     ~chili_mc() {
         if (producer_ev) hipEventDestroy(producer_ev);
         for (auto& e : ev_live) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
@@ -900,7 +904,62 @@ MCArgs make_args(chili_mc* m, const double* u, int64_t ustride, uint64_t seed, u
     A.P2 = m->P2;
     A.sqrts = m->set.model.sqrts;
     A.ptmax = m->set.model.sqrts / 2.0;
+    A.fhost = m->host_fn ? m->d_fhost.ptr : nullptr;
     return A;
+}
+
+// a PreProcess rejection of the host callback, in the per-attempt records
+__global__ void k_host_reject(int64_t n, const uint8_t* __restrict__ acc, int32_t* __restrict__ status) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n && !acc[a] && status[a] == 1) status[a] = 0;
+}
+
+// The host integrand and PreProcess (Integrand.hh:41-47, MultiChannel.hh:
+// 219-229) of the attempts of a sub-batch that passed the device cut: their
+// momenta go to the host in attempt order, the callback returns f and the
+// PreProcess decision, the decisions join the attempt flags and the values
+// wait, by sorted position, for the weight kernel.
+void host_pass(chili_mc* m, int64_t n, const PointRec& R) {
+    const int NT = m->NT;
+    std::vector<uint8_t> acc((size_t)n);
+    std::vector<uint32_t> sorted((size_t)n);
+    std::vector<double> cols((size_t)4 * NT * n);
+    HIP_CHECK(hipMemcpyAsync(acc.data(), m->d_acc.ptr, (size_t)n, hipMemcpyDeviceToHost, m->stream));
+    HIP_CHECK(hipMemcpyAsync(sorted.data(), m->d_sorted.ptr, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, m->stream));
+    for (int r = 0; r < 4 * NT; ++r)
+        HIP_CHECK(hipMemcpyAsync(cols.data() + (size_t)r * n, m->d_mom.ptr + (size_t)r * m->nmax, sizeof(double) * (size_t)n,
+                                 hipMemcpyDeviceToHost, m->stream));
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+    std::vector<int64_t> pos((size_t)n);                 // attempt -> sorted position
+    for (int64_t e = 0; e < n; ++e) pos[sorted[(size_t)e]] = e;
+    std::vector<int64_t> live;
+    for (int64_t a = 0; a < n; ++a)
+        if (acc[(size_t)a]) live.push_back(a);
+    const int64_t nl = (int64_t)live.size();
+    std::vector<double> mom((size_t)nl * NT * 4), fv((size_t)nl, 0.0), fpos((size_t)n, 0.0);
+    std::vector<int32_t> pass((size_t)nl, 1);
+    for (int64_t k = 0; k < nl; ++k) {
+        const int64_t e = pos[(size_t)live[(size_t)k]];
+        for (int r = 0; r < 4 * NT; ++r) mom[(size_t)k * NT * 4 + r] = cols[(size_t)r * n + e];
+    }
+    if (nl > 0 && m->host_fn(m->host_user, nl, NT, mom.data(), fv.data(), pass.data()) != 0)
+        throw chili_error(CHILI_EINVAL, "the host integrand callback reported an error");
+    for (int64_t k = 0; k < nl; ++k) {
+        const int64_t a = live[(size_t)k];
+        if (!pass[(size_t)k]) acc[(size_t)a] = 0;
+        fpos[(size_t)pos[(size_t)a]] = fv[(size_t)k];
+    }
+    if (m->d_fhost.n < (size_t)n) throw chili_error(CHILI_EINVAL, "host value buffer smaller than the sub-batch");
+    HIP_CHECK(hipMemcpyAsync(m->d_acc.ptr, acc.data(), (size_t)n, hipMemcpyHostToDevice, m->stream));
+    HIP_CHECK(hipMemcpyAsync(m->d_fhost.ptr, fpos.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice, m->stream));
+    if (R.status)
+        hipLaunchKernelGGL(k_host_reject, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, n, m->d_acc.ptr, R.status);
+    // the staging vectors are freed on return: finish the copies first
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+}
+
+void no_host_callback(const chili_mc* m, const char* what) {
+    if (m->host_fn) throw chili_error(CHILI_EINVAL, std::string(what) + " evaluates the integrand on the device: clear the host callback first");
 }
 
 template <int NT>
@@ -1151,9 +1210,11 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
     ensure_capacity(m, n);
     const int C = m->C;
     const int nblk = (int)((n + kTile - 1) / kTile);
+    if (m->host_fn) m->d_fhost.resize((size_t)m->nmax);
     MCArgs A = make_args(m, u, ustride, seed, stream, a0);
     A.ua0 = ua0;
     const int tgrid = pick_sort_forward(m, A, n, R);
+    if (m->host_fn) host_pass(m, n, R);
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
     if (kept > 0) {
         const int mgrid = (int)((kept + kMainTile - 1) / kMainTile) + C;
@@ -2022,7 +2083,10 @@ int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0;
-        iterate_loop(m, uni, a0, ncalls, int64_t(1) << 23, tot, att, stats,
+        // the host callback path moves every attempt's momenta to the host:
+        // smaller sub-batches
+        const int64_t cap = m->host_fn ? (int64_t(1) << 16) : (int64_t(1) << 23);
+        iterate_loop(m, uni, a0, ncalls, cap, tot, att, stats,
                      [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
                          int64_t& used) {
                          run_subbatch(m, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, train, vals_dev, got, vals_cap, kept,
@@ -2048,6 +2112,7 @@ int chili_mc_generate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64
         if (!(wmax > 0) || !std::isfinite(wmax)) throw chili_error(CHILI_EINVAL, "wmax must be positive and finite");
         if (cap < nevents) throw chili_error(CHILI_ERANGE, "event buffer too small");
         if (uni->u) throw chili_error(CHILI_EINVAL, "the unweighting stage needs device-generated (Philox) uniforms");
+        no_host_callback(m, "chili_mc_generate");
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0;
@@ -2075,6 +2140,7 @@ int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t 
         if (n <= 0 || n > (int64_t(1) << 26)) throw chili_error(CHILI_ERANGE, "n must be in [1, 2^26]");
         if (channel < -1 || channel >= m->C) throw chili_error(CHILI_EINVAL, "channel out of range");
         if (ld < m->d + (channel < 0 ? 1 : 0)) throw chili_error(CHILI_EINVAL, "row stride shorter than the randoms needed");
+        no_host_callback(m, "chili_mc_evaluate");
         select_device(m->device);
         // the randoms (and the output's allocation) are ordered on the
         // caller's stream: the context's non-blocking stream waits for it
@@ -2120,6 +2186,20 @@ int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t 
     });
 }
 
+// Host integrand / PreProcess (the reference's std::function slow path,
+// include/Channel/Integrand.hh:41-47): chili_mc_iterate and chili_mc_points
+// hand the momenta of the attempts passing the device cut to fn; fn = NULL
+// returns to the device integrand.
+int chili_mc_set_host_integrand(chili_mc* m, chili_host_integrand_fn fn, void* user) {
+    return guarded([&] {
+        if (!m) throw chili_error(CHILI_EINVAL, "null context");
+        select_device(m->device);
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+        m->host_fn = fn;
+        m->host_user = user;
+    });
+}
+
 // Kernel timing on the context's stream (HIP events around every k_main and
 // k_forward launch).  enable: 1 on, 0 off; reset clears the totals.  Outputs
 // (each may be NULL): summed k_main milliseconds, its launch count and the
@@ -2150,6 +2230,7 @@ int chili_mc_points(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t
     return guarded([&] {
         if (!m || !uni) throw chili_error(CHILI_EINVAL, "null argument");
         if (n <= 0 || n > (int64_t(1) << 24)) throw chili_error(CHILI_ERANGE, "n must be in [1, 2^24]");
+        if (m->host_fn && n > (int64_t(1) << 22)) throw chili_error(CHILI_ERANGE, "with a host callback n must be <= 2^22");
         select_device(m->device);
         const double* ud = nullptr;
         int64_t ustride = 0;
@@ -2206,6 +2287,7 @@ int chili_mc_flow_iterate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni
         if (!m || !uni || !stats) throw chili_error(CHILI_EINVAL, "null argument");
         if (ncalls <= 0) throw chili_error(CHILI_EINVAL, "ncalls must be positive");
         check_flow(m, fl);
+        no_host_callback(m, "the flow path");
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0;
@@ -2229,6 +2311,7 @@ int chili_mc_flow_data(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, u
         if (!m || !uni || !stats || !r_dev || !beta_dev || !w2_dev) throw chili_error(CHILI_EINVAL, "null argument");
         if (ncalls <= 0 || cap <= 0) throw chili_error(CHILI_EINVAL, "ncalls and cap must be positive");
         check_flow(m, fl);
+        no_host_callback(m, "the flow path");
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0, total_kept = 0;
@@ -2253,6 +2336,7 @@ int chili_mc_flow_points(chili_mc* m, chili_flow* fl, const chili_uniforms* uni,
         if (!m || !uni) throw chili_error(CHILI_EINVAL, "null argument");
         if (n <= 0 || n > (int64_t(1) << 22)) throw chili_error(CHILI_ERANGE, "n must be in [1, 2^22]");
         check_flow(m, fl);
+        no_host_callback(m, "the flow path");
         select_device(m->device);
         int64_t ustride = 0;
         const double* ud = stage_uniforms(m, uni, ustride);
@@ -2301,6 +2385,7 @@ int chili_batch_create(int njobs, chili_mc* const* jobs, chili_batch** out) {
             if (jobs[j]->B != b->B) throw chili_error(CHILI_EINVAL, "jobs of a batch share one bin count");
             if (jobs[j]->f.cuts == CHILI_CUTS_KT_JETS)
                 throw chili_error(CHILI_EINVAL, "the kT-jet cut runs in the single-context path only");
+            no_host_callback(jobs[j], "the batched path");
             b->jobs.push_back(jobs[j]);
         }
         select_device(b->device);
@@ -2382,6 +2467,7 @@ int chili_batch_iterate(chili_batch* b, const uint64_t* seed, const uint32_t* st
         const int NJ = (int)b->jobs.size();
         for (int j = 0; j < NJ; ++j)
             if (ncalls[j] <= 0) throw chili_error(CHILI_EINVAL, "ncalls must be positive");
+        for (chili_mc* jm : b->jobs) no_host_callback(jm, "the batched path");
         select_device(b->device);
         for (chili_mc* m : b->jobs) HIP_CHECK(hipStreamSynchronize(m->stream));   // state uploads
         train = train ? 1 : 0;

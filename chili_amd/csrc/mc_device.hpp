@@ -172,7 +172,7 @@ __device__ __forceinline__ void main_tile(const MCArgs& A, int ch, int e0, int e
                 P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
                 P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
             }
-            fv = integrand<NT>(A.f, P);
+            fv = A.fhost ? A.fhost[sp] : integrand<NT>(A.f, P);
         } else {
 #pragma unroll
             for (int i = 0; i < NT; ++i) P.set(i, mk(1., 0., 0., 1.));

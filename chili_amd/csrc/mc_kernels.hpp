@@ -63,6 +63,7 @@ struct MCArgs {
     uint32_t stream;
     int C, d, B, P2;
     double sqrts, ptmax;
+    const double* fhost;     // integrand values from the host callback, by sorted position (null: device integrand)
 };
 
 // --------------------------------------------------------------------------

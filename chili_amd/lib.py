@@ -20,7 +20,7 @@ EXPORTS = [
     "chili_mc_create", "chili_mc_destroy", "chili_mc_info", "chili_mc_set_grids", "chili_mc_get_grids",
     "chili_mc_set_alpha", "chili_mc_get_alpha", "chili_mc_iterate", "chili_mc_points", "chili_mc_timing",
     "chili_vegas_create", "chili_vegas_destroy", "chili_vegas_set_grid", "chili_vegas_get_grid", "chili_vegas_iterate",
-    "chili_vegas_iterate_static", "chili_mc_optimize", "chili_vegas_optimize", "chili_mc_params_default",
+    "chili_vegas_iterate_static", "chili_mc_set_host_integrand", "chili_mc_optimize", "chili_vegas_optimize", "chili_mc_params_default",
     "chili_mc_summary_init", "chili_vegas_params_default",
     "chili_grid_adapt", "chili_grid_split", "chili_alpha_adapt", "chili_max_difference",
     "chili_mc_flow_iterate", "chili_mc_flow_data", "chili_mc_flow_points",
@@ -30,6 +30,11 @@ EXPORTS = [
     "chili_batch_create", "chili_batch_destroy", "chili_batch_iterate", "chili_batch_set_shape", "chili_batch_get_shape",
     "chili_batch_tune", "chili_mc_specialize", "chili_flow_set_mixture", "chili_flow_set_moment", "chili_mc_generate", "chili_mc_evaluate", "chili_checkpoint_write", "chili_checkpoint_info", "chili_checkpoint_read", "chili_channels_kernel_source", "chili_channels_compile_check",
 ]
+
+
+# chili_host_integrand_fn: int (void* user, int64_t n, int ntot, const double* mom, double* f, int32_t* pass)
+HOST_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                      C.POINTER(C.c_int32))
 
 
 class ChiliError(RuntimeError):
@@ -77,6 +82,7 @@ def load():
     L.chili_vegas_set_grid.argtypes = [P, D, C.c_int]
     L.chili_vegas_get_grid.argtypes = [P, D, C.POINTER(C.c_int)]
     L.chili_vegas_iterate.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), D, C.c_void_p, I64]
+    L.chili_mc_set_host_integrand.argtypes = [P, HOST_FN, C.c_void_p]
     L.chili_mc_optimize.argtypes = [P, C.POINTER(abi.Uniforms), C.POINTER(abi.MCParams), C.POINTER(abi.MCSummary), D,
                                     C.POINTER(abi.Stats), I64, I64]
     L.chili_vegas_optimize.argtypes = [P, C.POINTER(abi.Uniforms), C.POINTER(abi.VegasParams), C.POINTER(abi.Stats),
@@ -259,6 +265,31 @@ class MultiChannelGPU:
         check(load().chili_mc_info(self.h, C.byref(nc), C.byref(nd), C.byref(nb), C.byref(nt)))
         self.nbins = nb.value
         return [res[k] for k in range(k0, min(summary.nresults, cap))]
+
+    def set_host_integrand(self, fn):
+        """Host integrand and PreProcess (chili_mc_set_host_integrand): fn(mom)
+        takes the momenta [n][ntot][4] of the attempts that passed the device
+        cut and returns (f [n], pass [n] booleans); None restores the device
+        integrand."""
+        if fn is None:
+            check(load().chili_mc_set_host_integrand(self.h, HOST_FN(), None))
+            self._host_cb = None
+            return
+
+        def cb(user, n, ntot, mom, f, ok):
+            try:
+                m = np.ctypeslib.as_array(mom, shape=(n, ntot, 4))
+                fv, pv = fn(m)
+                np.ctypeslib.as_array(f, shape=(n,))[:] = fv
+                np.ctypeslib.as_array(ok, shape=(n,))[:] = np.asarray(pv, dtype=np.int32)
+                return 0
+            except Exception:          # reported as an error code, never across the C boundary
+                import traceback
+                traceback.print_exc()
+                return 1
+
+        self._host_cb = HOST_FN(cb)
+        check(load().chili_mc_set_host_integrand(self.h, self._host_cb, None))
 
     def evaluate(self, rans, channel=0):
         """Batch op of the reference's ChiliCPU.evaluate / CallChili on the

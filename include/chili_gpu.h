@@ -216,6 +216,22 @@ int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t 
  * ps[C][n], x[d][n] (picked channel's inverse point), bins[d][n], mom[ntot*4][n]. */
 int chili_mc_points(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_t n, int32_t* status, int32_t* pick, double* w,
                     double* f, double* val, double* ps, double* x, int32_t* bins, double* mom);
+/* Host integrand and PreProcess: the slow path for arbitrary user functions
+ * (the reference's std::function integrand and PreProcess, include/Channel/
+ * Integrand.hh:41-47, called in MultiChannel.hh:219-229).  With a callback
+ * set, chili_mc_iterate and chili_mc_points hand the momenta of every
+ * attempt of a sub-batch that passed the device cut (f->cuts of the context;
+ * CHILI_CUTS_NONE passes every attempt) to fn in attempt order, attempts
+ * beyond the iteration's last accepted one included: mom [n][ntot][4]
+ * (E, px, py, pz per leg, incoming legs first with negative energies, as the
+ * reference's FSMapper writes them); fn fills f[n] (the integrand) and
+ * pass[n] (the PreProcess decision, 0 rejects) and returns 0, or nonzero to
+ * abort the call (CHILI_EINVAL).  The multichannel weight, statistics and
+ * training stay on the device.  Sub-batches are limited to 2^16 attempts.
+ * The batched, flow, unweighting and batch-evaluation entry points refuse a
+ * context with a callback.  fn = NULL restores the device integrand. */
+typedef int (*chili_host_integrand_fn)(void* user, int64_t n, int ntot, const double* mom, double* f, int32_t* pass);
+int chili_mc_set_host_integrand(chili_mc* m, chili_host_integrand_fn fn, void* user);
 /* Run-time specialisation (no reference counterpart): compile the forward
  * and weight kernels with this context's channel programs as compile-time
  * constants (hipRTC, gfx950); enable bit 0 uses the specialised forward
