@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -51,9 +52,7 @@ static_assert(sizeof(size_t) == 8, "the checkpoint layout assumes a 64-bit size_
 
 class Writer {
   public:
-    explicit Writer(const char* path) : out_(path, std::ios::binary) {
-        if (!out_) throw chili_error(CHILI_EINVAL, std::string("cannot open ") + path + " for writing");
-    }
+    explicit Writer(std::ostream& out) : out_(out) {}
     template <class T>
     void put(const T& v) { out_.write(reinterpret_cast<const char*>(&v), sizeof(T)); }
     void put_size(size_t v) { put<size_t>(v); }
@@ -69,14 +68,12 @@ class Writer {
     }
 
   private:
-    std::ofstream out_;
+    std::ostream& out_;
 };
 
 class Reader {
   public:
-    explicit Reader(const char* path) : in_(path, std::ios::binary) {
-        if (!in_) throw chili_error(CHILI_EINVAL, std::string("cannot open ") + path);
-    }
+    explicit Reader(std::istream& in) : in_(in) {}
     template <class T>
     T get() {
         T v{};
@@ -106,7 +103,7 @@ class Reader {
     }
 
   private:
-    std::ifstream in_;
+    std::istream& in_;
 };
 
 // One channel as the file holds it.
@@ -157,9 +154,8 @@ bool same_channel(const FileChannel& f, const Topology& t) {
     return true;
 }
 
-FileState read_file(const char* path) {
-    Reader r(path);
-    FileState fs;
+// Integrand::Serialize (Integrand.hh:48-55): the channels
+void read_channels(Reader& r, FileState& fs) {
     const size_t C = r.get_size(1 << 20);
     fs.channels.resize(C);
     for (auto& ch : fs.channels) {
@@ -184,6 +180,10 @@ FileState read_file(const char* path) {
         const size_t np = r.get_size(CHILI_MAX_FLAV);
         for (size_t i = 0; i < np; ++i) ch.ptmin.push_back(r.get<double>());
     }
+}
+
+// MultiChannel::Serialize (MultiChannel.cc:94-107)
+void read_multichannel(Reader& r, FileState& fs) {
     fs.ndims = r.get_size(1 << 10);
     chili_checkpoint_state& st = fs.st;
     st.ncalls = r.get<uint64_t>();
@@ -211,7 +211,64 @@ FileState read_file(const char* path) {
     st.nresults = (int64_t)nres;
     const size_t nb2 = r.get_size(1 << 20);
     for (size_t i = 0; i < nb2; ++i) (void)r.get<double>();   // the summary's copy of best_weights
+}
+
+FileState read_file(const char* path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw chili_error(CHILI_EINVAL, std::string("cannot open ") + path);
+    Reader r(in);
+    FileState fs;
+    read_channels(r, fs);
+    read_multichannel(r, fs);
     return fs;
+}
+
+// MultiChannel::Serialize's bytes for a state.
+void write_multichannel(Writer& w, size_t ndims, const chili_checkpoint_state* st, size_t nbest, const double* best,
+                        const chili_stats* results) {
+    w.put_size(ndims);
+    w.put<uint64_t>(st->ncalls);
+    w.put<uint64_t>(st->niterations);
+    w.put<double>(st->rtol);
+    w.put<uint64_t>(st->nrefine);
+    w.put<double>(st->beta);
+    w.put<double>(st->min_alpha);
+    w.put<double>(st->refine_size);
+    w.put<uint64_t>(st->max_bins);
+    w.put_size(nbest);
+    for (size_t c = 0; c < nbest; ++c) w.put<double>(best[c]);
+    w.put<double>(st->min_diff);
+    w.put_size((size_t)st->nresults);
+    for (int64_t i = 0; i < st->nresults; ++i) {
+        const chili_stats& s = results[i];
+        for (double v : {s.n, s.min, s.max, s.sum, s.sum2, s.n_finite}) w.put<double>(v);
+    }
+    w.put_size(nbest);
+    for (size_t c = 0; c < nbest; ++c) w.put<double>(best[c]);
+}
+
+// Index in the channel set of each file channel (matched by description);
+// checks the process, the grids' shape and the pT minima.
+std::vector<int> match_channels(const ChannelSet& set, const FileState& fs) {
+    const size_t C = set.topo.size(), d = (size_t)set.ndims;
+    if (fs.channels.size() != C) throw chili_error(CHILI_EINVAL, "a different number of channels");
+    if (C == 0) return {};
+    const size_t B = fs.channels[0].bins;
+    std::vector<int> used(C, 0), perm(C, -1);
+    for (size_t k = 0; k < C; ++k) {
+        const FileChannel& f = fs.channels[k];
+        if (f.bins != B || f.dims != d) throw chili_error(CHILI_EINVAL, "grids differ in shape");
+        if (f.ntot != (size_t)set.ntot || f.nout != (size_t)set.ntot - 2 || f.sqrts != set.model.sqrts)
+            throw chili_error(CHILI_EINVAL, "the channels belong to a different process");
+        size_t c = C;
+        for (size_t j = 0; j < C; ++j)
+            if (!used[j] && same_channel(f, set.topo[j])) { c = j; break; }
+        if (c == C) throw chili_error(CHILI_EINVAL, "channel " + std::to_string(k) + " matches none of the process's");
+        used[c] = 1;
+        if (f.ptmin != mapper_ptmin(set, set.topo[c])) throw chili_error(CHILI_EINVAL, "pT minima differ from the cuts");
+        perm[k] = (int)c;
+    }
+    return perm;
 }
 
 }  // namespace
@@ -226,7 +283,9 @@ int chili_checkpoint_write(const chili_channels* ch, int nbins, const double* gr
         if (nbins < 1) throw chili_error(CHILI_EINVAL, "nbins must be positive");
         const ChannelSet& set = ch->set;
         const size_t C = set.topo.size(), d = (size_t)set.ndims, B = (size_t)nbins;
-        Writer w(path);
+        std::ofstream out(path, std::ios::binary);
+        if (!out) throw chili_error(CHILI_EINVAL, std::string("cannot open ") + path + " for writing");
+        Writer w(out);
         w.put_size(C);
         for (size_t c = 0; c < C; ++c) {
             const Topology& t = set.topo[c];
@@ -252,25 +311,7 @@ int chili_checkpoint_write(const chili_channels* ch, int nbins, const double* gr
             w.put_size(ptmin.size());
             for (double v : ptmin) w.put<double>(v);
         }
-        w.put_size(d);
-        w.put<uint64_t>(st->ncalls);
-        w.put<uint64_t>(st->niterations);
-        w.put<double>(st->rtol);
-        w.put<uint64_t>(st->nrefine);
-        w.put<double>(st->beta);
-        w.put<double>(st->min_alpha);
-        w.put<double>(st->refine_size);
-        w.put<uint64_t>(st->max_bins);
-        w.put_size(C);
-        for (size_t c = 0; c < C; ++c) w.put<double>(best_weights[c]);
-        w.put<double>(st->min_diff);
-        w.put_size((size_t)st->nresults);
-        for (int64_t i = 0; i < st->nresults; ++i) {
-            const chili_stats& s = results[i];
-            for (double v : {s.n, s.min, s.max, s.sum, s.sum2, s.n_finite}) w.put<double>(v);
-        }
-        w.put_size(C);
-        for (size_t c = 0; c < C; ++c) w.put<double>(best_weights[c]);
+        write_multichannel(w, d, st, C, best_weights, results);
         w.finish();
     });
 }
@@ -301,24 +342,82 @@ int chili_checkpoint_read(const chili_channels* ch, const char* path, double* gr
         if (fs.best.size() != C) throw chili_error(CHILI_EINVAL, "checkpoint weights do not match its channels");
         if (fs.st.nresults > results_cap) throw chili_error(CHILI_ERANGE, "results buffer too small");
         const size_t B = fs.channels[0].bins;
-        std::vector<int> used(C, 0);
+        // the reference's channel order need not be this build's: match by description
+        const std::vector<int> perm = match_channels(set, fs);
         for (size_t k = 0; k < C; ++k) {
-            const FileChannel& f = fs.channels[k];
-            if (f.bins != B || f.dims != d) throw chili_error(CHILI_EINVAL, "checkpoint grids differ in shape");
-            if (f.ntot != (size_t)set.ntot || f.nout != (size_t)set.ntot - 2 || f.sqrts != set.model.sqrts)
-                throw chili_error(CHILI_EINVAL, "checkpoint belongs to a different process");
-            // the reference's channel order need not be this build's: match by description
-            size_t c = C;
-            for (size_t j = 0; j < C; ++j)
-                if (!used[j] && same_channel(f, set.topo[j])) { c = j; break; }
-            if (c == C) throw chili_error(CHILI_EINVAL, "checkpoint channel " + std::to_string(k) + " matches none of the process's");
-            used[c] = 1;
-            if (f.ptmin != mapper_ptmin(set, set.topo[c])) throw chili_error(CHILI_EINVAL, "checkpoint pT minima differ from the cuts");
-            std::copy(f.edges.begin(), f.edges.end(), grids + c * d * (B + 1));
+            const size_t c = (size_t)perm[k];
+            std::copy(fs.channels[k].edges.begin(), fs.channels[k].edges.end(), grids + c * d * (B + 1));
             best_weights[c] = fs.best[k];
         }
         *st = fs.st;
         for (int64_t i = 0; i < fs.st.nresults; ++i) results[i] = fs.results[(size_t)i];
+    });
+}
+
+// The channels of an Integrand::Serialize byte stream (include/Channel/
+// Integrand.hh:48-55) matched to a channel set by description: perm[k] is
+// the set's index of the stream's channel k; grids (optional, [C][d][B+1])
+// receive the stream's grids in the set's order, nbins their bin count.
+int chili_channels_match(const chili_channels* ch, const void* bytes, int64_t len, int32_t* perm, double* grids, int* nbins) {
+    return guarded([&] {
+        if (!ch || !bytes || len < 0 || !perm) throw chili_error(CHILI_EINVAL, "null argument");
+        std::istringstream in(std::string((const char*)bytes, (size_t)len), std::ios::binary);
+        Reader r(in);
+        FileState fs;
+        read_channels(r, fs);
+        const std::vector<int> p = match_channels(ch->set, fs);
+        const size_t C = p.size(), d = (size_t)ch->set.ndims;
+        const size_t B = C ? fs.channels[0].bins : 0;
+        for (size_t k = 0; k < C; ++k) {
+            perm[k] = p[k];
+            if (grids) std::copy(fs.channels[k].edges.begin(), fs.channels[k].edges.end(), grids + (size_t)p[k] * d * (B + 1));
+        }
+        if (nbins) *nbins = (int)B;
+    });
+}
+
+// MultiChannel::Deserialize's input (src/Channel/MultiChannel.cc:79-92), read
+// from a MultiChannel::Serialize byte stream: ndims, the parameters and
+// min_diff in st (st->nresults = the summary's entry count), best weights
+// (*nbest of them, at most best_cap) and the summary's results (at most
+// results_cap).
+int chili_mc_state_read(const void* bytes, int64_t len, int* ndims, chili_checkpoint_state* st, double* best, int best_cap, int* nbest,
+                        chili_stats* results, int64_t results_cap) {
+    return guarded([&] {
+        if (!bytes || len < 0 || !st || !nbest) throw chili_error(CHILI_EINVAL, "null argument");
+        std::istringstream in(std::string((const char*)bytes, (size_t)len), std::ios::binary);
+        Reader r(in);
+        FileState fs;
+        read_multichannel(r, fs);
+        // a NULL buffer only asks for the counts
+        if ((best && (int64_t)fs.best.size() > best_cap) || (results && fs.st.nresults > results_cap))
+            throw chili_error(CHILI_ERANGE, "output buffers too small");
+        if (ndims) *ndims = (int)fs.ndims;
+        *st = fs.st;
+        *nbest = (int)fs.best.size();
+        if (best) std::copy(fs.best.begin(), fs.best.end(), best);
+        if (results) std::copy(fs.results.begin(), fs.results.end(), results);
+    });
+}
+
+// MultiChannel::Serialize's bytes (src/Channel/MultiChannel.cc:94-107) of a
+// state: written to buf (cap bytes); *len receives the size (call with
+// buf = NULL to query it).
+int chili_mc_state_write(int ndims, const chili_checkpoint_state* st, int nbest, const double* best, const chili_stats* results, void* buf,
+                         int64_t cap, int64_t* len) {
+    return guarded([&] {
+        if (!st || !len || (nbest > 0 && !best) || (st->nresults > 0 && !results) || ndims < 0 || nbest < 0)
+            throw chili_error(CHILI_EINVAL, "null argument");
+        std::ostringstream out(std::ios::binary);
+        Writer w(out);
+        write_multichannel(w, (size_t)ndims, st, (size_t)nbest, best, results);
+        w.finish();
+        const std::string s = out.str();
+        *len = (int64_t)s.size();
+        if (buf) {
+            if (cap < (int64_t)s.size()) throw chili_error(CHILI_ERANGE, "buffer too small");
+            std::memcpy(buf, s.data(), s.size());
+        }
     });
 }
 

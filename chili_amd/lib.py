@@ -20,7 +20,8 @@ EXPORTS = [
     "chili_mc_create", "chili_mc_destroy", "chili_mc_info", "chili_mc_set_grids", "chili_mc_get_grids",
     "chili_mc_set_alpha", "chili_mc_get_alpha", "chili_mc_iterate", "chili_mc_points", "chili_mc_timing",
     "chili_vegas_create", "chili_vegas_destroy", "chili_vegas_set_grid", "chili_vegas_get_grid", "chili_vegas_iterate",
-    "chili_vegas_iterate_static", "chili_mc_set_host_integrand", "chili_mc_optimize", "chili_vegas_optimize", "chili_mc_params_default",
+    "chili_vegas_iterate_static", "chili_mc_set_host_integrand", "chili_channels_match", "chili_mc_state_read",
+    "chili_mc_state_write", "chili_mc_optimize", "chili_vegas_optimize", "chili_mc_params_default",
     "chili_mc_summary_init", "chili_vegas_params_default",
     "chili_grid_adapt", "chili_grid_split", "chili_alpha_adapt", "chili_max_difference",
     "chili_mc_flow_iterate", "chili_mc_flow_data", "chili_mc_flow_points",

@@ -396,6 +396,27 @@ int chili_checkpoint_info(const char* path, int* nchannels, int* ndims, int* nbi
 int chili_checkpoint_read(const chili_channels* ch, const char* path, double* grids, double* best_weights,
                           chili_checkpoint_state* st, chili_stats* results, int64_t results_cap);
 
+/* The state exchange of the C++ facade (chili_gpu_facade.hh) with the
+ * reference's own objects, through their public byte streams. */
+/* Channels of an Integrand::Serialize stream (include/Channel/Integrand.hh:
+ * 48-55: per channel its AdaptiveMap and FSMapper) matched to ch by
+ * description: perm[k] = ch's index of the stream's channel k; grids
+ * (optional, [C][d][nbins+1]) receive the stream's grids in ch's order.  The
+ * stream's process (sqrts, particle count, pT minima) must be ch's. */
+int chili_channels_match(const chili_channels* ch, const void* bytes, int64_t len, int32_t* perm, double* grids, int* nbins);
+/* A MultiChannel::Serialize stream (src/Channel/MultiChannel.cc:94-107)
+ * read: ndims, MultiChannelParams and min_diff (st; st->nresults = the
+ * summary's entries), best_weights (*nbest <= best_cap) and the summary's
+ * StatsData (<= results_cap); best / results may be NULL to ask for the
+ * counts first. */
+int chili_mc_state_read(const void* bytes, int64_t len, int* ndims, chili_checkpoint_state* st, double* best, int best_cap, int* nbest,
+                        chili_stats* results, int64_t results_cap);
+/* The MultiChannel::Serialize bytes of a state (what MultiChannel::Deserialize,
+ * MultiChannel.cc:79-92, reads): written to buf (cap bytes), *len = size;
+ * buf = NULL queries the size. */
+int chili_mc_state_write(int ndims, const chili_checkpoint_state* st, int nbest, const double* best, const chili_stats* results, void* buf,
+                         int64_t cap, int64_t* len);
+
 /* ---- learned proposal: per-channel coupling-layer spline flows on the
  *      VEGAS hypercube.  No counterpart in the reference; the sampler it
  *      feeds replaces MultiChannel::GeneratePoint's uniforms
