@@ -176,6 +176,19 @@ __device__ __forceinline__ float sigmoidf(float z) {
     return 1.f / (1.f + __builtin_amdgcn_exp2f(-z * kLog2e));
 }
 
+// 1 / x in FP64 from the hardware's reciprocal estimate and two Newton
+// steps, e = 1 - x r, r += r e (each step squares the relative error: within
+// an ulp or two of the correctly rounded quotient, for 5 instead of ~11
+// instructions); x is a spline width, slope or denominator, positive and far
+// from the range's ends.
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 struct SplineBin {
     int k;
     double X0, Y0, w, h, d0, d1;
@@ -219,8 +232,10 @@ __device__ __forceinline__ SplineBin spline_bin(const U& u, float shift, double 
         SW[i + 1] = SW[i] + ew[i];
         SH[i + 1] = SH[i] + eh[i];
     }
-    rw = 1.f / SW[K];
-    rh = 1.f / SH[K];
+    // hardware reciprocal (about one ulp): the knots only need to be the same
+    // wherever they are computed, which they are (one expression)
+    rw = __builtin_amdgcn_rcpf(SW[K]);
+    rh = __builtin_amdgcn_rcpf(SH[K]);
     // knots X_i = fma((1 - K minW) / sum, S_i, i minW) for 0 < i < K (the
     // same expression wherever a knot is needed), X_0 = 0, X_K = 1; Y alike
     const float aw = (float)(1.0 - K * kMinW) * rw, ah = (float)(1.0 - K * kMinH) * rh;
@@ -307,14 +322,17 @@ __device__ __forceinline__ SplineBin spline_bin(const U& u, float shift, double 
 }
 
 // Forward transform x -> y, jac = dy/dx.
+// The transforms are this build's own arithmetic (no reference operation
+// order to keep): multiply-adds may fuse, divisions are reciprocals.
 template <int K, class U>
 __device__ __forceinline__ double rqs_forward_j(const U& u, float shift, double x, double& jac) {
+#pragma clang fp contract(fast)
     float ew[K], eh[K], rw, rh;
     x = fmin(fmax(x, 0.0), 1.0);
     const SplineBin b = spline_bin<K>(u, shift, x, false, ew, eh, rw, rh);
-    const double iw = 1.0 / b.w;
+    const double iw = rcp64(b.w);
     const double xi = (x - b.X0) * iw, s = b.h * iw, t = xi * (1 - xi);
-    const double iden = 1.0 / (s + (b.d1 + b.d0 - 2 * s) * t);
+    const double iden = rcp64(s + (b.d1 + b.d0 - 2 * s) * t);
     const double D = b.d1 * xi * xi + 2 * s * t + b.d0 * (1 - xi) * (1 - xi);
     jac = s * s * D * iden * iden;
     return b.Y0 + b.h * (s * xi * xi + b.d0 * t) * iden;
@@ -332,18 +350,19 @@ __device__ __forceinline__ double rqs_forward(const U& u, float shift, double x,
 // Inverse transform y -> x; jac receives dy/dx at that x.
 template <int K, class U>
 __device__ __forceinline__ double rqs_inverse_j(const U& u, float shift, double y, double& jac) {
+#pragma clang fp contract(fast)
     float ew[K], eh[K], rw, rh;
     y = fmin(fmax(y, 0.0), 1.0);
     const SplineBin b = spline_bin<K>(u, shift, y, true, ew, eh, rw, rh);
-    const double s = b.h / b.w, dy = y - b.Y0, E = b.d1 + b.d0 - 2 * s;
+    const double s = b.h * rcp64(b.w), dy = y - b.Y0, E = b.d1 + b.d0 - 2 * s;
     const double a = b.h * (s - b.d0) + dy * E;
     const double bq = b.h * b.d0 - dy * E;
     const double c = -s * dy;
     const double disc = fmax(bq * bq - 4 * a * c, 0.0);
-    double xi = (2 * c) / (-bq - sqrt(disc));
+    double xi = (2 * c) * rcp64(-bq - sqrt(disc));
     xi = fmin(fmax(xi, 0.0), 1.0);
     const double t = xi * (1 - xi);
-    const double iden = 1.0 / (s + E * t);
+    const double iden = rcp64(s + E * t);
     const double D = b.d1 * xi * xi + 2 * s * t + b.d0 * (1 - xi) * (1 - xi);
     jac = s * s * D * iden * iden;
     return b.X0 + xi * b.w;
