@@ -107,7 +107,7 @@ def measured_traffic(workload, variant, pts_launch):
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/),
     scaled by the counters' reading of a copy kernel of known size; None if
     absent."""
-    kind = "specialized" if variant in ("specialized", "specialized-weight") else "generic"
+    kind = "split" if "split" in variant else ("specialized" if variant in ("specialized", "specialized-weight") else "generic")
     path = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}_{kind}.json")
     if not os.path.exists(path):
         return None, None
@@ -358,6 +358,9 @@ def main():
     ap.add_argument("--no-specialize", action="store_true",
                     help="keep the generic kernels (default: the autotuner compiles kernels with the channel programs as "
                          "constants and keeps them when a production step is faster)")
+    ap.add_argument("--specialize-mode", type=int, default=5,
+                    help="kernel variant of --traffic-probe (chili_mc_specialize bits: 1 forward, 2 fused weight, 4 split "
+                         "weight)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: adapt, run two production steps and a 1 GiB device copy (the "
                          "calibration kernel of known traffic), print nothing else")
@@ -392,7 +395,7 @@ def main():
     ad = adapt(gpu, W, seed)
     if args.traffic_probe:
         if not args.no_specialize:
-            gpu.specialize(True)
+            gpu.specialize(args.specialize_mode)
         for s in range(2):
             gpu.iterate(abi.philox_uniforms(seed, stream=S_PROD + s), W["batch"], train=False)
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
@@ -456,9 +459,10 @@ def main():
                 best = dt if best is None else min(best, dt)
             return best
         t_variant["generic"] = one_step(900)
-        names = {1: "specialized-forward", 2: "specialized-weight", 3: "specialized"}
+        names = {1: "specialized-forward", 2: "specialized-weight", 3: "specialized", 4: "split-weight",
+                 5: "specialized-forward+split-weight"}
         try:
-            for mode in (1, 2, 3):
+            for mode in (1, 2, 3, 4, 5):
                 compile_s = gpu.specialize(mode)
                 t_variant[names[mode]] = one_step(900 + 10 * mode)
             variant = min(t_variant, key=t_variant.get)

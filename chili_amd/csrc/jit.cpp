@@ -74,7 +74,9 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
           << arr(p.dec_smax) << ", " << arr(p.t_slot) << ", " << arr(p.t_ptmin) << ", " << arr(p.t_etacut) << ", "
           << arr(p.s_par) << ", " << arr(p.s_a) << ", " << arr(p.s_b) << ", 0}; }\n";
     }
+    s << "#ifndef CHILI_INV_WAVES\n#define CHILI_INV_WAVES 2\n#endif\n";
     s << "struct JitLoop {\n"
+      << "    static constexpr bool kPre = false;\n"
       << "    template <class F>\n"
       << "    __device__ __forceinline__ static void each(const MCArgs& A, F&& f) {\n";
     for (size_t j = 0; j < progs.size(); ++j) s << "        f(" << j << ", chili_prog_" << j << "());\n";
@@ -90,6 +92,24 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
              "    if (!locate_tile(A.C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;\n"
              "    main_tile<NT, kRounds, JitLoop, " << (ls ? "true" : "false") << ">(A, ch, e0, e1, s_mem, blockIdx.x, psbuf, psstride,\n"
              "        train, comp, mom, nmax, part + (int64_t)blockIdx.x * rec, vals_out, vals_off, vals_cap, sorted, R);\n"
+             "}\n\n";
+    // split weight pass: one kernel per channel (its inverse map alone, so
+    // that its momentum slots stay in registers)
+    for (size_t j = 0; j < progs.size(); ++j)
+        s << "extern \"C\" __global__ __launch_bounds__(kTile, CHILI_INV_WAVES) void chili_k_inv_" << j
+          << "(MCArgs A, int64_t kept, const int* __restrict__ seg2,\n"
+             "        const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax, double* __restrict__ pv,\n"
+             "        short* __restrict__ bins, const uint32_t* __restrict__ sorted, PointRec R, int lds_search) {\n"
+             "    extern __shared__ double s_srch[];\n"
+             "    if (lds_search) {\n"
+             "        const double* g = A.search + (int64_t)" << j << " * A.d * A.P2;\n"
+             "        for (int i = threadIdx.x; i < A.d * A.P2; i += kTile) s_srch[i] = g[i];\n"
+             "        __syncthreads();\n"
+             "    }\n"
+             "    const int64_t e = (int64_t)blockIdx.x * kTile + threadIdx.x;\n"
+             "    if (e >= kept) return;\n"
+             "    inverse_one<NT>(A, chili_prog_" << j << "(), " << j << ", e, seg2, comp, mom, nmax, pv, bins, sorted, R,\n"
+             "                    lds_search ? s_srch : nullptr);\n"
              "}\n\n";
     s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward(MCArgs A, const int* __restrict__ seg_start,\n"
          "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"
@@ -111,7 +131,7 @@ Module::~Module() {
     if (mod) (void)hipModuleUnload(mod);
 }
 
-void compile(const std::string& src, Module& out, double* seconds, bool load) {
+void compile(const std::string& src, Module& out, double* seconds, bool load, int nchannels) {
     std::vector<const char*> hdr, names;
     for (int i = 0; i < kJitHeaderCount; ++i) {
         names.push_back(kJitHeaders[i].name);
@@ -143,6 +163,9 @@ void compile(const std::string& src, Module& out, double* seconds, bool load) {
     HIP_CHECK(hipModuleGetFunction(&out.main, out.mod, "chili_k_main"));
     HIP_CHECK(hipModuleGetFunction(&out.main_lds, out.mod, "chili_k_main_lds"));
     HIP_CHECK(hipModuleGetFunction(&out.forward, out.mod, "chili_k_forward"));
+    out.inv.assign((size_t)nchannels, nullptr);
+    for (int j = 0; j < nchannels; ++j)
+        HIP_CHECK(hipModuleGetFunction(&out.inv[(size_t)j], out.mod, ("chili_k_inv_" + std::to_string(j)).c_str()));
 }
 
 }  // namespace chili_jit

@@ -12,6 +12,7 @@ namespace chili_jit {
 struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t main = nullptr, main_lds = nullptr, forward = nullptr;
+    std::vector<hipFunction_t> inv;      // split weight pass: one inverse kernel per channel
     Module() = default;
     Module(const Module&) = delete;
     Module& operator=(const Module&) = delete;
@@ -22,6 +23,6 @@ struct Module {
 std::string kernel_source(const std::vector<chili_host::ChanProg>& progs, int NT);
 // hipRTC compile for gfx950 and (load) load; seconds (optional) gets the
 // compile time.
-void compile(const std::string& src, Module& out, double* seconds, bool load = true);
+void compile(const std::string& src, Module& out, double* seconds, bool load = true, int nchannels = 0);
 
 }  // namespace chili_jit

@@ -328,6 +328,21 @@ __global__ __launch_bounds__(kTile) void k_main(MCArgs A, const int* __restrict_
                            vals_out, vals_off, vals_cap, sorted, R);
 }
 
+// The combination step of the split weight pass (PreLoop: ps, VEGAS weights
+// and bins come from the per-channel inverse kernels).
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_main_pre(MCArgs A, const int* __restrict__ seg2, const int* __restrict__ tile2,
+                                                    const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
+                                                    double* __restrict__ psbuf, int64_t psstride, int train,
+                                                    double* __restrict__ part, int64_t rec, double* __restrict__ vals_out,
+                                                    int64_t vals_off, int64_t vals_cap, const uint32_t* __restrict__ sorted, PointRec R) {
+    extern __shared__ double s_mem[];
+    int ch, e0, e1;
+    if (!locate_tile(A.C, seg2, tile2, blockIdx.x, ch, e0, e1, kMainTile)) return;
+    main_tile<NT, kRounds, PreLoop, false>(A, ch, e0, e1, s_mem, blockIdx.x, psbuf, psstride, train, comp, mom, nmax,
+                                           part + (int64_t)blockIdx.x * rec, vals_out, vals_off, vals_cap, sorted, R);
+}
+
 // ---------------------------------------------------------------------------
 // Fixed-order reduction of the workgroup partials into the running totals
 // tot = [stats(7), T(C), hist(C x d x B)], in two levels so that no thread
@@ -785,6 +800,9 @@ struct chili_mc {
     // run-time specialised forward / weight kernels (channel programs as constants)
     std::unique_ptr<chili_jit::Module> jit;
     bool jit_fwd = false, jit_main = false;
+    bool jit_split = false;      // split weight pass: per-channel specialised inverse kernels + combine
+    DevBuf<double> d_pre;        // split pass: ps and VEGAS weights [2 C][kept]
+    DevBuf<short> d_prebins;     // split pass: the picked channel's bins [d][kept]
     bool lds_search = true;      // weight kernel: search tables in LDS when they fit
     double gen_rate = 0.01;      // unweighting stage: kept events per attempt (running estimate)
     double jit_compile_s = 0;
@@ -987,8 +1005,39 @@ void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, c
 }
 template <int NT>
 void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals, int64_t vals_off, int64_t vals_cap,
-                 const PointRec& R) {
+                 const PointRec& R, int64_t kept) {
     constexpr int D = 3 * (NT - 2) - 2;
+    if (m->jit_split) {
+        // split weight pass: every channel's inverse in its own specialised
+        // kernel (ps, VEGAS weight, the picked channel's bins by kept index),
+        // then the combination (W, value, statistics, training sums)
+        m->d_pre.resize((size_t)2 * m->C * kept);
+        m->d_prebins.resize((size_t)D * kept);
+        MCArgs a = A;
+        a.pre_pv = m->d_pre.ptr;
+        a.pre_bins = m->d_prebins.ptr;
+        a.pre_stride = kept;
+        const size_t one_bytes = sizeof(double) * (size_t)D * m->P2;
+        int lds_search = (m->lds_search && one_bytes <= kLdsSearchMax) ? 1 : 0;
+        const int* seg2 = m->d_seg2.ptr;
+        const uint32_t* comp = m->d_comp.ptr;
+        const double* mom = m->d_mom.ptr;
+        int64_t nmax = m->nmax, kp = kept;
+        double* pv = m->d_pre.ptr;
+        short* bins = m->d_prebins.ptr;
+        const uint32_t* sorted = m->d_sorted.ptr;
+        PointRec r = R;
+        void* args[] = {&a, &kp, &seg2, &comp, &mom, &nmax, &pv, &bins, &sorted, &r, &lds_search};
+        const unsigned g = (unsigned)((kept + kTile - 1) / kTile);
+        for (int j = 0; j < m->C; ++j)
+            HIP_CHECK(hipModuleLaunchKernel(m->jit->inv[(size_t)j], g, 1, 1, kTile, 1, 1, lds_search ? (unsigned)one_bytes : 0u, m->stream,
+                                            args, nullptr));
+        const size_t lds = sizeof(double) * (D * m->B + kWaves * (8 + m->C)) + sizeof(short) * D * kTile;
+        hipLaunchKernelGGL((k_main_pre<NT>), dim3(grid), dim3(kTile), lds, m->stream, a, m->d_seg2.ptr, m->d_tiles2.ptr, m->d_comp.ptr,
+                           m->d_mom.ptr, m->nmax, m->d_ps.ptr, (int64_t)grid * kTile, train, m->d_part.ptr,
+                           (int64_t)(kStatW + m->C + (int64_t)m->d * m->B), vals, vals_off, vals_cap, m->d_sorted.ptr, R);
+        return;
+    }
     size_t lds = sizeof(double) * (D * m->B + kWaves * (8 + m->C)) + sizeof(short) * D * kTile;
     if (lds > 160 * 1024) throw chili_error(CHILI_ERANGE, "channel count x bins exceed the LDS of one workgroup");
     // small search tables go to LDS with the rest of the workgroup's state
@@ -1180,7 +1229,7 @@ void run_subbatch_gen(chili_mc* m, uint64_t seed, uint32_t stream, uint64_t a0, 
             evt = take_events(m);
             HIP_CHECK(hipEventRecord(evt.first, m->stream));
         }
-        CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, 0, m->d_gval.ptr, 0, passed, PointRec{}));
+        CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, 0, m->d_gval.ptr, 0, passed, PointRec{}, passed));
         if (m->timing) {
             HIP_CHECK(hipEventRecord(evt.second, m->stream));
             m->ev_live.push_back(evt);
@@ -1223,7 +1272,7 @@ void run_subbatch(chili_mc* m, const double* u, int64_t ustride, uint64_t ua0, u
             ev = take_events(m);
             HIP_CHECK(hipEventRecord(ev.first, m->stream));
         }
-        CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, train, vals, vals_off, vals_cap, R));
+        CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, train, vals, vals_off, vals_cap, R, kept));
         if (m->timing) {
             HIP_CHECK(hipEventRecord(ev.second, m->stream));
             m->ev_live.push_back(ev);
@@ -2177,7 +2226,7 @@ int chili_mc_evaluate(chili_mc* m, int channel, const double* rans_dev, int64_t 
         if (passed > 0) {
             m->d_gval.resize(passed);
             const int mgrid = (int)((passed + kMainTile - 1) / kMainTile) + m->C;
-            CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, 0, m->d_gval.ptr, 0, passed, PointRec{}));
+            CHILI_DISPATCH_NT(m->NT, launch_main<NT>(m, A, mgrid, 0, m->d_gval.ptr, 0, passed, PointRec{}, passed));
             hipLaunchKernelGGL(k_vals_to_rows, dim3((unsigned)((passed + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, passed,
                                m->d_comp.ptr, m->d_sorted.ptr, m->d_gval.ptr, out_dev);
         }
@@ -2612,11 +2661,12 @@ int chili_mc_specialize(chili_mc* m, int enable, double* compile_s) {
         select_device(m->device);
         if (enable && !m->jit) {
             auto mod = std::make_unique<chili_jit::Module>();
-            chili_jit::compile(chili_jit::kernel_source(m->set.progs, m->NT), *mod, &m->jit_compile_s);
+            chili_jit::compile(chili_jit::kernel_source(m->set.progs, m->NT), *mod, &m->jit_compile_s, true, m->C);
             m->jit = std::move(mod);
         }
         m->jit_fwd = (enable & 1) != 0;
-        m->jit_main = (enable & 2) != 0;
+        m->jit_split = (enable & 4) != 0;
+        m->jit_main = (enable & 2) != 0 && !m->jit_split;
         if (compile_s) *compile_s = m->jit_compile_s;
     });
 }

@@ -180,14 +180,23 @@ __device__ __forceinline__ void main_tile(const MCArgs& A, int ch, int e0, int e
         // Integrand::GenerateWeight (Integrand.hh:112-123)
         double wsum = 0;
         Loop::each(A, [&](int j, const ChanProg& pj) {
-            double vw;
-            short* bo = (j == ch) ? (s_bins + threadIdx.x) : nullptr;
-            const bool recx = R.status && valid && j == ch;
-            double ps = generate_weight<NT>(A, pj, j, P, &vw, bo, kTile, recx ? R.x + att : nullptr, recx ? R.bins + att : nullptr, R.n,
-                                            nullptr, LS ? s_srch : nullptr);
+            double vw, ps;
+            if constexpr (Loop::kPre) {
+                // split pass: every channel's inverse ran in its own kernel
+                const int64_t ek = valid ? e : 0;
+                ps = A.pre_pv[(int64_t)j * A.pre_stride + ek];
+                vw = A.pre_pv[(int64_t)(A.C + j) * A.pre_stride + ek];
+                if (j == ch)
+                    for (int k = 0; k < D; ++k) s_bins[k * kTile + threadIdx.x] = A.pre_bins[(int64_t)k * A.pre_stride + ek];
+            } else {
+                short* bo = (j == ch) ? (s_bins + threadIdx.x) : nullptr;
+                const bool recx = R.status && valid && j == ch;
+                ps = generate_weight<NT>(A, pj, j, P, &vw, bo, kTile, recx ? R.x + att : nullptr, recx ? R.bins + att : nullptr, R.n,
+                                         nullptr, LS ? s_srch : nullptr);
+                if (R.status && valid) R.ps[(int64_t)j * R.n + att] = ps;
+            }
             wsum += A.alpha[j] / ps / vw;
             psbuf[(int64_t)j * psstride + gt] = ps;
-            if (R.status && valid) R.ps[(int64_t)j * R.n + att] = ps;
         });
         if (valid) {
             W = 1.0 / wsum;
@@ -275,10 +284,54 @@ __device__ __forceinline__ void main_tile(const MCArgs& A, int ch, int e0, int e
 
 // Channel loop over the programs in device memory.
 struct GenericLoop {
+    static constexpr bool kPre = false;
     template <class F>
     __device__ __forceinline__ static void each(const MCArgs& A, F&& f) {
         for (int j = 0; j < A.C; ++j) f(j, A.progs[j]);
     }
 };
+
+// Channel loop of the split weight pass: the per-channel inverse kernels
+// (inverse_one, run-time specialised) left ps, the VEGAS weight and the
+// picked channel's bins in A.pre_*; no program is evaluated here.
+struct PreLoop {
+    static constexpr bool kPre = true;
+    template <class F>
+    __device__ __forceinline__ static void each(const MCArgs& A, F&& f) {
+        for (int j = 0; j < A.C; ++j) f(j, A.progs[0]);
+    }
+};
+
+// One kept point e of the split weight pass for channel j (program p): its
+// inverse map and VEGAS weight (FSMapper::GenerateWeight, Channel.cc:40-61;
+// AdaptiveMap::GenerateWeight, AdaptiveMap.cc:58-65), and, when j is the
+// point's picked channel, its bins; search: this channel's table in LDS or
+// null (device memory).
+template <int NT>
+__device__ __forceinline__ void inverse_one(const MCArgs& A, const ChanProg& p, int j, int64_t e, const int* __restrict__ seg2,
+                                            const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
+                                            double* __restrict__ pv, short* __restrict__ bins, const uint32_t* __restrict__ sorted,
+                                            const PointRec& R, const double* search) {
+    const uint32_t sp = comp[e];
+    MomRF<2 * NT> P;
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
+        P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
+        P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
+        P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+    }
+    int pc = 0;                         // picked channel: the kept segment holding e
+    for (int c = 1; c < A.C; ++c) pc = (seg2[c] <= e) ? c : pc;
+    const bool own = pc == j;
+    const uint32_t att = R.status ? sorted[sp] : 0;
+    const bool recx = R.status && own;
+    double vw;
+    const double ps = generate_weight<NT>(A, p, j, P, &vw, own ? bins + e : nullptr, (int)A.pre_stride, recx ? R.x + att : nullptr,
+                                          recx ? R.bins + att : nullptr, R.n, nullptr, search, search != nullptr);
+    pv[(int64_t)j * A.pre_stride + e] = ps;
+    pv[(int64_t)(A.C + j) * A.pre_stride + e] = vw;
+    if (R.status) R.ps[(int64_t)j * R.n + att] = ps;
+}
 
 }  // namespace chili_dev

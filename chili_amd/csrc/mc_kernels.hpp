@@ -64,6 +64,12 @@ struct MCArgs {
     int C, d, B, P2;
     double sqrts, ptmax;
     const double* fhost;     // integrand values from the host callback, by sorted position (null: device integrand)
+    // split weight pass: the per-channel inverse kernels' results by kept
+    // index e: ps [C][pre_stride], VEGAS weight [C][pre_stride] after them,
+    // the picked channel's bins [d][pre_stride]
+    const double* pre_pv;
+    const short* pre_bins;
+    int64_t pre_stride;
 };
 
 // --------------------------------------------------------------------------
@@ -242,14 +248,16 @@ CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* 
 template <int NT>
 CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, MomRF<2 * NT>& P, double* vw_out,
                                  short* bins_out, int bstride, double* x_out = nullptr, int32_t* bin_rec = nullptr,
-                                 int64_t xstride = 0, double* r_out = nullptr, const double* search = nullptr) {
+                                 int64_t xstride = 0, double* r_out = nullptr, const double* search = nullptr,
+                                 bool search_one = false) {
     constexpr int ND = NT - 3;
     constexpr int NTT = NT - 2;
     const int nd = p.nd, nt = p.nt;
     const double Bd = (double)A.B;
     const double* __restrict__ edges = A.edges + (int64_t)ch * A.d * (A.B + 1);
-    // search tables of every channel: device memory, or a workgroup's LDS copy
-    const double* __restrict__ srch = (search ? search : A.search) + (int64_t)ch * A.d * A.P2;
+    // search tables: device memory, a workgroup's LDS copy of every channel's,
+    // or (search_one) an LDS copy of this channel's alone
+    const double* __restrict__ srch = (search ? search : A.search) + (search_one ? 0 : (int64_t)ch * A.d * A.P2);
     double vw = 1.0;
     int slot = 0;
     const uint16_t* __restrict__ cells = (search || !A.cells) ? nullptr : A.cells + (int64_t)ch * A.d * (kCells + 1);

@@ -119,7 +119,7 @@ def test_points_parity(case, grids):
     assert np.median(d) <= 1e-10 and d.max() <= 1e-6
 
 
-@pytest.mark.parametrize("case", ["gg_2g", "gg_3g", "tt_6j"])
+@pytest.mark.parametrize("case", ["gg_2g", "gg_3g", "gg_4g", "tt_6j"])
 def test_specialized_kernels_parity(case):
     """Kernels compiled at run time with the channel programs as constants
     give the generic kernels' results (and so the oracle's)."""
@@ -146,12 +146,27 @@ def test_specialized_kernels_parity(case):
     assert_rel(spec["w"][ok], generic["w"][ok], 1e-12, what=f"{case} weights")
     staged = ora.points(uni, 3, n, mom_in=spec["mom"])
     compare_weights(spec, staged, f"{case}/specialized")
-    # one kernel specialised, the other generic
-    for mode in (1, 2):
+    # one kernel specialised, the other generic; 4: the split weight pass
+    # (per-channel specialised inverse kernels + combine), 5: with the
+    # specialised forward kernel
+    for mode in (1, 2, 4, 5):
         gpu.specialize(mode)
         mixed = gpu.points(uni, 3, n)
         assert np.array_equal(generic["status"], mixed["status"]), f"{case} mode {mode} status"
         assert_rel(mixed["w"][ok], generic["w"][ok], 1e-12, what=f"{case} mode {mode} weights")
+        assert_rel(mixed["ps"][ok], generic["ps"][ok], 1e-12, what=f"{case} mode {mode} phase-space weights")
+        assert np.array_equal(mixed["bins"][ok], generic["bins"][ok]), f"{case} mode {mode} bins"
+        assert_rel(mixed["xpick"][ok], generic["xpick"][ok], 1e-12, scale=1.0, what=f"{case} mode {mode} inverse points")
+        if mode & 4:
+            # iterate with training through the split pass
+            sm, Tm, hm, nm = gpu.iterate(uni, 5000, a0=0, train=True)
+            gpu.specialize(False)
+            s0, T0_, h0_, n0_ = gpu.iterate(uni, 5000, a0=0, train=True)
+            gpu.specialize(mode)
+            assert nm == n0_ and sm.n == s0.n and sm.n_finite == s0.n_finite
+            assert_rel(np.array([sm.sum, sm.sum2]), np.array([s0.sum, s0.sum2]), 1e-12, what=f"{case} mode {mode} sums")
+            assert_rel(Tm, T0_, 1e-12, what=f"{case} mode {mode} T")
+            assert_rel(hm, h0_, 1e-12, scale=max(np.abs(h0_).max(), 1e-300), what=f"{case} mode {mode} histograms")
     gpu.specialize(True)
     # iterate with training through the specialised kernels
     st1, T1, h1, n1 = gpu.iterate(uni, 5000, a0=0, train=True)
