@@ -1083,6 +1083,8 @@ void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals
         case 6: { constexpr int NT = 6; CALL; } break;  \
         case 7: { constexpr int NT = 7; CALL; } break;  \
         case 8: { constexpr int NT = 8; CALL; } break;  \
+        case 9: { constexpr int NT = 9; CALL; } break;  \
+        case 10: { constexpr int NT = 10; CALL; } break; \
         default: throw chili_error(CHILI_EINVAL, "unsupported number of external particles"); \
     }
 
@@ -2022,7 +2024,7 @@ int chili_mc_create(const chili_channels* ch, int nbins, const chili_integrand* 
         if (!ch || !f || !out) throw chili_error(CHILI_EINVAL, "null argument");
         if (nbins < 2 || nbins > 256) throw chili_error(CHILI_ERANGE, "nbins must be in [2, 256]");
         if (ch->set.topo.empty()) throw chili_error(CHILI_EINVAL, "process has no channels");
-        if (ch->set.ntot < 4 || ch->set.ntot > 8) throw chili_error(CHILI_ERANGE, "device path supports 2->2 .. 2->6");
+        if (ch->set.ntot < 4 || ch->set.ntot > CHILI_MAX_FLAV) throw chili_error(CHILI_ERANGE, "device path supports 2->2 .. 2->8");
         if (ch->set.topo.size() > 4096) throw chili_error(CHILI_ERANGE, "at most 4096 channels");
         if (f->cuts < CHILI_CUTS_NONE || f->cuts > CHILI_CUTS_KT_JETS) throw chili_error(CHILI_EINVAL, "unknown cut kind");
         if (f->cuts == CHILI_CUTS_KT_JETS && !(f->drmin > 0)) throw chili_error(CHILI_EINVAL, "kT jets need a radius drmin > 0");
@@ -2442,7 +2444,7 @@ int chili_batch_create(int njobs, chili_mc* const* jobs, chili_batch** out) {
         HIP_CHECK(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device));
         b->acc_est.assign(njobs, 0.3);
         b->job_group.assign(njobs, -1);
-        for (int nt = 4; nt <= 8; ++nt) {
+        for (int nt = 4; nt <= CHILI_MAX_FLAV; ++nt) {
             BatchGroup g;
             g.NT = nt;
             g.D = 3 * (nt - 2) - 2;

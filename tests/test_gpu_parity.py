@@ -43,6 +43,12 @@ CASES = {
     # the reference example's PreProcess: every parton a kT jet (R = 0.4)
     "gg_4g_kt": lambda: (P.gluon_process(4, smax=3), P.integrand_propprod(cuts=("kt", 0.4, 900.0))),
     "tt_6j_kt": lambda: (P.tt_process(smax=5), P.integrand_resonant(cuts=("kt", 0.4))),
+    # 2->7 and 2->8 (19 and 22 VEGAS dimensions, 22 and 29 channels at
+    # smax = 1) at 14 TeV with looser cuts, so that a good share passes
+    "gg_7g": lambda: (P.gluon_process(7, sqrts=14000.0, smax=1, cuts=dict(ptmin=10.0, etamax=4.0, deltaR=0.2)),
+                      P.integrand_propprod(cuts=(10.0, 4.0, 0.2))),
+    "gg_8g": lambda: (P.gluon_process(8, sqrts=14000.0, smax=1, cuts=dict(ptmin=10.0, etamax=4.0, deltaR=0.2)),
+                      P.integrand_unit((10.0, 4.0, 0.2))),
 }
 
 
@@ -63,7 +69,15 @@ def compare_forward(g, o, what):
     assert_rel(g["mom"][phys], o["mom"][phys], 1e-12, scale=sc[phys], what=f"{what} momenta")
 
 
-def compare_weights(g, o, what):
+# Cases whose inverse maps have ill-conditioned points: there the phase-space
+# weight itself moves by ~2e-11 when the momenta move by one ulp (shown on the
+# oracle alone, 2->7 at a point with weights ~1e27), so the last bits of the
+# device's and glibc's libm calls inside the inverse map can reach it.  Those
+# cases are held to 1e-10, and to 1e-12 on at least 99% of the entries.
+CONDITIONED = {"gg_7g", "gg_8g"}
+
+
+def compare_weights(g, o, what, case=None):
     """Stage 2: the same momenta -> every channel's inverse map, VEGAS bins,
     multichannel weight, integrand."""
     n = len(o["pick"])
@@ -72,10 +86,13 @@ def compare_weights(g, o, what):
     live = ok & (o["ps"][np.arange(n), o["pick"]] > 0)
     assert np.array_equal(g["bins"][live], o["bins"][live]), f"{what}: FindBin indices differ"
     assert_rel(g["xpick"][live], o["xpick"][live], 1e-12, scale=1.0, what=f"{what} inverse point")
-    assert_rel(g["ps"][ok], o["ps"][ok], 1e-12, what=f"{what} phase-space weights")
-    assert_rel(g["w"][ok], o["w"][ok], 1e-12, what=f"{what} multichannel weight")
-    assert_rel(g["val"][ok], o["val"][ok], 1e-12, what=f"{what} values")
-    assert_rel(g["f"][ok], o["f"][ok], 1e-12, what=f"{what} integrand")
+    rtol = 1e-10 if case in CONDITIONED else 1e-12
+    for key, label in (("ps", "phase-space weights"), ("w", "multichannel weight"), ("val", "values"), ("f", "integrand")):
+        assert_rel(g[key][ok], o[key][ok], rtol, what=f"{what} {label}")
+        if rtol > 1e-12:
+            a, b = g[key][ok], o[key][ok]
+            strict = np.abs(a - b) <= 1e-12 * np.maximum(np.abs(a), np.abs(b))
+            assert strict.mean() >= 0.99, f"{what} {label}: {strict.mean():.4f} within 1e-12"
 
 
 def end_to_end_spread(g, o):
@@ -109,7 +126,7 @@ def test_points_parity(case, grids):
     gg = gpu.points(uni, 17, n)
     compare_forward(gg, og, f"{case}/{grids}")
     staged = ora.points(uni, 17, n, mom_in=gg["mom"])
-    compare_weights(gg, staged, f"{case}/{grids}")
+    compare_weights(gg, staged, f"{case}/{grids}", case)
     d = end_to_end_spread(gg, og)
     print(f"{case}/{grids}: C={gpu.nchannels} accepted {np.sum(og['status'] == 1)}/{n}; end-to-end rel dW "
           f"median {np.median(d):.1e} p99 {np.quantile(d, 0.99):.1e} max {d.max():.1e}")
@@ -191,7 +208,7 @@ def test_points_host_uniforms():
     compare_weights(gg, ora.points(uni, 0, n, mom_in=gg["mom"]), "host uniforms")
 
 
-@pytest.mark.parametrize("case", ["gg_2g", "gg_4g", "gg_5g", "tt_6j", "gg_4g_kt"])
+@pytest.mark.parametrize("case", ["gg_2g", "gg_4g", "gg_5g", "tt_6j", "gg_4g_kt", "gg_7g", "gg_8g"])
 def test_iterate_parity(case):
     """Stage 3: the retry loop and the accumulations (StatsData, alpha
     training, per-channel f^2 histograms), the oracle replaying the GPU's
