@@ -116,6 +116,17 @@ def measured_traffic(workload, variant, pts_launch):
     return round(t["bytes_per_point"] * pts_launch, 1), t["bytes_per_point"]
 
 
+def fixture_f_main(workload):
+    """F_main of the committed op-count fixture (tests/golden/opcounts.json:
+    uniform grids, alpha = 1/C), for reference next to the run's own count."""
+    path = os.path.join(ROOT, "tests", "golden", "opcounts.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)["workloads"][workload]["F_main_per_point"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _cpu_worker(job):
     """One host core: the oracle's MultiChannel loop over a share of the sample."""
     proc_name, grids, alpha, seed, stream, ncalls = job
@@ -197,9 +208,28 @@ def leg_vegas_eemumu():
                 note="a 1e5-point iteration is launch-latency bound (two kernels)")
 
 
-def leg_multichannel(name, steps=5):
+def timed_steps(step, n, steps, seed, vals):
+    """`steps` production steps of n accepted points (values into vals),
+    after one warm-up step: (seconds, attempts, <w>, w_max)."""
+    import torch
+    from chili_amd import abi
+    step(abi.philox_uniforms(seed, stream=1000), None, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    atts = 0
+    for s in range(steps):
+        atts += step(abi.philox_uniforms(seed, stream=S_PROD + s), ctypes.c_void_p(vals.data_ptr() + 8 * n * s), n)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    w = vals[: n * steps]
+    return dt, atts, float(w.mean()), float(w.max())
+
+
+def leg_multichannel(name, steps=5, flow_epochs=0):
     """A workload's plain leg, short: adaptation through chili_mc_optimize,
-    then `steps` timed production iterations (values in HBM)."""
+    then `steps` timed production iterations (values in HBM); with
+    flow_epochs > 0 also its flow-assisted leg (flows trained for that many
+    epochs on 2^20 method-generated points, mixture 0, moment-16 loss)."""
     import torch
     from chili_amd import abi, lib
     W = workloads()[name]
@@ -208,25 +238,43 @@ def leg_multichannel(name, steps=5):
     ad = adapt(gpu, W, seed)
     n = W["batch"]
     vals = torch.empty(n * steps, dtype=torch.float64, device="cuda:0")
-    gpu.iterate(abi.philox_uniforms(seed, stream=1000), n, train=False)
-    torch.cuda.synchronize()
+
+    def plain(uni, vp, cap):
+        return gpu.iterate(uni, n, train=False, vals_ptr=vp, vals_cap=cap)[3]
+
     gpu.timing(enable=True, reset=True)
-    t0 = time.perf_counter()
-    atts = 0
-    for s in range(steps):
-        _, _, _, na = gpu.iterate(abi.philox_uniforms(seed, stream=S_PROD + s), n, train=False,
-                                  vals_ptr=ctypes.c_void_p(vals.data_ptr() + 8 * n * s), vals_cap=n)
-        atts += na
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt, atts, wmean, wmax = timed_steps(plain, n, steps, seed, vals)
     tm = gpu.timing(enable=False)
-    w = vals
-    wmean, wmax = float(w.mean()), float(w.max())
-    return dict(workload=W["desc"], channels=gpu.nchannels, dims=gpu.ndims, adaptation_s=round(ad["time_adapt_s"], 3),
-                time_to_1pct_s=ad["time_to_1pct_s"], points_per_s=round(n * steps / dt, 1), ms_per_step=round(dt / steps * 1e3, 3),
-                eps_unw=wmean / wmax, unweighted_per_s=round(n * steps / dt * wmean / wmax, 1), acceptance=n * steps / atts,
-                k_main_avg_ms=round(tm["main_ms"] / max(tm["launches"], 1), 4),
-                k_main_points_per_launch=round(tm["points"] / max(tm["launches"], 1), 1))
+    out = dict(workload=W["desc"], channels=gpu.nchannels, dims=gpu.ndims, adaptation_s=round(ad["time_adapt_s"], 3),
+               time_to_1pct_s=ad["time_to_1pct_s"], points_per_s=round(n * steps / dt, 1), ms_per_step=round(dt / steps * 1e3, 3),
+               eps_unw=wmean / wmax, unweighted_per_s=round(n * steps / dt * wmean / wmax, 1), acceptance=n * steps / atts,
+               k_main_avg_ms=round(tm["main_ms"] / max(tm["launches"], 1), 4),
+               k_main_points_per_launch=round(tm["points"] / max(tm["launches"], 1), 1))
+    if flow_epochs > 0:
+        C, d, cap = gpu.nchannels, gpu.ndims, 1 << 20
+        r = torch.empty(C * cap * d, dtype=torch.float64, device="cuda:0")
+        beta = torch.empty(C * cap, dtype=torch.float64, device="cuda:0")
+        w2 = torch.empty(cap, dtype=torch.float64, device="cuda:0")
+        fl = lib.FlowGPU(d, W["flow_layers"], C, seed=seed)
+        fl.set_moment(16)
+        t0 = time.perf_counter()
+        _, nk, _ = gpu.flow_data(None, abi.philox_uniforms(seed, stream=500), cap, r.data_ptr(), beta.data_ptr(), w2.data_ptr(), cap)
+        losses = fl.train(nk, r.data_ptr(), beta.data_ptr(), w2.data_ptr(), 1 << 16, flow_epochs, 1e-3, seed=seed)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        del r, beta, w2
+
+        def flow(uni, vp, cap_):
+            return gpu.flow_iterate(fl, uni, n, vals_ptr=vp, vals_cap=cap_)[1]
+
+        dt, atts, wmean, wmax = timed_steps(flow, n, steps, seed, vals)
+        out["flow"] = dict(layers=W["flow_layers"], epochs=flow_epochs, train_points=int(nk), train_s=round(t_train, 2),
+                           loss_first=losses[0], loss_last=losses[-1], points_per_s=round(n * steps / dt, 1),
+                           ms_per_step=round(dt / steps * 1e3, 3), eps_unw=wmean / wmax,
+                           unweighted_per_s=round(n * steps / dt * wmean / wmax, 1), acceptance=n * steps / atts,
+                           note="short recipe (config: 200 epochs on 2^22 points, weighted-variance loss)")
+        out["flow"]["vs_plain_unweighted"] = round(out["flow"]["unweighted_per_s"] / out["unweighted_per_s"], 4)
+    return out
 
 
 def leg_batched(iters=5, adapt_iters=3):
@@ -636,16 +684,18 @@ def main():
                 kernel="k_main (all-channel inverse maps + VEGAS densities + multichannel weight)",
                 compute="FP64 vector ALU (no matrix work); MI355X's FP64 vector peak 78.6 TFLOP/s",
                 scope="the multichannel weight kernel of the plain leg; the flow leg's coupling kernel is roofline_flow",
-                flop_eq_per_point=round(f_main, 1), flop_eq_count="N_arith + 20 N_trans, op-counting oracle, 4000 attempts",
+                flop_eq_per_point=round(f_main, 1),
+                flop_eq_count="N_arith + 20 N_trans, op-counting oracle on 4000 attempts of the frozen state",
+                flop_eq_fixture=fixture_f_main(args.workload),
                 points_per_launch=round(pts_launch, 1), avg_launch_ms=round(avg_ms, 4), launches=tm["launches"],
                 forward_kernel_ms_per_step=round(tm["forward_ms"] / (args.steps + args.warmup), 4))
 
     # ---- short legs of the other workloads ----
     configs = {}
     if not args.no_configs:
-        for key, fn in (("ee_mumu", leg_vegas_eemumu), ("gg_4g_plain", lambda: leg_multichannel("gg_4g")),
+        for key, fn in (("ee_mumu", leg_vegas_eemumu), ("gg_4g", lambda: leg_multichannel("gg_4g", flow_epochs=6)),
                         ("batched_32_jobs", leg_batched)):
-            if key == "gg_4g_plain" and args.workload == "gg_4g":
+            if key == "gg_4g" and args.workload == "gg_4g":
                 continue
             try:
                 configs[key] = fn()
