@@ -225,6 +225,43 @@ def timed_steps(step, n, steps, seed, vals):
     return dt, atts, float(w.mean()), float(w.max())
 
 
+VARIANTS = {0: "generic", 1: "specialized-forward", 2: "specialized-weight", 3: "specialized", 4: "split-weight",
+            5: "specialized-forward+split-weight"}
+
+
+def choose_variant(gpu, step, seed, modes=(1, 2, 3, 4, 5), log_fn=None):
+    """Times one production step (a warm-up, then the fastest of three) with
+    the generic kernels and with each run-time specialised mode
+    (chili_mc_specialize bits: 1 forward, 2 fused weight, 4 split weight),
+    leaves the fastest selected and returns (name, compile seconds, times)."""
+    import torch
+    from chili_amd import abi, lib
+
+    def one_step(stream):
+        step(abi.philox_uniforms(seed, stream=stream), None, 0)
+        best = None
+        for k in range(3):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            step(abi.philox_uniforms(seed, stream=stream + 1 + k), None, 0)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        return best
+
+    times, compile_s = {"generic": one_step(900)}, {}
+    try:
+        for mode in modes:
+            compile_s[VARIANTS[mode]] = gpu.specialize(mode)
+            times[VARIANTS[mode]] = one_step(900 + 10 * mode)
+    except lib.ChiliError as e:
+        if log_fn:
+            log_fn(f"  specialisation unavailable: {e}")
+    variant = min(times, key=times.get)
+    gpu.specialize({v: k for k, v in VARIANTS.items()}[variant])
+    return variant, compile_s.get(variant), times
+
+
 def leg_multichannel(name, steps=5, flow_epochs=0):
     """A workload's plain leg, short: adaptation through chili_mc_optimize,
     then `steps` timed production iterations (values in HBM); with
@@ -242,12 +279,17 @@ def leg_multichannel(name, steps=5, flow_epochs=0):
     def plain(uni, vp, cap):
         return gpu.iterate(uni, n, train=False, vals_ptr=vp, vals_cap=cap)[3]
 
+    # the fused specialised weight kernel spills beyond a few channels: the
+    # generic kernels, the specialised forward kernel and the split pass
+    variant, compile_s, t_var = choose_variant(gpu, plain, seed, modes=(1, 4, 5))
     gpu.timing(enable=True, reset=True)
     dt, atts, wmean, wmax = timed_steps(plain, n, steps, seed, vals)
     tm = gpu.timing(enable=False)
     out = dict(workload=W["desc"], channels=gpu.nchannels, dims=gpu.ndims, adaptation_s=round(ad["time_adapt_s"], 3),
                time_to_1pct_s=ad["time_to_1pct_s"], points_per_s=round(n * steps / dt, 1), ms_per_step=round(dt / steps * 1e3, 3),
                eps_unw=wmean / wmax, unweighted_per_s=round(n * steps / dt * wmean / wmax, 1), acceptance=n * steps / atts,
+               kernel_variant=variant, variant_step_ms={k: round(v * 1e3, 3) for k, v in t_var.items()},
+               specialize_compile_s=compile_s,
                k_main_avg_ms=round(tm["main_ms"] / max(tm["launches"], 1), 4),
                k_main_points_per_launch=round(tm["points"] / max(tm["launches"], 1), 1))
     if flow_epochs > 0:
@@ -494,30 +536,7 @@ def main():
     # time (programs as constants); the faster production step wins
     variant, compile_s, t_variant = "generic", None, {}
     if not args.no_specialize:
-        def one_step(stream):
-            # one warm-up step, then the fastest of three
-            plain_step(abi.philox_uniforms(seed, stream=stream), None, 0)
-            best = None
-            for k in range(3):
-                torch.cuda.synchronize()
-                t = time.perf_counter()
-                plain_step(abi.philox_uniforms(seed, stream=stream + 1 + k), None, 0)
-                torch.cuda.synchronize()
-                dt = time.perf_counter() - t
-                best = dt if best is None else min(best, dt)
-            return best
-        t_variant["generic"] = one_step(900)
-        names = {1: "specialized-forward", 2: "specialized-weight", 3: "specialized", 4: "split-weight",
-                 5: "specialized-forward+split-weight"}
-        try:
-            for mode in (1, 2, 3, 4, 5):
-                compile_s = gpu.specialize(mode)
-                t_variant[names[mode]] = one_step(900 + 10 * mode)
-            variant = min(t_variant, key=t_variant.get)
-            gpu.specialize({v: k for k, v in names.items()}.get(variant, 0))
-        except lib.ChiliError as e:
-            gpu.specialize(False)
-            log(f"  specialisation unavailable: {e}")
+        variant, compile_s, t_variant = choose_variant(gpu, plain_step, seed, log_fn=log)
         log(f"  kernel variant: {variant} (step s: {t_variant}, compile {compile_s} s)")
 
     plain_sel = production(plain_step, "plain, selection sample", stream0=S_SEL, steps=args.sel_steps, warmup=0)

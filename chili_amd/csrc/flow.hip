@@ -869,11 +869,13 @@ int chili_flow_destroy(chili_flow* f) {
         if (!f) return;
         select_device(f->eng.device);
         hipStream_t s = f->eng.stream;
-        if (s) hipStreamSynchronize(s);
+        // teardown is best effort: a sticky error of earlier work must not
+        // keep the context from being freed
+        if (s) (void)hipStreamSynchronize(s);
         f->eng.release_lanes();
         f->eng.release_timing();
         delete f;
-        if (s) hipStreamDestroy(s);
+        if (s) (void)hipStreamDestroy(s);
     });
 }
 

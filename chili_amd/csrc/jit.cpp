@@ -108,8 +108,12 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
              "    }\n"
              "    const int64_t e = (int64_t)blockIdx.x * kTile + threadIdx.x;\n"
              "    if (e >= kept) return;\n"
-             "    inverse_one<NT>(A, chili_prog_" << j << "(), " << j << ", e, seg2, comp, mom, nmax, pv, bins, sorted, R,\n"
-             "                    lds_search ? s_srch : nullptr);\n"
+             "    // two inlined copies, so that the LDS table's address space is known\n"
+             "    // (ds_read) rather than resolved at run time (flat loads)\n"
+             "    if (lds_search)\n"
+             "        inverse_one<NT, true>(A, chili_prog_" << j << "(), " << j << ", e, seg2, comp, mom, nmax, pv, bins, sorted, R, s_srch);\n"
+             "    else\n"
+             "        inverse_one<NT, false>(A, chili_prog_" << j << "(), " << j << ", e, seg2, comp, mom, nmax, pv, bins, sorted, R, nullptr);\n"
              "}\n\n";
     s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward(MCArgs A, const int* __restrict__ seg_start,\n"
          "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"

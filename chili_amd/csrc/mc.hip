@@ -801,7 +801,7 @@ struct chili_mc {
     std::unique_ptr<chili_jit::Module> jit;
     bool jit_fwd = false, jit_main = false;
     bool jit_split = false;      // split weight pass: per-channel specialised inverse kernels + combine
-    DevBuf<double> d_pre;        // split pass: ps and VEGAS weights [2 C][kept]
+    DevBuf<double> d_pre;        // split pass: ps and VEGAS weights [2 C][kept], then f [kept]
     DevBuf<short> d_prebins;     // split pass: the picked channel's bins [d][kept]
     bool lds_search = true;      // weight kernel: search tables in LDS when they fit
     double gen_rate = 0.01;      // unweighting stage: kept events per attempt (running estimate)
@@ -1011,7 +1011,7 @@ void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals
         // split weight pass: every channel's inverse in its own specialised
         // kernel (ps, VEGAS weight, the picked channel's bins by kept index),
         // then the combination (W, value, statistics, training sums)
-        m->d_pre.resize((size_t)2 * m->C * kept);
+        m->d_pre.resize((size_t)(2 * m->C + 1) * kept);
         m->d_prebins.resize((size_t)D * kept);
         MCArgs a = A;
         a.pre_pv = m->d_pre.ptr;
@@ -2066,10 +2066,12 @@ int chili_mc_destroy(chili_mc* m) {
     return guarded([&] {
         if (!m) return;
         select_device(m->device);
-        if (m->stream) hipStreamSynchronize(m->stream);
+        // teardown is best effort: a sticky error of earlier work must not
+        // keep the context from being freed
+        if (m->stream) (void)hipStreamSynchronize(m->stream);
         hipStream_t s = m->stream;
         delete m;
-        if (s) hipStreamDestroy(s);
+        if (s) (void)hipStreamDestroy(s);
     });
 }
 

@@ -163,17 +163,24 @@ __device__ __forceinline__ void main_tile(const MCArgs& A, int ch, int e0, int e
         bool isnanv = false;
         uint32_t att = 0;
         if (valid) {
-            const uint32_t sp = comp[e];
-            if (R.status) att = sorted[sp];
+            if constexpr (Loop::kPre) {
+                // split pass: the first channel's inverse kernel, which holds
+                // the point's momenta, left f; no momentum is read here
+                if (R.status) att = sorted[comp[e]];
+                fv = A.pre_pv[(int64_t)2 * C * A.pre_stride + e];
+            } else {
+                const uint32_t sp = comp[e];
+                if (R.status) att = sorted[sp];
 #pragma unroll
-            for (int i = 0; i < NT; ++i) {
-                P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
-                P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
-                P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
-                P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+                for (int i = 0; i < NT; ++i) {
+                    P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
+                    P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
+                    P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
+                    P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+                }
+                fv = A.fhost ? A.fhost[sp] : integrand<NT>(A.f, P);
             }
-            fv = A.fhost ? A.fhost[sp] : integrand<NT>(A.f, P);
-        } else {
+        } else if constexpr (!Loop::kPre) {
 #pragma unroll
             for (int i = 0; i < NT; ++i) P.set(i, mk(1., 0., 0., 1.));
         }
@@ -305,9 +312,10 @@ struct PreLoop {
 // One kept point e of the split weight pass for channel j (program p): its
 // inverse map and VEGAS weight (FSMapper::GenerateWeight, Channel.cc:40-61;
 // AdaptiveMap::GenerateWeight, AdaptiveMap.cc:58-65), and, when j is the
-// point's picked channel, its bins; search: this channel's table in LDS or
-// null (device memory).
-template <int NT>
+// point's picked channel, its bins; channel 0 also evaluates the integrand
+// (row 2 C of pv), so that the combination kernel reads no momenta; kLds:
+// search is this channel's table in LDS, else null (device memory).
+template <int NT, bool kLds>
 __device__ __forceinline__ void inverse_one(const MCArgs& A, const ChanProg& p, int j, int64_t e, const int* __restrict__ seg2,
                                             const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax,
                                             double* __restrict__ pv, short* __restrict__ bins, const uint32_t* __restrict__ sorted,
@@ -321,14 +329,16 @@ __device__ __forceinline__ void inverse_one(const MCArgs& A, const ChanProg& p, 
         P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
         P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
     }
+    if (j == 0) pv[(int64_t)2 * A.C * A.pre_stride + e] = A.fhost ? A.fhost[sp] : integrand<NT>(A.f, P);
     int pc = 0;                         // picked channel: the kept segment holding e
     for (int c = 1; c < A.C; ++c) pc = (seg2[c] <= e) ? c : pc;
     const bool own = pc == j;
     const uint32_t att = R.status ? sorted[sp] : 0;
     const bool recx = R.status && own;
     double vw;
-    const double ps = generate_weight<NT>(A, p, j, P, &vw, own ? bins + e : nullptr, (int)A.pre_stride, recx ? R.x + att : nullptr,
-                                          recx ? R.bins + att : nullptr, R.n, nullptr, search, search != nullptr);
+    const double ps = generate_weight<NT, kLds>(A, p, j, P, &vw, own ? bins + e : nullptr, (int)A.pre_stride,
+                                                recx ? R.x + att : nullptr, recx ? R.bins + att : nullptr, R.n, nullptr,
+                                                kLds ? search : nullptr, kLds);
     pv[(int64_t)j * A.pre_stride + e] = ps;
     pv[(int64_t)(A.C + j) * A.pre_stride + e] = vw;
     if (R.status) R.ps[(int64_t)j * R.n + att] = ps;

@@ -245,7 +245,7 @@ CHILI_DEV void generate_point(const MCArgs& A, const ChanProg& p, const double* 
 // Returns the phase-space weight; *vw receives the VEGAS weight; for the
 // picked channel the bins of its inverse point go to `bins_out` (LDS).
 // --------------------------------------------------------------------------
-template <int NT>
+template <int NT, bool kOneLds = false>
 CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, MomRF<2 * NT>& P, double* vw_out,
                                  short* bins_out, int bstride, double* x_out = nullptr, int32_t* bin_rec = nullptr,
                                  int64_t xstride = 0, double* r_out = nullptr, const double* search = nullptr,
@@ -256,11 +256,14 @@ CHILI_DEV double generate_weight(const MCArgs& A, const ChanProg& p, int ch, Mom
     const double Bd = (double)A.B;
     const double* __restrict__ edges = A.edges + (int64_t)ch * A.d * (A.B + 1);
     // search tables: device memory, a workgroup's LDS copy of every channel's,
-    // or (search_one) an LDS copy of this channel's alone
-    const double* __restrict__ srch = (search ? search : A.search) + (search_one ? 0 : (int64_t)ch * A.d * A.P2);
+    // or (search_one) an LDS copy of this channel's alone; kOneLds: the last,
+    // fixed at compile time, so that the pointer is known to address LDS and
+    // the search compiles to LDS reads rather than flat loads
+    const double* __restrict__ srch =
+        kOneLds ? search : (search ? search : A.search) + (search_one ? 0 : (int64_t)ch * A.d * A.P2);
     double vw = 1.0;
     int slot = 0;
-    const uint16_t* __restrict__ cells = (search || !A.cells) ? nullptr : A.cells + (int64_t)ch * A.d * (kCells + 1);
+    const uint16_t* __restrict__ cells = (kOneLds || search || !A.cells) ? nullptr : A.cells + (int64_t)ch * A.d * (kCells + 1);
     auto take = [&](double x) {
         int b = cells ? find_bin_cells(srch + slot * A.P2, cells + slot * (kCells + 1), A.P2, x) : find_bin(srch + slot * A.P2, A.P2, x);
         const double* e = edges + slot * (A.B + 1);

@@ -64,6 +64,14 @@ def test_host_integrand_matches_device(case):
     assert_rel(b_[1:5], a_[1:5], 1e-12, what="stats")
     assert_rel(Th, Td, 1e-12, what="T")
     assert_rel(hh, hd, 1e-12, scale=np.abs(hd).max(), what="histograms")
+    # the split weight pass (its first inverse kernel hands f on) with the
+    # host callback: the same values
+    host.specialize(5)
+    ps5 = host.points(uni, 0, n)
+    assert np.array_equal(pd["status"], ps5["status"])
+    assert_rel(ps5["f"][ok], pd["f"][ok], 1e-12, what="integrand, split pass")
+    assert_rel(ps5["val"][ok], pd["val"][ok], 1e-12, what="values, split pass")
+    host.specialize(False)
     # back to the device integrand
     host.set_host_integrand(None)
     s2, _, _, n2 = host.iterate(uni, 5000, train=False)

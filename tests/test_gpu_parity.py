@@ -171,6 +171,8 @@ def test_specialized_kernels_parity(case):
         mixed = gpu.points(uni, 3, n)
         assert np.array_equal(generic["status"], mixed["status"]), f"{case} mode {mode} status"
         assert_rel(mixed["w"][ok], generic["w"][ok], 1e-12, what=f"{case} mode {mode} weights")
+        assert_rel(mixed["f"][ok], generic["f"][ok], 1e-12, what=f"{case} mode {mode} integrand")
+        assert_rel(mixed["val"][ok], generic["val"][ok], 1e-12, what=f"{case} mode {mode} values")
         assert_rel(mixed["ps"][ok], generic["ps"][ok], 1e-12, what=f"{case} mode {mode} phase-space weights")
         assert np.array_equal(mixed["bins"][ok], generic["bins"][ok]), f"{case} mode {mode} bins"
         assert_rel(mixed["xpick"][ok], generic["xpick"][ok], 1e-12, scale=1.0, what=f"{case} mode {mode} inverse points")
