@@ -316,8 +316,11 @@ __device__ __forceinline__ SplineBin spline_bin(const U& u, float shift, double 
     b.Y0 = (double)Y0;
     b.w = (double)X1 - b.X0;
     b.h = (double)Y1 - b.Y0;
-    b.d0 = (k == 0) ? 1.0 : kMinD + (double)softplusf(ud0 + shift);
-    b.d1 = (k == K - 1) ? 1.0 : kMinD + (double)softplusf(ud1 + shift);
+    // both softplus values are computed on every lane and then selected, so
+    // that the end bins (fixed slope 1) add no branch to the caller's code
+    const float sp0 = softplusf(ud0 + shift), sp1 = softplusf(ud1 + shift);
+    b.d0 = (k == 0) ? 1.0 : kMinD + (double)sp0;
+    b.d1 = (k == K - 1) ? 1.0 : kMinD + (double)sp1;
     return b;
 }
 
@@ -547,6 +550,14 @@ __device__ __forceinline__ void relu_pack(const f32x16& a0, const f32x16& a1, bf
 
 // NP = pairs of transformed dimensions (ceil(nB / 2)), a compile-time
 // constant so that the per-lane operands of a tile live in registers.
+//
+// Software pipeline: the matrix work of the next unit -- the output rows of
+// pair P + 1, or after the last pair the next tile's hidden layers and the
+// output rows of its pair 0 -- is issued in the same scheduling region as the
+// spline of pair P, which does not depend on it, so a wave keeps the matrix
+// core and the vector ALU busy at once.  The spline runs on every lane
+// (lanes past the chunk or without a transformed dimension compute on zero
+// inputs and keep nothing), so that no branch splits the region.
 template <int K, bool INV, int NP>
 __global__ __launch_bounds__(512) void k_couple(CoupleArgs a) {
     extern __shared__ __align__(16) unsigned char s_raw[];
@@ -578,6 +589,7 @@ __global__ __launch_bounds__(512) void k_couple(CoupleArgs a) {
     __syncthreads();
     const int64_t start = a.start[lo] + c0;
     const int64_t ntile = (cnt + 31) / 32;
+    if (wave >= ntile) return;
     int aidx[8];                         // conditioning dims this lane feeds to layer 1
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -587,84 +599,106 @@ __global__ __launch_bounds__(512) void k_couple(CoupleArgs a) {
     int bidx[NP];                        // transformed dim of this lane in pair P (or -1)
 #pragma unroll
     for (int P = 0; P < NP; ++P) bidx[P] = (2 * P + h < a.nB) ? a.B[2 * P + h] : -1;
-    // inputs of a tile: conditioning values, transformed values, running log q
-    double nxa[8], nxb[NP], nlq = 0.0;
+    // raw inputs of the next tile: conditioning values, transformed values,
+    // running log q.  The loads are unconditional -- a row past the chunk
+    // reads the chunk's first row, a missing dimension column 0 -- and the
+    // values are masked where they are used (npv: the row exists), so that no
+    // branch splits the loop body, a fetch past the last tile is safe, and the
+    // loads stay in flight until their first use.
+    double rxa[8], rxb[NP], rlq;
+    bool npv;
     auto fetch = [&](int64_t t) {
-        const int64_t row = start + t * 32 + pl;
-        const bool pv = t * 32 + pl < cnt;
+        npv = t * 32 + pl < cnt;
+        const int64_t row = npv ? start + t * 32 + pl : start;
         const double* xp = a.x + row * a.d;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) nxa[j] = (pv && aidx[j] >= 0) ? xp[aidx[j]] : 0.0;
+        for (int j = 0; j < 8; ++j) rxa[j] = xp[aidx[j] >= 0 ? aidx[j] : 0];
 #pragma unroll
-        for (int P = 0; P < NP; ++P) nxb[P] = (pv && bidx[P] >= 0) ? xp[bidx[P]] : 0.0;
-        nlq = (pv && h == 0) ? a.lq[row] : 0.0;
+        for (int P = 0; P < NP; ++P) rxb[P] = xp[bidx[P] >= 0 ? bidx[P] : 0];
+        rlq = a.lq[row];
     };
-    for (int64_t t = wave; t < ntile; t += nw) {
-        const int64_t row = start + t * 32 + pl;
-        const bool pv = t * 32 + pl < cnt;
-        double* xp = a.x + row * a.d;
-        fetch(t);
-        double xb[NP];
-#pragma unroll
-        for (int P = 0; P < NP; ++P) xb[P] = nxb[P];
-        const double lq0 = nlq;
-        // layer-1 operand: lanes of half h hold conditioning features 8h..8h+7
+    // hidden layers of the fetched tile: lanes of half h feed conditioning
+    // features 8h..8h+7 to layer 1; hb receives the last hidden layer as the
+    // output layer's B fragments
+    auto hidden = [&](bf16x8* hb) {
         bf16x8 b0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) b0[j] = f2bf((float)nxa[j]);
-        f32x16 c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW1, lane), b0, bias_tile(sB + FL.ob1, h), 0, 0, 0);
-        f32x16 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW1 + 512, lane), b0, bias_tile(sB + FL.ob1 + 32, h),
+        for (int j = 0; j < 8; ++j) b0[j] = f2bf((npv & (aidx[j] >= 0)) ? (float)rxa[j] : 0.f);
+        f32x16 g0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW1, lane), b0, bias_tile(sB + FL.ob1, h), 0, 0, 0);
+        f32x16 g1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW1 + 512, lane), b0, bias_tile(sB + FL.ob1 + 32, h),
                                                             0, 0, 0);
-        bf16x8 hb[4];
-        relu_pack(c0, c1, hb);
+        relu_pack(g0, g1, hb);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const bf16* Wq = sW + (q == 0 ? FL.oW2 : FL.oW3);
             const float* bq = sB + (q == 0 ? FL.ob2 : FL.ob3);
-            c0 = bias_tile(bq, h);
-            c1 = bias_tile(bq + 32, h);
+            g0 = bias_tile(bq, h);
+            g1 = bias_tile(bq + 32, h);
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(Wq + (0 * 4 + s) * 512, lane), hb[s], c0, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(Wq + (1 * 4 + s) * 512, lane), hb[s], c1, 0, 0, 0);
+                g0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(Wq + (0 * 4 + s) * 512, lane), hb[s], g0, 0, 0, 0);
+                g1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(Wq + (1 * 4 + s) * 512, lane), hb[s], g1, 0, 0, 0);
             }
-            relu_pack(c0, c1, hb);
+            relu_pack(g0, g1, hb);
         }
-        // output layer and splines, one pair of transformed dimensions at a
-        // time: lane half h receives the 48 raw parameters of dimension
-        // 2P + h in c[q / 16][q % 16] (the swizzled W4 rows put them there),
-        // the accumulators starting at their biases
+    };
+    // output rows of pair P: lane half h receives the 48 raw parameters of
+    // dimension 2P + h in c[q / 16][q % 16] (the swizzled W4 rows put them
+    // there), the accumulators starting at their biases
+    auto outp = [&](int P, const bf16x8* hb, f32x16* c) {
+        const float4* bq = reinterpret_cast<const float4*>(sB + FL.ob4 + kPairRows * P + 48 * h);
+#pragma unroll
+        for (int tt = 0; tt < 3; ++tt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float4 b4 = bq[4 * tt + v];
+                c[tt][4 * v] = b4.x;
+                c[tt][4 * v + 1] = b4.y;
+                c[tt][4 * v + 2] = b4.z;
+                c[tt][4 * v + 3] = b4.w;
+            }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int tt = 0; tt < 3; ++tt)
+                c[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW4 + ((int64_t)(3 * P + tt) * 4 + s) * 512, lane),
+                                                                hb[s], c[tt], 0, 0, 0);
+    };
+    fetch(wave);
+    bf16x8 hb[4];
+    hidden(hb);
+    f32x16 cc[3];
+    outp(0, hb, cc);
+    for (int64_t t = wave; t < ntile; t += nw) {
+        const int64_t row = start + t * 32 + pl;
+        const bool pv = t * 32 + pl < cnt;
+        double* xp = a.x + row * a.d;
+        double xb[NP];
+#pragma unroll
+        for (int P = 0; P < NP; ++P) xb[P] = (pv & (bidx[P] >= 0)) ? rxb[P] : 0.0;
+        const double lq0 = (pv & (h == 0)) ? rlq : 0.0;
+        fetch(t + nw);                   // the next tile's inputs, in flight during this tile
         double jprod = 1.0;              // product of this lane's dy/dx (one log per tile)
 #pragma unroll
         for (int P = 0; P < NP; ++P) {
-            f32x16 c[3];
-            const float4* bq = reinterpret_cast<const float4*>(sB + FL.ob4 + kPairRows * P + 48 * h);
-#pragma unroll
-            for (int tt = 0; tt < 3; ++tt)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const float4 b4 = bq[4 * tt + v];
-                    c[tt][4 * v] = b4.x;
-                    c[tt][4 * v + 1] = b4.y;
-                    c[tt][4 * v + 2] = b4.z;
-                    c[tt][4 * v + 3] = b4.w;
-                }
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-#pragma unroll
-                for (int tt = 0; tt < 3; ++tt)
-                    c[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(sW + FL.oW4 + ((int64_t)(3 * P + tt) * 4 + s) * 512, lane),
-                                                                    hb[s], c[tt], 0, 0, 0);
-            if (pv && bidx[P] >= 0) {
-                float u[48];
-#pragma unroll
-                for (int q = 0; q < 48; ++q) u[q] = c[q >> 4][q & 15];
-                double jac;
-                xp[bidx[P]] = INV ? rqs_inverse_j<K>(u, a.shift, xb[P], jac) : rqs_forward_j<K>(u, a.shift, xb[P], jac);
-                jprod *= jac;
+            f32x16 cn[3];
+            if (P + 1 < NP) {
+                outp(P + 1, hb, cn);
+            } else {
+                hidden(hb);
+                outp(0, hb, cn);
             }
-            // one pair's parameters live at a time: keep the scheduler from
-            // hoisting the next pair's products above this spline
+            float u[48];
+#pragma unroll
+            for (int q = 0; q < 48; ++q) u[q] = cc[q >> 4][q & 15];
+            double jac;
+            const double y = INV ? rqs_inverse_j<K>(u, a.shift, xb[P], jac) : rqs_forward_j<K>(u, a.shift, xb[P], jac);
+            const bool keep = pv && bidx[P] >= 0;
+            if (keep) xp[bidx[P]] = y;
+            jprod *= keep ? jac : 1.0;
+#pragma unroll
+            for (int tt = 0; tt < 3; ++tt) cc[tt] = cn[tt];
+            // one step's operands live at a time
             __builtin_amdgcn_sched_barrier(0);
         }
         double ldsum = log(jprod);

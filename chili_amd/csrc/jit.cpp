@@ -115,19 +115,58 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
              "    else\n"
              "        inverse_one<NT, false>(A, chili_prog_" << j << "(), " << j << ", e, seg2, comp, mom, nmax, pv, bins, sorted, R, nullptr);\n"
              "}\n\n";
-    s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward(MCArgs A, const int* __restrict__ seg_start,\n"
-         "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"
-         "        uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {\n"
-         "    int ch, e0, e1;\n"
-         "    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;\n"
-         "    const int e = e0 + threadIdx.x;\n"
-         "    if (e >= e1) return;\n"
-         "    const uint32_t a = sorted[e];\n"
-         "    switch (ch) {\n";
+    // forward map: from the attempt's uniforms (chili_k_forward) or, on the
+    // learned-proposal path, from the proposal's hypercube points (_z)
+    for (int zs = 0; zs < 2; ++zs) {
+        s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward" << (zs ? "_z" : "")
+          << "(MCArgs A, const int* __restrict__ seg_start,\n"
+             "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"
+             "        uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {\n"
+             "    int ch, e0, e1;\n"
+             "    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;\n"
+             "    const int e = e0 + threadIdx.x;\n"
+             "    if (e >= e1) return;\n"
+             "    const uint32_t a = sorted[e];\n"
+             "    switch (ch) {\n";
+        for (size_t j = 0; j < progs.size(); ++j)
+            s << "        case " << j << ": forward_one<NT, " << (zs ? "true" : "false") << ">(A, chili_prog_" << j << "(), " << j
+              << ", e, a, A.a0 + a, mom, nmax, acc, R, zsrc); break;\n";
+        s << "        default: break;\n    }\n}\n\n";
+    }
+    // learned-proposal path: one kernel per channel writes r_j and a_j of
+    // every kept point (channel 0 also f), rows staged in LDS (pitch D + 1)
+    // and written out contiguously, as the generic k_geom does per channel
     for (size_t j = 0; j < progs.size(); ++j)
-        s << "        case " << j << ": forward_one<NT, false>(A, chili_prog_" << j << "(), " << j
-          << ", e, a, A.a0 + a, mom, nmax, acc, R, zsrc); break;\n";
-    s << "        default: break;\n    }\n}\n";
+        s << "extern \"C\" __global__ __launch_bounds__(kTile, CHILI_INV_WAVES) void chili_k_geom_" << j
+          << "(MCArgs A, int64_t kept, const uint32_t* __restrict__ comp,\n"
+             "        const double* __restrict__ mom, int64_t nmax, double* __restrict__ rall, double* __restrict__ abuf,\n"
+             "        double* __restrict__ fbuf, int64_t stride, int lds_search) {\n"
+             "    constexpr int D = 3 * (NT - 2) - 2;\n"
+             "    constexpr int PR = D + 1;\n"
+             "    extern __shared__ double s_mem[];   // rows [kTile][PR], then this channel's search table [D][P2]\n"
+             "    double* s_r = s_mem;\n"
+             "    double* s_srch = s_mem + kTile * PR;\n"
+             "    if (lds_search) {\n"
+             "        const double* g = A.search + (int64_t)" << j << " * A.d * A.P2;\n"
+             "        for (int i = threadIdx.x; i < A.d * A.P2; i += kTile) s_srch[i] = g[i];\n"
+             "        __syncthreads();\n"
+             "    }\n"
+             "    const int64_t k0 = (int64_t)blockIdx.x * kTile, k = k0 + threadIdx.x;\n"
+             "    const int nrow = (int)(kept - k0 < kTile ? kept - k0 : kTile);\n"
+             "    double* row = s_r + threadIdx.x * PR;\n"
+             "    if (k < kept) {\n"
+             "        MomRF<2 * NT> P;\n"
+             "        load_mom<NT>(mom, nmax, comp[k], P);\n"
+          << (j == 0 ? "        fbuf[k] = integrand<NT>(A.f, P);\n" : "")
+          << "        if (lds_search)\n"
+             "            geom_one<NT, true>(A, chili_prog_" << j << "(), " << j << ", k, P, row, abuf, stride, s_srch);\n"
+             "        else\n"
+             "            geom_one<NT, false>(A, chili_prog_" << j << "(), " << j << ", k, P, row, abuf, stride, nullptr);\n"
+             "    }\n"
+             "    __syncthreads();\n"
+             "    double* dst = rall + ((int64_t)" << j << " * stride + k0) * D;\n"
+             "    for (int e = threadIdx.x; e < nrow * D; e += kTile) dst[e] = s_r[(e / D) * PR + e % D];\n"
+             "}\n\n";
     return s.str();
 }
 
@@ -167,9 +206,13 @@ void compile(const std::string& src, Module& out, double* seconds, bool load, in
     HIP_CHECK(hipModuleGetFunction(&out.main, out.mod, "chili_k_main"));
     HIP_CHECK(hipModuleGetFunction(&out.main_lds, out.mod, "chili_k_main_lds"));
     HIP_CHECK(hipModuleGetFunction(&out.forward, out.mod, "chili_k_forward"));
+    HIP_CHECK(hipModuleGetFunction(&out.forward_z, out.mod, "chili_k_forward_z"));
     out.inv.assign((size_t)nchannels, nullptr);
-    for (int j = 0; j < nchannels; ++j)
+    out.geom.assign((size_t)nchannels, nullptr);
+    for (int j = 0; j < nchannels; ++j) {
         HIP_CHECK(hipModuleGetFunction(&out.inv[(size_t)j], out.mod, ("chili_k_inv_" + std::to_string(j)).c_str()));
+        HIP_CHECK(hipModuleGetFunction(&out.geom[(size_t)j], out.mod, ("chili_k_geom_" + std::to_string(j)).c_str()));
+    }
 }
 
 }  // namespace chili_jit

@@ -479,27 +479,12 @@ __global__ __launch_bounds__(kTile) void k_geom(MCArgs A, int64_t kept, const ui
     const int nrow = (int)std::min<int64_t>(kTile, kept - k0);
     MomRF<2 * NT> P;
     if (valid) {
-        const uint32_t sp = comp[k];
-#pragma unroll
-        for (int i = 0; i < NT; ++i) {
-            P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
-            P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
-            P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
-            P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
-        }
+        load_mom<NT>(mom, nmax, comp[k], P);
         fbuf[k] = integrand<NT>(A.f, P);
     }
     double* row = s_r + threadIdx.x * PR;
     for (int j = 0; j < A.C; ++j) {
-        if (valid) {
-            // every slot is written by the channel program; 0.5 stands in
-            // for any a degenerate point leaves out
-#pragma unroll
-            for (int q = 0; q < D; ++q) row[q] = 0.5;
-            double vw;
-            const double ps = generate_weight<NT>(A, A.progs[j], j, P, &vw, nullptr, 0, nullptr, nullptr, 0, row);
-            abuf[(int64_t)j * stride + k] = A.alpha[j] / ps / vw;
-        }
+        if (valid) geom_one<NT, false>(A, A.progs[j], j, k, P, row, abuf, stride, nullptr);
         __syncthreads();
         double* dst = rall + ((int64_t)j * stride + k0) * D;
         for (int e = threadIdx.x; e < nrow * D; e += kTile) dst[e] = s_r[(e / D) * PR + e % D];
@@ -982,7 +967,7 @@ void no_host_callback(const chili_mc* m, const char* what) {
 
 template <int NT>
 void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, const double* z = nullptr) {
-    if (m->jit_fwd && !z) {
+    if (m->jit_fwd) {
         MCArgs a = A;
         const int* seg = m->d_seg.ptr;
         const int* tile = m->d_tiles.ptr;
@@ -991,9 +976,9 @@ void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, c
         int64_t nmax = m->nmax;
         uint8_t* acc = m->d_acc.ptr;
         PointRec r = R;
-        const double* zs = nullptr;
+        const double* zs = z;
         void* args[] = {&a, &seg, &tile, &sorted, &mom, &nmax, &acc, &r, &zs};
-        HIP_CHECK(hipModuleLaunchKernel(m->jit->forward, grid, 1, 1, kTile, 1, 1, 0, m->stream, args, nullptr));
+        HIP_CHECK(hipModuleLaunchKernel(z ? m->jit->forward_z : m->jit->forward, grid, 1, 1, kTile, 1, 1, 0, m->stream, args, nullptr));
         return;
     }
     if (z)
@@ -1395,8 +1380,28 @@ void launch_zgen(chili_mc* m, const MCArgs& A, int64_t n) {
 }
 template <int NT>
 void launch_geom(chili_mc* m, const MCArgs& A, int64_t kept) {
-    hipLaunchKernelGGL(k_geom<NT>, dim3((unsigned)((kept + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, kept, m->d_comp.ptr,
-                       m->d_mom.ptr, m->nmax, m->d_rall.ptr, m->d_abuf.ptr, m->d_fbuf.ptr, m->fmax);
+    const unsigned g = (unsigned)((kept + kTile - 1) / kTile);
+    if (m->jit_split) {
+        // run-time specialised: one kernel per channel (the split weight
+        // pass's structure), this channel's search table in LDS when small
+        constexpr int D = 3 * (NT - 2) - 2;
+        const size_t one_bytes = sizeof(double) * (size_t)D * m->P2;
+        int lds_search = (m->lds_search && one_bytes <= kLdsSearchMax) ? 1 : 0;
+        const size_t lds = sizeof(double) * (size_t)kTile * (D + 1) + (lds_search ? one_bytes : 0);
+        MCArgs a = A;
+        int64_t kp = kept, nmax = m->nmax, stride = m->fmax;
+        const uint32_t* comp = m->d_comp.ptr;
+        const double* mom = m->d_mom.ptr;
+        double* rall = m->d_rall.ptr;
+        double* abuf = m->d_abuf.ptr;
+        double* fbuf = m->d_fbuf.ptr;
+        void* args[] = {&a, &kp, &comp, &mom, &nmax, &rall, &abuf, &fbuf, &stride, &lds_search};
+        for (int j = 0; j < m->C; ++j)
+            HIP_CHECK(hipModuleLaunchKernel(m->jit->geom[(size_t)j], g, 1, 1, kTile, 1, 1, (unsigned)lds, m->stream, args, nullptr));
+        return;
+    }
+    hipLaunchKernelGGL(k_geom<NT>, dim3(g), dim3(kTile), 0, m->stream, A, kept, m->d_comp.ptr, m->d_mom.ptr, m->nmax, m->d_rall.ptr,
+                       m->d_abuf.ptr, m->d_fbuf.ptr, m->fmax);
 }
 
 // One sub-batch of the learned-proposal path (fl == null: every q_j = 1,
@@ -2656,9 +2661,10 @@ int chili_batch_tune(chili_batch* b, const uint64_t* seed, const uint32_t* strea
 // Run-time specialisation (no reference counterpart): compile the forward and
 // weight kernels of this context with its channel programs as compile-time
 // constants (hipRTC, gfx950) and use them from now on: enable bit 0 selects
-// the specialised forward kernel, bit 1 the specialised weight kernel (0:
-// the generic kernels).  compile_s (optional) receives the compile time of
-// the first enable.
+// the specialised forward kernel (both paths), bit 1 the specialised fused
+// weight kernel, bit 2 the split weight pass and on the learned-proposal
+// path the per-channel geometry kernels (0: the generic kernels).
+// compile_s (optional) receives the compile time of the first enable.
 int chili_mc_specialize(chili_mc* m, int enable, double* compile_s) {
     return guarded([&] {
         if (!m) throw chili_error(CHILI_EINVAL, "null context");

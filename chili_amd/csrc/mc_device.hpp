@@ -344,4 +344,34 @@ __device__ __forceinline__ void inverse_one(const MCArgs& A, const ChanProg& p, 
     if (R.status) R.ps[(int64_t)j * R.n + att] = ps;
 }
 
+// Momenta of sorted position sp from the SoA columns [4 NT][nmax].
+template <int NT>
+__device__ __forceinline__ void load_mom(const double* __restrict__ mom, int64_t nmax, uint32_t sp, MomRF<2 * NT>& P) {
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
+        P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
+        P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
+        P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+    }
+}
+
+// Learned-proposal path, one kept point k with momenta P for channel j
+// (program p): the hypercube point r_j of its inverse map (written to `row`,
+// D doubles) and a_j = alpha_j / ps_j / vegas_j (abuf[j][k]).  The generic
+// k_geom loops over every channel; the run-time specialised kernels run one
+// channel each (search: this channel's table in LDS when kLds).
+template <int NT, bool kLds>
+__device__ __forceinline__ void geom_one(const MCArgs& A, const ChanProg& p, int j, int64_t k, MomRF<2 * NT>& P, double* row,
+                                         double* __restrict__ abuf, int64_t stride, const double* search) {
+    constexpr int D = 3 * (NT - 2) - 2;
+    // every slot is written by the channel program; 0.5 stands in for any a
+    // degenerate point leaves out
+#pragma unroll
+    for (int q = 0; q < D; ++q) row[q] = 0.5;
+    double vw;
+    const double ps = generate_weight<NT, kLds>(A, p, j, P, &vw, nullptr, 0, nullptr, nullptr, 0, row, kLds ? search : nullptr, kLds);
+    abuf[(int64_t)j * stride + k] = A.alpha[j] / ps / vw;
+}
+
 }  // namespace chili_dev

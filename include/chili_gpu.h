@@ -235,8 +235,12 @@ int chili_mc_set_host_integrand(chili_mc* m, chili_host_integrand_fn fn, void* u
 /* Run-time specialisation (no reference counterpart): compile the forward
  * and weight kernels with this context's channel programs as compile-time
  * constants (hipRTC, gfx950); enable bit 0 uses the specialised forward
- * kernel, bit 1 the specialised weight kernel, 0 the generic ones;
- * compile_s receives the compile time.  Same results. */
+ * kernel (also on the learned-proposal path), bit 1 the specialised fused
+ * weight kernel, bit 2 the split weight pass (one specialised inverse kernel
+ * per channel; on the learned-proposal path, one specialised kernel per
+ * channel for the hypercube points and densities of the kept points; takes
+ * precedence over bit 1), 0 the generic ones; compile_s receives the compile
+ * time.  Same results. */
 int chili_mc_specialize(chili_mc* m, int enable, double* compile_s);
 /* The generated specialised kernel source of a channel set (diagnostics),
  * and its run-time compilation without loading it (no GPU needed). */

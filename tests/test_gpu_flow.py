@@ -395,14 +395,17 @@ def test_flow_mixture_unbiased():
     assert abs(m1 - m2) < 4 * np.hypot(e1, e2)
 
 
+@pytest.mark.parametrize("mode", [1, 4, 5])
 @pytest.mark.parametrize("mix", [0.0, 0.4])
-def test_flow_path_specialised_context(mix):
-    """A context with the run-time specialised forward kernel gives the flow
-    path's attempts, statistics and values bit for bit (two identically
-    adapted contexts, one specialised)."""
+def test_flow_path_specialised_context(mix, mode):
+    """A context with run-time specialised kernels gives the flow path's
+    attempts, statistics and values bit for bit (two identically adapted
+    contexts, one specialised): mode 1 the forward map of the proposal's
+    points, 4 the per-channel hypercube points and densities of the kept
+    points, 5 both."""
     _, _, _, g0 = adapted_context()
     _, _, _, g1 = adapted_context()
-    g1.specialize(1)
+    g1.specialize(mode)
     fl, _ = make_flow(g0.ndims, 4, g0.nchannels, 0.05, seed=29)
     fl.set_mixture(mix)
     n = 60000
