@@ -117,21 +117,42 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
              "}\n\n";
     // forward map: from the attempt's uniforms (chili_k_forward) or, on the
     // learned-proposal path, from the proposal's hypercube points (_z)
+    // (packed tiles when sorted_out is given, as in the generic k_forward:
+    // passing attempts first, momenta at the packed position, lq rows moved)
     for (int zs = 0; zs < 2; ++zs) {
         s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward" << (zs ? "_z" : "")
           << "(MCArgs A, const int* __restrict__ seg_start,\n"
              "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"
-             "        uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {\n"
+             "        uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc, uint32_t* __restrict__ sorted_out,\n"
+             "        const double* __restrict__ lq_in, double* __restrict__ lq_out) {\n"
+             "    __shared__ int s_cnt[2 * kWaves];\n"
              "    int ch, e0, e1;\n"
              "    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;\n"
              "    const int e = e0 + threadIdx.x;\n"
-             "    if (e >= e1) return;\n"
-             "    const uint32_t a = sorted[e];\n"
-             "    switch (ch) {\n";
+             "    const bool live = e < e1;\n"
+             "    MomRF<2 * NT> P;\n"
+             "    bool ok = false;\n"
+             "    uint32_t a = 0;\n"
+             "    if (live) {\n"
+             "        a = sorted[e];\n"
+             "        switch (ch) {\n";
         for (size_t j = 0; j < progs.size(); ++j)
-            s << "        case " << j << ": forward_one<NT, " << (zs ? "true" : "false") << ">(A, chili_prog_" << j << "(), " << j
-              << ", e, a, A.a0 + a, mom, nmax, acc, R, zsrc); break;\n";
-        s << "        default: break;\n    }\n}\n\n";
+            s << "            case " << j << ": ok = forward_point<NT, " << (zs ? "true" : "false") << ">(A, chili_prog_" << j << "(), "
+              << j << ", e, a, A.a0 + a, acc, R, zsrc, P); break;\n";
+        s << "            default: break;\n"
+             "        }\n"
+             "    }\n";
+        s << "    if (!sorted_out) {\n"
+             "        if (live && ok) store_mom<NT>(mom, nmax, e, P);\n"
+             "        return;\n"
+             "    }\n"
+             "    const int ep = tile_pack_pos(live, ok, e0, s_cnt);\n"
+             "    if (live) {\n"
+             "        sorted_out[ep] = a;\n"
+             "        if (ok) store_mom<NT>(mom, nmax, ep, P);\n"
+             "        if (lq_out) lq_out[ep] = lq_in[e];\n"
+             "    }\n"
+             "}\n\n";
     }
     // learned-proposal path: one kernel per channel writes r_j and a_j of
     // every kept point (channel 0 also f), rows staged in LDS (pitch D + 1)

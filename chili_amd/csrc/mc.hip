@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "channels.hpp"
@@ -197,20 +198,44 @@ __global__ void k_segments_mix(int C, const int* __restrict__ offs, int nblk, in
 }
 
 // ---------------------------------------------------------------------------
-// k_forward: VEGAS forward map, FSMapper::GeneratePoint and the cut for one
-// channel per workgroup.  Momenta of external legs go to mom[(i*4+k)*nmax + e].
+// k_forward: VEGAS forward map, FSMapper::GeneratePoint and the cut for the
+// attempts of one channel tile per workgroup.  Momenta of external legs go
+// to mom[(i*4+k)*nmax + e'].  Packed (sorted_out non-null): within the tile
+// the passing attempts move to the front (tile_pack_pos), sorted_out[e']
+// receives the attempt of new position e' and e' is where the momenta go,
+// so that the weight pass reads the kept points' momenta from contiguous
+// columns; the set and order of kept points are unchanged.  Otherwise
+// e' = e.  lq_in / lq_out (optional): a per-position FP64 row (the
+// learned proposal's sampling log-density) moved to the packed order.
 // ---------------------------------------------------------------------------
-// k_forward: forward_one for the attempts of one channel tile per workgroup.
 template <int NT, bool ZS>
 __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restrict__ seg_start, const int* __restrict__ tile_start,
                                                    const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,
-                                                   uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc) {
+                                                   uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc,
+                                                   uint32_t* __restrict__ sorted_out, const double* __restrict__ lq_in,
+                                                   double* __restrict__ lq_out) {
+    __shared__ int s_cnt[2 * kWaves];
     int ch, e0, e1;
     if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;
     const int e = e0 + threadIdx.x;
-    if (e >= e1) return;
-    const uint32_t a = sorted[e];
-    forward_one<NT, ZS>(A, A.progs[ch], ch, e, a, A.a0 + a, mom, nmax, acc, R, zsrc);
+    const bool live = e < e1;
+    if (!sorted_out) {
+        if (live) forward_one<NT, ZS>(A, A.progs[ch], ch, e, sorted[e], A.a0 + sorted[e], mom, nmax, acc, R, zsrc);
+        return;
+    }
+    MomRF<2 * NT> P;
+    bool ok = false;
+    uint32_t a = 0;
+    if (live) {
+        a = sorted[e];
+        ok = forward_point<NT, ZS>(A, A.progs[ch], ch, e, a, A.a0 + a, acc, R, zsrc, P);
+    }
+    const int ep = tile_pack_pos(live, ok, e0, s_cnt);
+    if (live) {
+        sorted_out[ep] = a;
+        if (ok) store_mom<NT>(mom, nmax, ep, P);
+        if (lq_out) lq_out[ep] = lq_in[e];      // per-position rows move with their attempt
+    }
 }
 
 // kT-jet PreProcess (CHILI_CUTS_KT_JETS), after the forward kernel's own
@@ -771,12 +796,13 @@ struct chili_mc {
     DevBuf<uint32_t> d_gpos;
     DevBuf<int> d_gblkoff;
     DevBuf<uint32_t> d_sorted, d_comp;
+    DevBuf<uint32_t> d_sorted2;     // packed order written by the forward kernel (swapped with d_sorted)
     DevBuf<uint8_t> d_acc;
     DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff, d_grp2;
     DevBuf<int64_t> d_total, d_bsum;
     // learned-proposal path: z/r [fmax][d], r_j [C][fmax][d], a_j and log q_j
     // [C][fmax], f [fmax], sampling log-density [fmax], StatsData partials
-    DevBuf<double> d_z, d_rall, d_abuf, d_lq, d_fbuf, d_lqs, d_fpart, d_xw;
+    DevBuf<double> d_z, d_rall, d_abuf, d_lq, d_fbuf, d_lqs, d_lqs2, d_fpart, d_xw;
     DevBuf<int64_t> d_own;         // own-channel kept ranges [C][2] of the density pass
     DevBuf<int> d_dseg, d_dend;    // density rows of each channel
     int64_t fmax = 0;
@@ -872,6 +898,7 @@ void ensure_capacity(chili_mc* m, int64_t n) {
     const int64_t ntiles = nblk + m->C + 1;
     m->d_pick.resize(n);
     m->d_sorted.resize(n);
+    m->d_sorted2.resize(n);
     m->d_comp.resize(n);
     m->d_acc.resize(n);
     m->d_mom.resize((size_t)n * m->NT * 4);
@@ -966,7 +993,15 @@ void no_host_callback(const chili_mc* m, const char* what) {
 }
 
 template <int NT>
-void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, const double* z = nullptr) {
+void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, const double* z = nullptr, bool pack = true) {
+    // packed tiles (the kept points' momenta contiguous): the packed order
+    // replaces the sorted order for the rest of the sub-batch, and on the
+    // learned-proposal path the sampling log-densities (d_lqs) move with it.
+    // A mixture proposal keeps the sorted order (pack = false): its tiles
+    // can hold both components of a channel, whose ranges must stay put.
+    uint32_t* out = pack ? m->d_sorted2.ptr : nullptr;
+    const double* lqi = (pack && z) ? m->d_lqs.ptr : nullptr;
+    double* lqo = (pack && z) ? m->d_lqs2.ptr : nullptr;
     if (m->jit_fwd) {
         MCArgs a = A;
         const int* seg = m->d_seg.ptr;
@@ -977,16 +1012,17 @@ void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, c
         uint8_t* acc = m->d_acc.ptr;
         PointRec r = R;
         const double* zs = z;
-        void* args[] = {&a, &seg, &tile, &sorted, &mom, &nmax, &acc, &r, &zs};
+        void* args[] = {&a, &seg, &tile, &sorted, &mom, &nmax, &acc, &r, &zs, &out, &lqi, &lqo};
         HIP_CHECK(hipModuleLaunchKernel(z ? m->jit->forward_z : m->jit->forward, grid, 1, 1, kTile, 1, 1, 0, m->stream, args, nullptr));
-        return;
-    }
-    if (z)
+    } else if (z) {
         hipLaunchKernelGGL((k_forward<NT, true>), dim3(grid), dim3(kTile), 0, m->stream, A, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr,
-                           m->d_mom.ptr, m->nmax, m->d_acc.ptr, R, z);
-    else
+                           m->d_mom.ptr, m->nmax, m->d_acc.ptr, R, z, out, lqi, lqo);
+    } else {
         hipLaunchKernelGGL((k_forward<NT, false>), dim3(grid), dim3(kTile), 0, m->stream, A, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr,
-                           m->d_mom.ptr, m->nmax, m->d_acc.ptr, R, z);
+                           m->d_mom.ptr, m->nmax, m->d_acc.ptr, R, z, out, lqi, lqo);
+    }
+    if (out) std::swap(m->d_sorted, m->d_sorted2);
+    if (lqo) std::swap(m->d_lqs, m->d_lqs2);
 }
 template <int NT>
 void launch_main(chili_mc* m, const MCArgs& A, int grid, int train, double* vals, int64_t vals_off, int64_t vals_cap,
@@ -1368,6 +1404,7 @@ void ensure_flow_capacity(chili_mc* m, int64_t n) {
     m->d_lq.resize((size_t)m->C * n);
     m->d_fbuf.resize(n);
     m->d_lqs.resize(n);
+    m->d_lqs2.resize(n);
     m->d_fpart.resize((size_t)((n + kTile - 1) / kTile) * kStatW);
     m->fmax = n;
 }
@@ -1449,7 +1486,7 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
         fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr, mixing ? m->d_fend.ptr : nullptr,
                            mixing ? (int64_t)((1.0 - fl->eng.mix) * (double)n) + 1 : n);
     }
-    CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr));
+    CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr, !mixing));
     launch_ktcut(m, n, R);
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
     if (kept == 0) return;

@@ -51,14 +51,14 @@ __device__ __forceinline__ double wave_max(double v) {
 
 
 // One attempt: uniforms (or the proposal's hypercube point), VEGAS forward
-// map, FSMapper::GeneratePoint and the cut.  e: sorted position (momentum
-// column), a: attempt slot of acc / per-attempt records, att: the attempt's
-// uniform counter.
+// map, FSMapper::GeneratePoint and the cut; returns the cut's decision and
+// leaves the external momenta in P.  e: sorted position (row of zsrc), a:
+// attempt slot of acc / per-attempt records, att: the attempt's uniform
+// counter.
 template <int NT, bool ZS>
-__device__ __forceinline__ void forward_one(const MCArgs& A, const ChanProg& p, int ch, int e, uint32_t a, uint64_t att,
-                                            double* __restrict__ mom,
-                                            int64_t nmax, uint8_t* __restrict__ acc, const PointRec& R,
-                                            const double* __restrict__ zsrc) {
+__device__ __forceinline__ bool forward_point(const MCArgs& A, const ChanProg& p, int ch, int e, uint32_t a, uint64_t att,
+                                              uint8_t* __restrict__ acc, const PointRec& R, const double* __restrict__ zsrc,
+                                              MomRF<2 * NT>& P) {
     constexpr int D = 3 * (NT - 2) - 2;
     double r[D + 1];
     if (ZS) {
@@ -85,7 +85,6 @@ __device__ __forceinline__ void forward_one(const MCArgs& A, const ChanProg& p, 
         double size = ed[idx + 1] - ed[idx];
         r[i] = left + loc * size;
     }
-    MomRF<2 * NT> P;
     double m2[2 * NT];
     generate_point<NT>(A, p, r, P, m2);
     bool ok = cut_pass<NT>(A.f, P, A.sqrts);
@@ -101,15 +100,56 @@ __device__ __forceinline__ void forward_one(const MCArgs& A, const ChanProg& p, 
             R.mom[(int64_t)(i * 4 + 3) * R.n + a] = P.v[4 * i + 3];
         }
     }
-    if (ok) {
+    return ok;
+}
+
+// External momenta of one point to column col of the SoA array [4 NT][nmax].
+template <int NT>
+__device__ __forceinline__ void store_mom(double* __restrict__ mom, int64_t nmax, int64_t col, const MomRF<2 * NT>& P) {
 #pragma unroll
-        for (int i = 0; i < NT; ++i) {
-            mom[(int64_t)(i * 4 + 0) * nmax + e] = P.v[4 * i + 0];
-            mom[(int64_t)(i * 4 + 1) * nmax + e] = P.v[4 * i + 1];
-            mom[(int64_t)(i * 4 + 2) * nmax + e] = P.v[4 * i + 2];
-            mom[(int64_t)(i * 4 + 3) * nmax + e] = P.v[4 * i + 3];
-        }
+    for (int i = 0; i < NT; ++i) {
+        mom[(int64_t)(i * 4 + 0) * nmax + col] = P.v[4 * i + 0];
+        mom[(int64_t)(i * 4 + 1) * nmax + col] = P.v[4 * i + 1];
+        mom[(int64_t)(i * 4 + 2) * nmax + col] = P.v[4 * i + 2];
+        mom[(int64_t)(i * 4 + 3) * nmax + col] = P.v[4 * i + 3];
     }
+}
+
+// forward_point with the momenta of a passing attempt stored at its sorted
+// position e.
+template <int NT, bool ZS>
+__device__ __forceinline__ void forward_one(const MCArgs& A, const ChanProg& p, int ch, int e, uint32_t a, uint64_t att,
+                                            double* __restrict__ mom,
+                                            int64_t nmax, uint8_t* __restrict__ acc, const PointRec& R,
+                                            const double* __restrict__ zsrc) {
+    MomRF<2 * NT> P;
+    if (forward_point<NT, ZS>(A, p, ch, e, a, att, acc, R, zsrc, P)) store_mom<NT>(mom, nmax, e, P);
+}
+
+// Packed order of a forward tile [e0, e1) (one workgroup of kTile threads,
+// thread t holding position e0 + t, live when inside the tile): the attempts
+// that pass the cut first, then the others, each group in its sorted order.
+// Returns the thread's new position; every thread of the workgroup must call
+// it (it synchronises the workgroup).  s_cnt: LDS, 2 kWaves ints.
+__device__ __forceinline__ int tile_pack_pos(bool live, bool ok, int e0, int* s_cnt) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t mo = __ballot(live && ok), mn = __ballot(live && !ok);
+    if (lane == 0) {
+        s_cnt[w] = __popcll(mo);
+        s_cnt[kWaves + w] = __popcll(mn);
+    }
+    __syncthreads();
+    int before_o = 0, before_n = 0, tot_o = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+        if (ww < w) {
+            before_o += s_cnt[ww];
+            before_n += s_cnt[kWaves + ww];
+        }
+        tot_o += s_cnt[ww];
+    }
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    return e0 + (ok ? before_o + __popcll(mo & lt) : tot_o + before_n + __popcll(mn & lt));
 }
 
 
