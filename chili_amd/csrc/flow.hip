@@ -582,20 +582,34 @@ void FlowEngine::backward(int f, int64_t nb, const double* dlq) {
 // given the kernel's registers and this layer's LDS (the weights are loaded
 // once per workgroup, so larger groups are preferred on ties; multiples of
 // four waves spread evenly over a CU's four SIMDs).
+// The choice depends only on the kernel and the LDS size, so it is made once
+// per (device, LDS size) and kept (the attribute calls cost host time on
+// every one of a step's many launches otherwise).
 template <bool INV, int NP>
 static void launch_couple_np(dim3 g, size_t lds, hipStream_t st, const CoupleArgs& a) {
-    const void* fn = (const void*)k_couple<kSplineBins, INV, NP>;
-    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int nw = 8, best = 0;
-    for (int w : {8, 4}) {
-        int nb = 0;
-        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64 * w, lds));
-        if (nb * w > best) {
-            best = nb * w;
-            nw = w;
+    struct Shape { int device; size_t lds; int nw; };
+    static thread_local std::vector<Shape> known;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    int nw = 0;
+    for (const Shape& k : known)
+        if (k.device == dev && k.lds == lds) nw = k.nw;
+    if (nw == 0) {
+        const void* fn = (const void*)k_couple<kSplineBins, INV, NP>;
+        HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int best = 0;
+        nw = 8;
+        for (int w : {8, 4}) {
+            int nb = 0;
+            HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64 * w, lds));
+            if (nb * w > best) {
+                best = nb * w;
+                nw = w;
+            }
         }
+        if (best == 0) throw chili_error(CHILI_ERANGE, "coupling layer does not fit one CU");
+        known.push_back(Shape{dev, lds, nw});
     }
-    if (best == 0) throw chili_error(CHILI_ERANGE, "coupling layer does not fit one CU");
     hipLaunchKernelGGL((k_couple<kSplineBins, INV, NP>), g, dim3(64 * nw), lds, st, a);
 }
 

@@ -115,45 +115,41 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
              "    else\n"
              "        inverse_one<NT, false>(A, chili_prog_" << j << "(), " << j << ", e, seg2, comp, mom, nmax, pv, bins, sorted, R, nullptr);\n"
              "}\n\n";
-    // forward map: from the attempt's uniforms (chili_k_forward) or, on the
-    // learned-proposal path, from the proposal's hypercube points (_z)
-    // (packed tiles when sorted_out is given, as in the generic k_forward:
-    // passing attempts first, momenta at the packed position, lq rows moved)
-    for (int zs = 0; zs < 2; ++zs) {
-        s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward" << (zs ? "_z" : "")
-          << "(MCArgs A, const int* __restrict__ seg_start,\n"
-             "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"
-             "        uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc, uint32_t* __restrict__ sorted_out,\n"
-             "        const double* __restrict__ lq_in, double* __restrict__ lq_out) {\n"
-             "    __shared__ int s_cnt[2 * kWaves];\n"
-             "    int ch, e0, e1;\n"
-             "    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;\n"
-             "    const int e = e0 + threadIdx.x;\n"
-             "    const bool live = e < e1;\n"
-             "    MomRF<2 * NT> P;\n"
-             "    bool ok = false;\n"
-             "    uint32_t a = 0;\n"
-             "    if (live) {\n"
-             "        a = sorted[e];\n"
-             "        switch (ch) {\n";
-        for (size_t j = 0; j < progs.size(); ++j)
-            s << "            case " << j << ": ok = forward_point<NT, " << (zs ? "true" : "false") << ">(A, chili_prog_" << j << "(), "
-              << j << ", e, a, A.a0 + a, acc, R, zsrc, P); break;\n";
-        s << "            default: break;\n"
-             "        }\n"
-             "    }\n";
-        s << "    if (!sorted_out) {\n"
-             "        if (live && ok) store_mom<NT>(mom, nmax, e, P);\n"
-             "        return;\n"
-             "    }\n"
-             "    const int ep = tile_pack_pos(live, ok, e0, s_cnt);\n"
-             "    if (live) {\n"
-             "        sorted_out[ep] = a;\n"
-             "        if (ok) store_mom<NT>(mom, nmax, ep, P);\n"
-             "        if (lq_out) lq_out[ep] = lq_in[e];\n"
-             "    }\n"
-             "}\n\n";
-    }
+    // forward map of the attempts' uniforms (packed tiles when sorted_out is
+    // given, as in the generic k_forward: passing attempts first, momenta at
+    // the packed position).  The learned-proposal path keeps the generic
+    // forward kernel: a specialised variant reading the proposal's points
+    // measured slower in the production step.
+    s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_forward(MCArgs A, const int* __restrict__ seg_start,\n"
+         "        const int* __restrict__ tile_start, const uint32_t* __restrict__ sorted, double* __restrict__ mom, int64_t nmax,\n"
+         "        uint8_t* __restrict__ acc, PointRec R, const double* __restrict__ zsrc, uint32_t* __restrict__ sorted_out) {\n"
+         "    __shared__ int s_cnt[2 * kWaves];\n"
+         "    int ch, e0, e1;\n"
+         "    if (!locate_tile(A.C, seg_start, tile_start, blockIdx.x, ch, e0, e1)) return;\n"
+         "    const int e = e0 + threadIdx.x;\n"
+         "    const bool live = e < e1;\n"
+         "    MomRF<2 * NT> P;\n"
+         "    bool ok = false;\n"
+         "    uint32_t a = 0;\n"
+         "    if (live) {\n"
+         "        a = sorted[e];\n"
+         "        switch (ch) {\n";
+    for (size_t j = 0; j < progs.size(); ++j)
+        s << "            case " << j << ": ok = forward_point<NT, false>(A, chili_prog_" << j << "(), " << j
+          << ", e, a, A.a0 + a, acc, R, zsrc, P); break;\n";
+    s << "            default: break;\n"
+         "        }\n"
+         "    }\n"
+         "    if (!sorted_out) {\n"
+         "        if (live && ok) store_mom<NT>(mom, nmax, e, P);\n"
+         "        return;\n"
+         "    }\n"
+         "    const int ep = tile_pack_pos(live, ok, e0, s_cnt);\n"
+         "    if (live) {\n"
+         "        sorted_out[ep] = a;\n"
+         "        if (ok) store_mom<NT>(mom, nmax, ep, P);\n"
+         "    }\n"
+         "}\n\n";
     // learned-proposal path: one kernel per channel writes r_j and a_j of
     // every kept point (channel 0 also f), rows staged in LDS (pitch D + 1)
     // and written out contiguously, as the generic k_geom does per channel
@@ -227,7 +223,6 @@ void compile(const std::string& src, Module& out, double* seconds, bool load, in
     HIP_CHECK(hipModuleGetFunction(&out.main, out.mod, "chili_k_main"));
     HIP_CHECK(hipModuleGetFunction(&out.main_lds, out.mod, "chili_k_main_lds"));
     HIP_CHECK(hipModuleGetFunction(&out.forward, out.mod, "chili_k_forward"));
-    HIP_CHECK(hipModuleGetFunction(&out.forward_z, out.mod, "chili_k_forward_z"));
     out.inv.assign((size_t)nchannels, nullptr);
     out.geom.assign((size_t)nchannels, nullptr);
     for (int j = 0; j < nchannels; ++j) {

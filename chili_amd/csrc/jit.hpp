@@ -12,7 +12,6 @@ namespace chili_jit {
 struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t main = nullptr, main_lds = nullptr, forward = nullptr;
-    hipFunction_t forward_z = nullptr;   // forward map of the learned proposal's hypercube points
     std::vector<hipFunction_t> inv;      // split weight pass: one inverse kernel per channel
     std::vector<hipFunction_t> geom;     // learned-proposal path: r_j and a_j, one kernel per channel
     Module() = default;

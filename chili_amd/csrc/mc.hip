@@ -1002,7 +1002,7 @@ void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, c
     uint32_t* out = pack ? m->d_sorted2.ptr : nullptr;
     const double* lqi = (pack && z) ? m->d_lqs.ptr : nullptr;
     double* lqo = (pack && z) ? m->d_lqs2.ptr : nullptr;
-    if (m->jit_fwd) {
+    if (m->jit_fwd && !z) {
         MCArgs a = A;
         const int* seg = m->d_seg.ptr;
         const int* tile = m->d_tiles.ptr;
@@ -1011,9 +1011,9 @@ void launch_forward(chili_mc* m, const MCArgs& A, int grid, const PointRec& R, c
         int64_t nmax = m->nmax;
         uint8_t* acc = m->d_acc.ptr;
         PointRec r = R;
-        const double* zs = z;
-        void* args[] = {&a, &seg, &tile, &sorted, &mom, &nmax, &acc, &r, &zs, &out, &lqi, &lqo};
-        HIP_CHECK(hipModuleLaunchKernel(z ? m->jit->forward_z : m->jit->forward, grid, 1, 1, kTile, 1, 1, 0, m->stream, args, nullptr));
+        const double* zs = nullptr;
+        void* args[] = {&a, &seg, &tile, &sorted, &mom, &nmax, &acc, &r, &zs, &out};
+        HIP_CHECK(hipModuleLaunchKernel(m->jit->forward, grid, 1, 1, kTile, 1, 1, 0, m->stream, args, nullptr));
     } else if (z) {
         hipLaunchKernelGGL((k_forward<NT, true>), dim3(grid), dim3(kTile), 0, m->stream, A, m->d_seg.ptr, m->d_tiles.ptr, m->d_sorted.ptr,
                            m->d_mom.ptr, m->nmax, m->d_acc.ptr, R, z, out, lqi, lqo);
@@ -1486,7 +1486,16 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
         fl->eng.fused_pass(true, C, 0, m->d_seg.ptr, 0, 0, n, m->d_z.ptr, m->d_lqs.ptr, mixing ? m->d_fend.ptr : nullptr,
                            mixing ? (int64_t)((1.0 - fl->eng.mix) * (double)n) + 1 : n);
     }
+    std::pair<hipEvent_t, hipEvent_t> fev{};
+    if (m->timing) {
+        fev = take_events(m);
+        HIP_CHECK(hipEventRecord(fev.first, m->stream));
+    }
     CHILI_DISPATCH_NT(m->NT, launch_forward<NT>(m, A, tgrid, R, fl ? m->d_z.ptr : nullptr, !mixing));
+    if (m->timing) {
+        HIP_CHECK(hipEventRecord(fev.second, m->stream));
+        m->fev_live.push_back(fev);
+    }
     launch_ktcut(m, n, R);
     cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
     if (kept == 0) return;
@@ -2698,7 +2707,7 @@ int chili_batch_tune(chili_batch* b, const uint64_t* seed, const uint32_t* strea
 // Run-time specialisation (no reference counterpart): compile the forward and
 // weight kernels of this context with its channel programs as compile-time
 // constants (hipRTC, gfx950) and use them from now on: enable bit 0 selects
-// the specialised forward kernel (both paths), bit 1 the specialised fused
+// the specialised forward kernel (uniform attempts), bit 1 the specialised fused
 // weight kernel, bit 2 the split weight pass and on the learned-proposal
 // path the per-channel geometry kernels (0: the generic kernels).
 // compile_s (optional) receives the compile time of the first enable.

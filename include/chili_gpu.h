@@ -235,7 +235,7 @@ int chili_mc_set_host_integrand(chili_mc* m, chili_host_integrand_fn fn, void* u
 /* Run-time specialisation (no reference counterpart): compile the forward
  * and weight kernels with this context's channel programs as compile-time
  * constants (hipRTC, gfx950); enable bit 0 uses the specialised forward
- * kernel (also on the learned-proposal path), bit 1 the specialised fused
+ * kernel (uniform attempts), bit 1 the specialised fused
  * weight kernel, bit 2 the split weight pass (one specialised inverse kernel
  * per channel; on the learned-proposal path, one specialised kernel per
  * channel for the hypercube points and densities of the kept points; takes
