@@ -239,8 +239,23 @@ __global__ __launch_bounds__(kTile) void k_forward(MCArgs A, const int* __restri
 }
 
 // kT-jet PreProcess (CHILI_CUTS_KT_JETS), after the forward kernel's own
-// checks: the ŝ limit and the clustering of the final state of every
-// attempt that passed; a failure clears its flag (and record).
+// checks: the s-hat limit and the clustering of the final state of an
+// attempt that passed (momenta at column e).
+template <int NT>
+__device__ __forceinline__ bool kt_cut_pass(const chili_integrand* __restrict__ f, const double* __restrict__ mom, int64_t nmax,
+                                            int64_t e) {
+    V4 p[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+        p[i] = mk(mom[(int64_t)(4 * i) * nmax + e], mom[(int64_t)(4 * i + 1) * nmax + e], mom[(int64_t)(4 * i + 2) * nmax + e],
+                  mom[(int64_t)(4 * i + 3) * nmax + e]);
+    const double smax = f->ptmin;
+    if (smax > 0 && vmass2(vadd(p[0], p[1])) > smax * smax) return false;      // example.cc:29-32
+    return kt_all_jets<NT - 2>(p + 2, f->drmin);
+}
+
+// The kT-jet cut of every attempt that passed the forward kernel; a failure
+// clears its flag (and record).
 template <int NT>
 __global__ __launch_bounds__(kTile) void k_ktcut(const chili_integrand* __restrict__ f, int64_t n, const uint32_t* __restrict__ sorted,
                                                  const double* __restrict__ mom, int64_t nmax, uint8_t* __restrict__ acc, PointRec R) {
@@ -248,16 +263,7 @@ __global__ __launch_bounds__(kTile) void k_ktcut(const chili_integrand* __restri
     if (e >= n) return;
     const uint32_t a = sorted[e];
     if (!acc[a]) return;
-    V4 p[NT];
-#pragma unroll
-    for (int i = 0; i < NT; ++i)
-        p[i] = mk(mom[(int64_t)(4 * i) * nmax + e], mom[(int64_t)(4 * i + 1) * nmax + e], mom[(int64_t)(4 * i + 2) * nmax + e],
-                  mom[(int64_t)(4 * i + 3) * nmax + e]);
-    bool ok = true;
-    const double smax = f->ptmin;
-    if (smax > 0 && vmass2(vadd(p[0], p[1])) > smax * smax) ok = false;        // example.cc:29-32
-    if (ok) ok = kt_all_jets<NT - 2>(p + 2, f->drmin);
-    if (!ok) {
+    if (!kt_cut_pass<NT>(f, mom, nmax, e)) {
         acc[a] = 0;
         if (R.status) R.status[a] = 0;
     }
@@ -1564,7 +1570,7 @@ struct BJob {
     int64_t cnt0;       // offset of the job's count matrix [C][nblk]
     int blk0, nblk;     // attempt blocks
     int g0;             // first flattened segment
-    int pad_;
+    int kt;             // the job's cut is the kT-jet PreProcess
     double* tot;        // running totals [7 + C + C d B]
 };
 
@@ -1649,6 +1655,27 @@ __global__ __launch_bounds__(kTile) void k_bforward(int G, const BJob* __restric
             const uint32_t a = sorted[e];
             forward_one<NT, false>(jb.A, jb.A.progs[g - jb.g0], g - jb.g0, e, a, jb.A.a0 + (uint64_t)(a - jb.att0), mom, nmax, acc,
                                    PointRec{}, nullptr);
+        }
+    }
+}
+
+// The kT-jet cut of the jobs that use it (persistent, over the forward
+// kernel's tiles): an attempt that passed the forward kernel and fails the
+// clustering loses its flag, as in the single-context path.
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_bktcut(int G, const BJob* __restrict__ J, const int* __restrict__ gjob,
+                                                  const int* __restrict__ seg, const int* __restrict__ tile,
+                                                  const uint32_t* __restrict__ sorted, const double* __restrict__ mom, int64_t nmax,
+                                                  uint8_t* __restrict__ acc) {
+    const int ntile = tile[G];
+    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+        int g, e0, e1;
+        if (!locate_tile(G, seg, tile, t, g, e0, e1)) break;
+        const BJob& jb = J[gjob[g]];
+        const int e = e0 + threadIdx.x;
+        if (e < e1 && jb.kt) {
+            const uint32_t a = sorted[e];
+            if (acc[a] && !kt_cut_pass<NT>(jb.A.f, mom, nmax, e)) acc[a] = 0;
         }
     }
 }
@@ -1810,6 +1837,7 @@ namespace {
 // ===========================================================================
 struct BatchGroup {
     int NT = 0, D = 0, G = 0, Cmax = 0;
+    bool any_kt = false;                   // some job of the group uses the kT-jet cut
     std::vector<int> jobs;                 // indices into chili_batch::jobs
     std::vector<int64_t> n;                // planned attempts per job (this group's order)
     int64_t natt = 0, ncnt = 0;
@@ -1978,6 +2006,9 @@ void enqueue_group(chili_batch* b, BatchGroup& g, int train, bool timed) {
     if (timed) HIP_CHECK(hipEventRecord(g.ev_fwd.first, st));
     hipLaunchKernelGGL(k_bforward<NT>, dim3(batch_fgrid(b, g)), dim3(kTile), 0, st, g.G, g.d_jobs.ptr, g.d_gjob.ptr, g.d_seg.ptr,
                        g.d_tile.ptr, g.d_sorted.ptr, g.d_mom.ptr, g.natt, g.d_acc.ptr);
+    if (g.any_kt)
+        hipLaunchKernelGGL(k_bktcut<NT>, dim3(batch_fgrid(b, g)), dim3(kTile), 0, st, g.G, g.d_jobs.ptr, g.d_gjob.ptr, g.d_seg.ptr,
+                           g.d_tile.ptr, g.d_sorted.ptr, g.d_mom.ptr, g.natt, g.d_acc.ptr);
     if (timed) HIP_CHECK(hipEventRecord(g.ev_fwd.second, st));
     hipLaunchKernelGGL(k_acc_count, dim3(g.nblk), dim3(kTile), 0, st, g.d_acc.ptr, g.natt, g.d_blkacc.ptr);
     hipLaunchKernelGGL(k_bcutoff, dim3(J), dim3(64), 0, st, g.d_jobs.ptr, g.d_blkacc.ptr, g.d_acc.ptr, g.d_cut.ptr, g.d_keptj.ptr);
@@ -2022,6 +2053,7 @@ void batch_upload_jobs(chili_batch* b, const uint64_t* seed, const uint32_t* str
             jb.blk0 = g.blk0[q];
             jb.nblk = (int)(g.n[q] / kTile);
             jb.g0 = g.g0[q];
+            jb.kt = (m->f.cuts == CHILI_CUTS_KT_JETS) ? 1 : 0;
             jb.tot = g.d_tot.ptr + g.tot_off[q];
         }
         g.d_jobs.upload(g.h_jobs, b->stream);
@@ -2487,8 +2519,6 @@ int chili_batch_create(int njobs, chili_mc* const* jobs, chili_batch** out) {
             if (!jobs[j]) throw chili_error(CHILI_EINVAL, "null job");
             if (jobs[j]->device != b->device) throw chili_error(CHILI_EINVAL, "jobs of a batch share one device");
             if (jobs[j]->B != b->B) throw chili_error(CHILI_EINVAL, "jobs of a batch share one bin count");
-            if (jobs[j]->f.cuts == CHILI_CUTS_KT_JETS)
-                throw chili_error(CHILI_EINVAL, "the kT-jet cut runs in the single-context path only");
             no_host_callback(jobs[j], "the batched path");
             b->jobs.push_back(jobs[j]);
         }
@@ -2506,6 +2536,7 @@ int chili_batch_create(int njobs, chili_mc* const* jobs, chili_batch** out) {
                     b->job_group[j] = (int)b->groups.size();
                     g.jobs.push_back(j);
                     g.Cmax = std::max(g.Cmax, b->jobs[j]->C);
+                    g.any_kt = g.any_kt || b->jobs[j]->f.cuts == CHILI_CUTS_KT_JETS;
                 }
             if (g.jobs.empty()) continue;
             HIP_CHECK(hipEventCreate(&g.ev_fwd.first));

@@ -106,3 +106,37 @@ def test_batch_autotune():
     streams = [300 + j for j in range(len(jobs))]
     res = b.iterate(seeds, streams, NCALLS, train=True, graph=True)
     compare(res, single(jobs, seeds, streams, True), "tuned graph")
+
+
+def test_batch_kt_jet_cut():
+    """Jobs with the kT-jet PreProcess in a batch, in the same multiplicity
+    class as a job without it: attempts, statistics and training sums equal
+    each job's single-context iteration (whose kT cut is pinned against the
+    oracle's JetCluster)."""
+    specs = [(P.gluon_process(4, smax=3), P.integrand_propprod(cuts=("kt", 0.4, 900.0))),
+             (P.gluon_process(4, smax=3), P.integrand_propprod()),
+             (P.gluon_process(3), P.integrand_propprod(cuts=("kt", 0.4)))]
+    ncalls = [15000, 12000, 20000]
+    rng = np.random.default_rng(5)
+    chans, jobs = [], []
+    for proc, f in specs:
+        ch = lib.Channels(proc)
+        gpu = lib.MultiChannelGPU(ch, 32, f)
+        random_state(gpu, rng)
+        chans.append(ch)
+        jobs.append(gpu)
+    b = lib.BatchGPU(jobs)
+    seeds = [SEED + 40 + j for j in range(len(jobs))]
+    b.iterate(seeds, [1] * len(jobs), ncalls, train=False, graph=False)   # acceptance estimates
+    for rep, graph in enumerate((False, True, True)):
+        streams = [400 + 10 * rep + j for j in range(len(jobs))]
+        stats, att, Ts, hs = b.iterate(seeds, streams, ncalls, train=True, graph=graph)
+        for j, (gpu, n) in enumerate(zip(jobs, ncalls)):
+            st1, T1, h1, na1 = gpu.iterate(abi.philox_uniforms(int(seeds[j]), stream=int(streams[j])), n, a0=0, train=True)
+            what = f"kt batch rep {rep} job {j}"
+            assert att[j] == na1, f"{what}: attempts {att[j]} vs {na1}"
+            assert stats[j].n == st1.n and stats[j].n_finite == st1.n_finite, what
+            assert stats[j].min == st1.min and stats[j].max == st1.max, what
+            assert_rel(np.array([stats[j].sum, stats[j].sum2]), np.array([st1.sum, st1.sum2]), 1e-12, what=what + " sums")
+            assert_rel(Ts[j], T1, 1e-12, scale=max(np.abs(T1).max(), 1e-300), what=what + " T")
+            assert_rel(hs[j], h1, 1e-12, scale=max(np.abs(h1).max(), 1e-300), what=what + " hist")
