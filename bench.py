@@ -568,6 +568,28 @@ def main():
                                   note="chili_mc_generate: the MultiChannel loop with an unweighting PostProcess, "
                                        "events (momenta, weights) written to HBM")
         log(f"  [generate] {ne} unweighted events in {dt:.3f} s: {ne / dt:.3e} events/s ({gatt} attempts)")
+        # the same events with exact early rejection (chili_mc_specialize bit
+        # 3): only points whose picked channel's bound can pass the accept
+        # test are weighted in full
+        if not args.no_specialize:
+            mode = {v: k for k, v in VARIANTS.items()}.get(variant, 0)
+            gpu.specialize(mode | 8)
+            gpu.generate(abi.philox_uniforms(seed, stream=3000), 1 << 16, wmax, ctypes.c_void_p(ev.data_ptr()),
+                         ctypes.c_void_p(ew.data_ptr()), ne)                  # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gst2, gatt2 = gpu.generate(abi.philox_uniforms(seed, stream=3001), ne, wmax, ctypes.c_void_p(ev.data_ptr()),
+                                       ctypes.c_void_p(ew.data_ptr()), ne)
+            torch.cuda.synchronize()
+            dt2 = time.perf_counter() - t0
+            gpu.specialize(mode)
+            plain["generated"]["early_rejection"] = dict(
+                seconds=round(dt2, 4), events_per_s=round(ne / dt2, 1), attempts=int(gatt2),
+                same_attempts=bool(gatt2 == gatt), same_sum=bool(gst2.sum == gst.sum),
+                note="chili_mc_specialize bit 3: a passing point is weighted in full only when the bound "
+                     "f / (alpha_c / ps_c / vegas_c) of its picked channel c can pass the accept test (the "
+                     "multichannel weight never exceeds it): the same events")
+            log(f"  [generate, early rejection] {ne / dt2:.3e} events/s ({gatt2} attempts)")
         del ev, ew
 
     # ---- flow leg ----

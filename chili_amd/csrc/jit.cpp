@@ -150,6 +150,30 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
          "        if (ok) store_mom<NT>(mom, nmax, ep, P);\n"
          "    }\n"
          "}\n\n";
+    // exact early rejection of the unweighting stage: every passing point's
+    // integrand and picked channel's term (switch on the channel; the
+    // channels' search tables in LDS when they fit), the bound test on acc
+    s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_own(MCArgs A, int64_t kept, const int* __restrict__ seg2,\n"
+         "        const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax, const uint32_t* __restrict__ sorted,\n"
+         "        uint8_t* __restrict__ acc, double wmax, int lds_search) {\n"
+         "    extern __shared__ double s_srch[];\n"
+         "    if (lds_search) {\n"
+         "        for (int i = threadIdx.x; i < A.C * A.d * A.P2; i += kTile) s_srch[i] = A.search[i];\n"
+         "        __syncthreads();\n"
+         "    }\n"
+         "    const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;\n"
+         "    if (k >= kept) return;\n"
+         "    int pc = 0;\n"
+         "    for (int c = 1; c < A.C; ++c) pc = (seg2[c] <= k) ? c : pc;\n"
+         "    switch (pc) {\n";
+    for (size_t j = 0; j < progs.size(); ++j)
+        s << "        case " << j << ":\n"
+             "            if (lds_search) own_bound_one<NT, true>(A, chili_prog_" << j << "(), " << j
+          << ", k, comp, mom, nmax, sorted, acc, wmax, s_srch + (int64_t)" << j << " * A.d * A.P2);\n"
+             "            else own_bound_one<NT, false>(A, chili_prog_" << j << "(), " << j
+          << ", k, comp, mom, nmax, sorted, acc, wmax, nullptr);\n"
+             "            break;\n";
+    s << "        default: break;\n    }\n}\n\n";
     // learned-proposal path: one kernel per channel writes r_j and a_j of
     // every kept point (channel 0 also f), rows staged in LDS (pitch D + 1)
     // and written out contiguously, as the generic k_geom does per channel
@@ -223,6 +247,7 @@ void compile(const std::string& src, Module& out, double* seconds, bool load, in
     HIP_CHECK(hipModuleGetFunction(&out.main, out.mod, "chili_k_main"));
     HIP_CHECK(hipModuleGetFunction(&out.main_lds, out.mod, "chili_k_main_lds"));
     HIP_CHECK(hipModuleGetFunction(&out.forward, out.mod, "chili_k_forward"));
+    HIP_CHECK(hipModuleGetFunction(&out.own, out.mod, "chili_k_own"));
     out.inv.assign((size_t)nchannels, nullptr);
     out.geom.assign((size_t)nchannels, nullptr);
     for (int j = 0; j < nchannels; ++j) {

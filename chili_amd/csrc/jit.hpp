@@ -12,6 +12,7 @@ namespace chili_jit {
 struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t main = nullptr, main_lds = nullptr, forward = nullptr;
+    hipFunction_t own = nullptr;         // unweighting stage: bound on the weight from the picked channel
     std::vector<hipFunction_t> inv;      // split weight pass: one inverse kernel per channel
     std::vector<hipFunction_t> geom;     // learned-proposal path: r_j and a_j, one kernel per channel
     Module() = default;

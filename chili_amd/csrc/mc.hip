@@ -818,6 +818,7 @@ struct chili_mc {
     std::unique_ptr<chili_jit::Module> jit;
     bool jit_fwd = false, jit_main = false;
     bool jit_split = false;      // split weight pass: per-channel specialised inverse kernels + combine
+    bool jit_early = false;      // unweighting stage: exact early rejection by the picked channel's bound
     DevBuf<double> d_pre;        // split pass: ps and VEGAS weights [2 C][kept], then f [kept]
     DevBuf<short> d_prebins;     // split pass: the picked channel's bins [d][kept]
     bool lds_search = true;      // weight kernel: search tables in LDS when they fit
@@ -1246,6 +1247,26 @@ void run_subbatch_gen(chili_mc* m, uint64_t seed, uint32_t stream, uint64_t a0, 
     const int tgrid = pick_sort_forward(m, A, n, PointRec{});
     int64_t passed = 0, all = 0;
     cutoff_and_compact(m, n, nblk, tgrid, std::numeric_limits<int64_t>::max(), passed, all);
+    if (m->jit_early && passed > 0) {
+        // exact early rejection: the points whose bound cannot pass the
+        // accept test lose their flag; the rest are compacted again and
+        // weighted in full (the same decisions as weighting every point)
+        constexpr size_t kOwnLdsMax = 32 * 1024;
+        const size_t tbl = sizeof(double) * (size_t)m->C * m->d * m->P2;
+        int lds_search = (m->lds_search && tbl <= kOwnLdsMax) ? 1 : 0;
+        MCArgs a = A;
+        int64_t kp = passed, nmax = m->nmax;
+        const int* seg2 = m->d_seg2.ptr;
+        const uint32_t* comp = m->d_comp.ptr;
+        const double* mom = m->d_mom.ptr;
+        const uint32_t* sorted = m->d_sorted.ptr;
+        uint8_t* acc = m->d_acc.ptr;
+        double wm = wmax;
+        void* args[] = {&a, &kp, &seg2, &comp, &mom, &nmax, &sorted, &acc, &wm, &lds_search};
+        HIP_CHECK(hipModuleLaunchKernel(m->jit->own, (unsigned)((passed + kTile - 1) / kTile), 1, 1, kTile, 1, 1,
+                                        lds_search ? (unsigned)tbl : 0u, m->stream, args, nullptr));
+        cutoff_and_compact(m, n, nblk, tgrid, std::numeric_limits<int64_t>::max(), passed, all);
+    }
     m->d_gflag.resize(n);
     m->d_gv.resize(n);
     m->d_gpos.resize(n);
@@ -2740,7 +2761,8 @@ int chili_batch_tune(chili_batch* b, const uint64_t* seed, const uint32_t* strea
 // constants (hipRTC, gfx950) and use them from now on: enable bit 0 selects
 // the specialised forward kernel (uniform attempts), bit 1 the specialised fused
 // weight kernel, bit 2 the split weight pass and on the learned-proposal
-// path the per-channel geometry kernels (0: the generic kernels).
+// path the per-channel geometry kernels, bit 3 the unweighting stage's exact
+// early rejection (0: the generic kernels, no early rejection).
 // compile_s (optional) receives the compile time of the first enable.
 int chili_mc_specialize(chili_mc* m, int enable, double* compile_s) {
     return guarded([&] {
@@ -2753,6 +2775,7 @@ int chili_mc_specialize(chili_mc* m, int enable, double* compile_s) {
         }
         m->jit_fwd = (enable & 1) != 0;
         m->jit_split = (enable & 4) != 0;
+        m->jit_early = (enable & 8) != 0;
         m->jit_main = (enable & 2) != 0 && !m->jit_split;
         if (compile_s) *compile_s = m->jit_compile_s;
     });

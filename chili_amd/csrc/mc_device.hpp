@@ -384,6 +384,38 @@ __device__ __forceinline__ void inverse_one(const MCArgs& A, const ChanProg& p, 
     if (R.status) R.ps[(int64_t)j * R.n + att] = ps;
 }
 
+// Exact early rejection of the unweighting stage, one passing point k whose
+// picked channel pc has program p: the multichannel weight
+// W = 1 / sum_j alpha_j / ps_j / vegas_j is at most the picked channel's
+// term's inverse, W_up = 1 / (alpha_pc / ps_pc / vegas_pc), computed with the
+// same expressions as the full pass (so fl(sum) >= that term and, the
+// operations being monotone, |f W| <= |f W_up| holds in floating point too).
+// The point stays a candidate (acc[a] kept at 1) only when the accept test
+// u < |f W_up| / wmax can pass; otherwise acc[a] = 0 -- the full test would
+// reject it with the same uniform u.  NaN bounds reject, as NaN values do.
+template <int NT, bool kLds>
+__device__ __forceinline__ void own_bound_one(const MCArgs& A, const ChanProg& p, int pc, int64_t k, const uint32_t* __restrict__ comp,
+                                              const double* __restrict__ mom, int64_t nmax, const uint32_t* __restrict__ sorted,
+                                              uint8_t* __restrict__ acc, double wmax, const double* search) {
+    const uint32_t sp = comp[k];
+    MomRF<2 * NT> P;
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        P.v[4 * i + 0] = mom[(int64_t)(i * 4 + 0) * nmax + sp];
+        P.v[4 * i + 1] = mom[(int64_t)(i * 4 + 1) * nmax + sp];
+        P.v[4 * i + 2] = mom[(int64_t)(i * 4 + 2) * nmax + sp];
+        P.v[4 * i + 3] = mom[(int64_t)(i * 4 + 3) * nmax + sp];
+    }
+    const double fv = integrand<NT>(A.f, P);
+    double vw;
+    const double ps = generate_weight<NT, kLds>(A, p, pc, P, &vw, nullptr, 0, nullptr, nullptr, 0, nullptr, kLds ? search : nullptr, kLds);
+    const double wup = 1.0 / (A.alpha[pc] / ps / vw);
+    const double vup = (wup == 0) ? 0.0 : fv * wup;
+    const uint32_t a = sorted[sp];
+    const bool cand = aux_uniform(A, A.a0 + a) < fabs(vup) / wmax;      // false for NaN
+    if (!cand) acc[a] = 0;
+}
+
 // Momenta of sorted position sp from the SoA columns [4 NT][nmax].
 template <int NT>
 __device__ __forceinline__ void load_mom(const double* __restrict__ mom, int64_t nmax, uint32_t sp, MomRF<2 * NT>& P) {

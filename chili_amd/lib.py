@@ -324,9 +324,9 @@ class MultiChannelGPU:
         """Switch to kernels compiled at run time with this context's channel
         programs as constants (chili_mc_specialize): True / 3 both kernels, 1
         the forward kernel, 2 the fused weight kernel, 4 the split weight pass
-        (and the learned-proposal path's per-channel geometry kernels), bits
-        combined (5: forward + split), False / 0 the generic ones.  Returns
-        the compile time."""
+        (and the learned-proposal path's per-channel geometry kernels), 8 the
+        unweighting stage's exact early rejection, bits combined (5: forward +
+        split), False / 0 the generic ones.  Returns the compile time."""
         mode = 3 if enable is True else (0 if enable is False else int(enable))
         t = C.c_double()
         check(load().chili_mc_specialize(self.h, mode, C.byref(t)))

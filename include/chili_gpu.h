@@ -239,8 +239,11 @@ int chili_mc_set_host_integrand(chili_mc* m, chili_host_integrand_fn fn, void* u
  * weight kernel, bit 2 the split weight pass (one specialised inverse kernel
  * per channel; on the learned-proposal path, one specialised kernel per
  * channel for the hypercube points and densities of the kept points; takes
- * precedence over bit 1), 0 the generic ones; compile_s receives the compile
- * time.  Same results. */
+ * precedence over bit 1), bit 3 exact early rejection in chili_mc_generate
+ * (a passing point is weighted in full only when the bound f / (alpha_c /
+ * ps_c / vegas_c) of its picked channel c can pass the accept test; the
+ * multichannel weight never exceeds it, so the events are the same), 0 the
+ * generic ones; compile_s receives the compile time.  Same results. */
 int chili_mc_specialize(chili_mc* m, int enable, double* compile_s);
 /* The generated specialised kernel source of a channel set (diagnostics),
  * and its run-time compilation without loading it (no GPU needed). */

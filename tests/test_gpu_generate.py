@@ -106,3 +106,25 @@ def test_generate_errors():
     hu = abi.host_uniforms(np.random.default_rng(1).random((gpu.ndims + 1, 100)))
     with pytest.raises(lib.ChiliError):
         gpu.generate(hu, 5, 1.0, p_ev, p_ew, 10)
+
+
+@pytest.mark.parametrize("case,modes", [("gg_3g", (0, 8)), ("gg_4g", (5, 13)), ("tt_6j", (5, 13)), ("gg_4g_kt", (1, 9))])
+def test_generate_early_rejection_same_events(case, modes):
+    """Exact early rejection (chili_mc_specialize bit 3): a passing point is
+    weighted in full only when its picked channel's bound can pass the accept
+    test.  Attempts, StatsData and the events (momenta and weights) equal
+    the stage without it bit for bit, on two sub-batch sizes of the loop."""
+    proc, ch, gpu, ora = setup(case)
+    uni = abi.philox_uniforms(SEED + 1, stream=5)
+    probe = gpu.points(uni, 1 << 20, 4000)
+    v = np.abs(probe["val"][probe["status"] == 1])
+    for q, n in ((0.9, 3000), (0.999, 20000)):
+        wmax = float(np.quantile(v, q))
+        out = []
+        for mode in modes:
+            gpu.specialize(mode)
+            out.append(generate(gpu, uni, n, wmax, a0=11))
+        (s0, n0, e0, w0), (s1, n1, e1, w1) = out
+        assert n0 == n1, f"attempts {n0} vs {n1}"
+        assert np.array_equal(stats_tuple(s0), stats_tuple(s1))
+        assert np.array_equal(e0, e1) and np.array_equal(w0, w1)
