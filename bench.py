@@ -694,6 +694,43 @@ def main():
                          vs_plain_unweighted=round(prod["unweighted_per_s"] / unw_s, 4),
                          vs_plain_unweighted_q999=round(prod["unweighted_per_s_q999"] / plain["unweighted_per_s_q999"], 4),
                          vs_plain_acceptance=round(prod["acceptance"] / plain["acceptance"], 4)))
+        # unweighted events actually generated on the learned proposal
+        # against its production w_max, without and with the exact early
+        # rejection (the latter needs the specialised kernels and no mixture)
+        if args.gen_events > 0:
+            ne = args.gen_events
+            ev = torch.empty(4 * gpu.ntot * ne, dtype=torch.float64, device="cuda:0")
+            ew = torch.empty(ne, dtype=torch.float64, device="cuda:0")
+            mode = {v: k for k, v in VARIANTS.items()}.get(variant, 0)
+            gen = {}
+            for tag, md in (("generated", mode), ("early_rejection", mode | 8)):
+                if md & 8 and (args.no_specialize or lam_best > 0):
+                    continue
+                gpu.specialize(md)
+                gpu.flow_generate(fl, abi.philox_uniforms(seed, stream=3000), 1 << 16, prod["w_max"], ctypes.c_void_p(ev.data_ptr()),
+                                  ctypes.c_void_p(ew.data_ptr()), ne)                    # warm-up
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                gst, gatt = gpu.flow_generate(fl, abi.philox_uniforms(seed, stream=3001), ne, prod["w_max"],
+                                              ctypes.c_void_p(ev.data_ptr()), ctypes.c_void_p(ew.data_ptr()), ne)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                gen[tag] = dict(events=ne, attempts=int(gatt), seconds=round(dt, 4), events_per_s=round(ne / dt, 1),
+                                sum=gst.sum)
+                log(f"  [flow generate{', early rejection' if md & 8 else ''}] {ne / dt:.3e} events/s ({gatt} attempts)")
+            gpu.specialize(mode)
+            if "generated" in gen:
+                g0 = gen["generated"]
+                prod["generated"] = dict(events=ne, attempts=g0["attempts"], seconds=g0["seconds"],
+                                         events_per_s=g0["events_per_s"], wmax=prod["w_max"],
+                                         overweight=int((ew.abs() > 1).sum()),
+                                         note="chili_mc_flow_generate: the unweighting stage on the learned proposal")
+                if "early_rejection" in gen:
+                    g1 = gen["early_rejection"]
+                    prod["generated"]["early_rejection"] = dict(
+                        seconds=g1["seconds"], events_per_s=g1["events_per_s"], attempts=g1["attempts"],
+                        same_attempts=bool(g1["attempts"] == g0["attempts"]), same_sum=bool(g1["sum"] == g0["sum"]))
+            del ev, ew
         del fl
         return prod, scan[lam_best], roof
 

@@ -155,7 +155,7 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
     // channels' search tables in LDS when they fit), the bound test on acc
     s << "extern \"C\" __global__ __launch_bounds__(kTile) void chili_k_own(MCArgs A, int64_t kept, const int* __restrict__ seg2,\n"
          "        const uint32_t* __restrict__ comp, const double* __restrict__ mom, int64_t nmax, const uint32_t* __restrict__ sorted,\n"
-         "        uint8_t* __restrict__ acc, double wmax, int lds_search) {\n"
+         "        uint8_t* __restrict__ acc, double wmax, const double* __restrict__ lqs, double mix, int lds_search) {\n"
          "    extern __shared__ double s_srch[];\n"
          "    if (lds_search) {\n"
          "        for (int i = threadIdx.x; i < A.C * A.d * A.P2; i += kTile) s_srch[i] = A.search[i];\n"
@@ -169,9 +169,9 @@ std::string kernel_source(const std::vector<ChanProg>& progs, int NT) {
     for (size_t j = 0; j < progs.size(); ++j)
         s << "        case " << j << ":\n"
              "            if (lds_search) own_bound_one<NT, true>(A, chili_prog_" << j << "(), " << j
-          << ", k, comp, mom, nmax, sorted, acc, wmax, s_srch + (int64_t)" << j << " * A.d * A.P2);\n"
+          << ", k, comp, mom, nmax, sorted, acc, wmax, lqs, mix, s_srch + (int64_t)" << j << " * A.d * A.P2);\n"
              "            else own_bound_one<NT, false>(A, chili_prog_" << j << "(), " << j
-          << ", k, comp, mom, nmax, sorted, acc, wmax, nullptr);\n"
+          << ", k, comp, mom, nmax, sorted, acc, wmax, lqs, mix, nullptr);\n"
              "            break;\n";
     s << "        default: break;\n    }\n}\n\n";
     // learned-proposal path: one kernel per channel writes r_j and a_j of

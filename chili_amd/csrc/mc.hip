@@ -823,6 +823,7 @@ struct chili_mc {
     DevBuf<short> d_prebins;     // split pass: the picked channel's bins [d][kept]
     bool lds_search = true;      // weight kernel: search tables in LDS when they fit
     double gen_rate = 0.01;      // unweighting stage: kept events per attempt (running estimate)
+    double flow_gen_rate = 0.01; // the same on the learned-proposal path
     double jit_compile_s = 0;
     // k_main timing with HIP events on the context's stream
     bool timing = false;
@@ -1233,11 +1234,37 @@ int pick_sort_forward(chili_mc* m, const MCArgs& A, int64_t n, const PointRec& R
     return tgrid;
 }
 
+void gen_tail(chili_mc* m, const MCArgs& A, int64_t n, int nblk, int64_t need, double wmax, double* ev, double* ew, int64_t cap,
+              int64_t got, int64_t passed, int64_t& kept, int64_t& used);
+
+// Exact early rejection (own_bound_one) over the passing points of the
+// current compaction: a point whose picked channel's bound cannot pass the
+// accept test loses its flag.  lqs / mix: the learned proposal's sampling
+// log-densities and mixture weight (null / 0 on the plain path).
+void launch_early_reject(chili_mc* m, const MCArgs& A, int64_t passed, double wmax, const double* lqs, double mix) {
+    constexpr size_t kOwnLdsMax = 32 * 1024;
+    const size_t tbl = sizeof(double) * (size_t)m->C * m->d * m->P2;
+    int lds_search = (m->lds_search && tbl <= kOwnLdsMax) ? 1 : 0;
+    MCArgs a = A;
+    int64_t kp = passed, nmax = m->nmax;
+    const int* seg2 = m->d_seg2.ptr;
+    const uint32_t* comp = m->d_comp.ptr;
+    const double* mom = m->d_mom.ptr;
+    const uint32_t* sorted = m->d_sorted.ptr;
+    uint8_t* acc = m->d_acc.ptr;
+    double wm = wmax, mx = mix;
+    const double* lq = lqs;
+    void* args[] = {&a, &kp, &seg2, &comp, &mom, &nmax, &sorted, &acc, &wm, &lq, &mx, &lds_search};
+    HIP_CHECK(hipModuleLaunchKernel(m->jit->own, (unsigned)((passed + kTile - 1) / kTile), 1, 1, kTile, 1, 1,
+                                    lds_search ? (unsigned)tbl : 0u, m->stream, args, nullptr));
+}
+
 // One sub-batch of the unweighting stage over attempts [a0, a0 + n): every
-// point passing the cut is weighted (production: no training sums), the
-// accept / reject decisions are made per attempt, the cutoff is the attempt
-// of the need-th accepted event; StatsData of the accepted events goes to
-// d_tot, the events to ev / ew from column got.
+// point passing the cut is weighted (production: no training sums; with
+// early rejection only the candidates), the accept / reject decisions are
+// made per attempt, the cutoff is the attempt of the need-th accepted event;
+// StatsData of the accepted events goes to d_tot, the events to ev / ew
+// from column got.
 void run_subbatch_gen(chili_mc* m, uint64_t seed, uint32_t stream, uint64_t a0, int64_t n, int64_t need, double wmax, double* ev,
                       double* ew, int64_t cap, int64_t got, int64_t& kept, int64_t& used) {
     ensure_capacity(m, n);
@@ -1248,29 +1275,12 @@ void run_subbatch_gen(chili_mc* m, uint64_t seed, uint32_t stream, uint64_t a0, 
     int64_t passed = 0, all = 0;
     cutoff_and_compact(m, n, nblk, tgrid, std::numeric_limits<int64_t>::max(), passed, all);
     if (m->jit_early && passed > 0) {
-        // exact early rejection: the points whose bound cannot pass the
-        // accept test lose their flag; the rest are compacted again and
-        // weighted in full (the same decisions as weighting every point)
-        constexpr size_t kOwnLdsMax = 32 * 1024;
-        const size_t tbl = sizeof(double) * (size_t)m->C * m->d * m->P2;
-        int lds_search = (m->lds_search && tbl <= kOwnLdsMax) ? 1 : 0;
-        MCArgs a = A;
-        int64_t kp = passed, nmax = m->nmax;
-        const int* seg2 = m->d_seg2.ptr;
-        const uint32_t* comp = m->d_comp.ptr;
-        const double* mom = m->d_mom.ptr;
-        const uint32_t* sorted = m->d_sorted.ptr;
-        uint8_t* acc = m->d_acc.ptr;
-        double wm = wmax;
-        void* args[] = {&a, &kp, &seg2, &comp, &mom, &nmax, &sorted, &acc, &wm, &lds_search};
-        HIP_CHECK(hipModuleLaunchKernel(m->jit->own, (unsigned)((passed + kTile - 1) / kTile), 1, 1, kTile, 1, 1,
-                                        lds_search ? (unsigned)tbl : 0u, m->stream, args, nullptr));
+        // the points whose bound cannot pass the accept test lose their
+        // flag; the rest are compacted again and weighted in full (the same
+        // decisions as weighting every point)
+        launch_early_reject(m, A, passed, wmax, nullptr, 0.0);
         cutoff_and_compact(m, n, nblk, tgrid, std::numeric_limits<int64_t>::max(), passed, all);
     }
-    m->d_gflag.resize(n);
-    m->d_gv.resize(n);
-    m->d_gpos.resize(n);
-    hipLaunchKernelGGL(chili_nf::k_fill<uint8_t>, dim3(nblk), dim3(kTile), 0, m->stream, n, (uint8_t)0, m->d_gflag.ptr);
     if (passed > 0) {
         m->d_gval.resize(passed);
         const int mgrid = (int)((passed + kMainTile - 1) / kMainTile) + C;
@@ -1285,9 +1295,23 @@ void run_subbatch_gen(chili_mc* m, uint64_t seed, uint32_t stream, uint64_t a0, 
             m->ev_live.push_back(evt);
             m->ev_points.push_back(passed);
         }
+    }
+    gen_tail(m, A, n, nblk, need, wmax, ev, ew, cap, got, passed, kept, used);
+}
+
+// The unweighting decisions of the passing points of the current compaction
+// (values in d_gval, compaction order), the cutoff at the need-th accepted
+// event, StatsData of the accepted events into d_tot and the events to
+// ev / ew from column got.
+void gen_tail(chili_mc* m, const MCArgs& A, int64_t n, int nblk, int64_t need, double wmax, double* ev, double* ew, int64_t cap,
+              int64_t got, int64_t passed, int64_t& kept, int64_t& used) {
+    m->d_gflag.resize(n);
+    m->d_gv.resize(n);
+    m->d_gpos.resize(n);
+    hipLaunchKernelGGL(chili_nf::k_fill<uint8_t>, dim3(nblk), dim3(kTile), 0, m->stream, n, (uint8_t)0, m->d_gflag.ptr);
+    if (passed > 0)
         hipLaunchKernelGGL(k_unw, dim3((unsigned)((passed + kTile - 1) / kTile)), dim3(kTile), 0, m->stream, A, passed, m->d_comp.ptr,
                            m->d_sorted.ptr, m->d_gval.ptr, wmax, m->d_gflag.ptr, m->d_gv.ptr, m->d_gpos.ptr);
-    }
     find_cutoff(m, m->d_gflag.ptr, n, nblk, need, kept, used);
     if (got + kept > cap) throw chili_error(CHILI_ERANGE, "event buffer too small");
     const int ublk = (int)((used + kTile - 1) / kTile);
@@ -1470,9 +1494,19 @@ void launch_geom(chili_mc* m, const MCArgs& A, int64_t kept) {
 
 // One sub-batch of the learned-proposal path (fl == null: every q_j = 1,
 // the plain multichannel weight through the same kernels).
+// The unweighting stage's outputs on the learned-proposal path (see
+// run_subbatch_gen): accept test against wmax, events to ev / ew from column
+// got.
+struct GenOut {
+    double wmax;
+    double* ev;
+    double* ew;
+    int64_t cap, got;
+};
+
 void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ustride, uint64_t ua0, uint64_t seed, uint32_t stream,
                        uint64_t a0, int64_t n, int64_t need, double* vals, int64_t vals_off, int64_t vals_cap, FlowData FD,
-                       int64_t& kept, int64_t& used, const PointRec& R = PointRec{}) {
+                       int64_t& kept, int64_t& used, const PointRec& R = PointRec{}, const GenOut* gen = nullptr) {
     ensure_capacity(m, n);
     ensure_flow_capacity(m, n);
     const int C = m->C, d = m->d;
@@ -1524,11 +1558,33 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
         m->fev_live.push_back(fev);
     }
     launch_ktcut(m, n, R);
-    cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
-    if (kept == 0) return;
-    CHILI_DISPATCH_NT(m->NT, launch_geom<NT>(m, A, kept));
     // own ranges need the [C][fmax] row layout within the int offsets of a pass
     const bool own = fl && (int64_t)C * m->fmax < (int64_t)INT32_MAX;
+    if (gen) {
+        // unweighting stage: every passing point is a candidate, or with
+        // early rejection those whose picked channel's bound (its sampling
+        // density, which the combination uses as is when the own ranges are
+        // in use and no mixture is) can pass the accept test
+        int64_t all = 0;
+        cutoff_and_compact(m, n, nblk, tgrid, std::numeric_limits<int64_t>::max(), kept, all);
+        if (m->jit_early && own && !mixing && kept > 0) {
+            launch_early_reject(m, A, kept, gen->wmax, m->d_lqs.ptr, 0.0);
+            cutoff_and_compact(m, n, nblk, tgrid, std::numeric_limits<int64_t>::max(), kept, all);
+        }
+        m->d_gval.resize(std::max<int64_t>(kept, 1));
+        vals = m->d_gval.ptr;
+        vals_off = 0;
+        vals_cap = kept;
+        if (kept == 0) {
+            const int64_t passed = 0;
+            gen_tail(m, A, n, nblk, need, gen->wmax, gen->ev, gen->ew, gen->cap, gen->got, passed, kept, used);
+            return;
+        }
+    } else {
+        cutoff_and_compact(m, n, nblk, tgrid, need, kept, used);
+        if (kept == 0) return;
+    }
+    CHILI_DISPATCH_NT(m->NT, launch_geom<NT>(m, A, kept));
     if (fl) {
         // log q_j(r_j) of every kept point and channel; the flows move a copy
         // of r_j (the training records keep the originals).  The picked
@@ -1559,6 +1615,13 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
                        fl ? fl->eng.mix : 0.0,
                        m->d_fbuf.ptr, m->d_rall.ptr, m->d_fpart.ptr, vals, vals_off, vals_cap, FD, m->d_comp.ptr, m->d_sorted.ptr, R,
                        own ? m->d_own.ptr : nullptr, m->d_lqs.ptr);
+    if (gen) {
+        // the candidates' values (compaction order) decide; StatsData is the
+        // accepted events'
+        const int64_t passed = kept;
+        gen_tail(m, A, n, nblk, need, gen->wmax, gen->ev, gen->ew, gen->cap, gen->got, passed, kept, used);
+        return;
+    }
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, m->stream, (int64_t)cg, m->d_fpart.ptr, m->d_tot.ptr);
 }
 
@@ -2454,6 +2517,35 @@ int chili_mc_flow_iterate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni
                          run_subbatch_flow(m, fl, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, vals_dev, got, vals_cap,
                                            FlowData{nullptr, nullptr, nullptr, 0, 0}, kept, used);
                      });
+        if (n_attempts) *n_attempts = att;
+    });
+}
+
+// The unweighting stage on the learned proposal (chili_mc_generate's loop
+// and outputs, the weights being the flow path's).  With the early
+// rejection of chili_mc_specialize bit 3 and no mixture, the other
+// channels' maps and densities are evaluated only for the points whose
+// picked channel's term can pass the accept test.
+int chili_mc_flow_generate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t nevents, double wmax,
+                           double* ev_dev, double* ew_dev, int64_t cap, chili_stats* stats, int64_t* n_attempts) {
+    return guarded([&] {
+        if (!m || !uni || !stats || !ev_dev || !ew_dev) throw chili_error(CHILI_EINVAL, "null argument");
+        if (nevents <= 0) throw chili_error(CHILI_EINVAL, "nevents must be positive");
+        if (!(wmax > 0) || !std::isfinite(wmax)) throw chili_error(CHILI_EINVAL, "wmax must be positive and finite");
+        if (cap < nevents) throw chili_error(CHILI_ERANGE, "event buffer too small");
+        if (uni->u) throw chili_error(CHILI_EINVAL, "the unweighting stage needs device-generated (Philox) uniforms");
+        check_flow(m, fl);
+        no_host_callback(m, "chili_mc_flow_generate");
+        select_device(m->device);
+        std::vector<double> tot;
+        int64_t att = 0;
+        iterate_loop(m, uni, a0, nevents, kFlowSubMax, tot, att, stats,
+                     [&](const double*, int64_t, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept, int64_t& used) {
+                         const GenOut g{wmax, ev_dev, ew_dev, cap, got};
+                         run_subbatch_flow(m, fl, nullptr, 0, ab, uni->seed, uni->stream, ab, n, need, nullptr, 0, 0,
+                                           FlowData{nullptr, nullptr, nullptr, 0, 0}, kept, used, PointRec{}, &g);
+                     },
+                     &m->flow_gen_rate, 1e-4);
         if (n_attempts) *n_attempts = att;
     });
 }

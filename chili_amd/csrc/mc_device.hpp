@@ -390,13 +390,18 @@ __device__ __forceinline__ void inverse_one(const MCArgs& A, const ChanProg& p, 
 // term's inverse, W_up = 1 / (alpha_pc / ps_pc / vegas_pc), computed with the
 // same expressions as the full pass (so fl(sum) >= that term and, the
 // operations being monotone, |f W| <= |f W_up| holds in floating point too).
-// The point stays a candidate (acc[a] kept at 1) only when the accept test
-// u < |f W_up| / wmax can pass; otherwise acc[a] = 0 -- the full test would
-// reject it with the same uniform u.  NaN bounds reject, as NaN values do.
+// On the learned-proposal path the picked channel's term carries its
+// proposal density, (alpha_pc / ps_pc / vegas_pc) ((1 - mix) exp(lqs) + mix)
+// with lqs the sampling pass's log-density of the point (the same
+// expression and value the combination uses for it).  The point stays a
+// candidate (acc[a] kept at 1) only when the accept test u < |f W_up| / wmax
+// can pass; otherwise acc[a] = 0 -- the full test would reject it with the
+// same uniform u.  NaN bounds reject, as NaN values do.
 template <int NT, bool kLds>
 __device__ __forceinline__ void own_bound_one(const MCArgs& A, const ChanProg& p, int pc, int64_t k, const uint32_t* __restrict__ comp,
                                               const double* __restrict__ mom, int64_t nmax, const uint32_t* __restrict__ sorted,
-                                              uint8_t* __restrict__ acc, double wmax, const double* search) {
+                                              uint8_t* __restrict__ acc, double wmax, const double* __restrict__ lqs, double mix,
+                                              const double* search) {
     const uint32_t sp = comp[k];
     MomRF<2 * NT> P;
 #pragma unroll
@@ -409,7 +414,8 @@ __device__ __forceinline__ void own_bound_one(const MCArgs& A, const ChanProg& p
     const double fv = integrand<NT>(A.f, P);
     double vw;
     const double ps = generate_weight<NT, kLds>(A, p, pc, P, &vw, nullptr, 0, nullptr, nullptr, 0, nullptr, kLds ? search : nullptr, kLds);
-    const double wup = 1.0 / (A.alpha[pc] / ps / vw);
+    const double term = A.alpha[pc] / ps / vw;
+    const double wup = 1.0 / (lqs ? term * ((1.0 - mix) * exp(lqs[sp]) + mix) : term);
     const double vup = (wup == 0) ? 0.0 : fv * wup;
     const uint32_t a = sorted[sp];
     const bool cand = aux_uniform(A, A.a0 + a) < fabs(vup) / wmax;      // false for NaN

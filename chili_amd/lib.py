@@ -24,7 +24,7 @@ EXPORTS = [
     "chili_mc_state_write", "chili_mc_optimize", "chili_vegas_optimize", "chili_mc_params_default",
     "chili_mc_summary_init", "chili_vegas_params_default",
     "chili_grid_adapt", "chili_grid_split", "chili_alpha_adapt", "chili_max_difference",
-    "chili_mc_flow_iterate", "chili_mc_flow_data", "chili_mc_flow_points",
+    "chili_mc_flow_iterate", "chili_mc_flow_generate", "chili_mc_flow_data", "chili_mc_flow_points",
     "chili_flow_create", "chili_flow_destroy", "chili_flow_info", "chili_flow_mask", "chili_flow_get_params",
     "chili_flow_set_params", "chili_flow_get_grads", "chili_flow_log_density", "chili_flow_sample", "chili_flow_loss_grad",
     "chili_flow_train", "chili_flow_timing",
@@ -102,6 +102,8 @@ def load():
     L.chili_max_difference.argtypes = [C.c_int, D]
     VP, F = C.c_void_p, C.POINTER(C.c_float)
     L.chili_mc_flow_iterate.argtypes = [P, P, C.POINTER(abi.Uniforms), U64, I64, C.POINTER(abi.Stats), C.POINTER(C.c_int64), VP, I64]
+    L.chili_mc_flow_generate.argtypes = [P, P, C.POINTER(abi.Uniforms), U64, I64, C.c_double, VP, VP, I64, C.POINTER(abi.Stats),
+                                         C.POINTER(C.c_int64)]
     L.chili_mc_flow_data.argtypes = [P, P, C.POINTER(abi.Uniforms), U64, I64, VP, VP, VP, I64, C.POINTER(abi.Stats),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.chili_mc_flow_points.argtypes = [P, P, C.POINTER(abi.Uniforms), U64, I64, I32, I32, D, D, D, D, D]
@@ -359,6 +361,16 @@ class MultiChannelGPU:
         na = C.c_int64()
         check(load().chili_mc_flow_iterate(self.h, flow.h if flow else None, C.byref(uni), a0, ncalls, C.byref(st), C.byref(na),
                                            vals_ptr, vals_cap))
+        return st, na.value
+
+    def flow_generate(self, flow, uni, nevents, wmax, ev_ptr, ew_ptr, cap, a0=0):
+        """Unweighted events on the learned proposal (chili_mc_flow_generate):
+        device buffers ev [4 ntot][cap], ew [cap] (raw pointers); returns
+        (stats, attempts)."""
+        st = abi.Stats()
+        na = C.c_int64()
+        check(load().chili_mc_flow_generate(self.h, flow.h if flow else None, C.byref(uni), a0, nevents, wmax, ev_ptr, ew_ptr, cap,
+                                            C.byref(st), C.byref(na)))
         return st, na.value
 
     def flow_data(self, flow, uni, ncalls, r_ptr, beta_ptr, w2_ptr, cap, a0=0):

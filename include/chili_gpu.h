@@ -473,6 +473,16 @@ int chili_flow_train(chili_flow* f, int64_t n, const double* r_dev, const double
 /* multichannel iteration with the flows' proposals (fl may be NULL: q = 1) */
 int chili_mc_flow_iterate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, chili_stats* stats,
                           int64_t* n_attempts, double* vals_dev, int64_t vals_cap);
+/* The unweighting stage on the learned proposal: chili_mc_generate's loop,
+ * accept rule and outputs (events ev_dev [4 ntot][cap], weights ew_dev
+ * [cap], StatsData of the accepted events, attempts) with the weights of
+ * chili_mc_flow_iterate.  No reference counterpart beyond
+ * chili_mc_generate's (MultiChannel.hh:240-245 with the learned proposal).
+ * With chili_mc_specialize bit 3 and no mixture the other channels' maps
+ * and flow densities are evaluated only for the points whose picked
+ * channel's term can pass the accept test; the events are the same. */
+int chili_mc_flow_generate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t nevents, double wmax,
+                           double* ev_dev, double* ew_dev, int64_t cap, chili_stats* stats, int64_t* n_attempts);
 /* training records drawn from the current proposal (device outputs) */
 int chili_mc_flow_data(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, uint64_t a0, int64_t ncalls, double* r_dev,
                        double* beta_dev, double* w2_dev, int64_t cap, chili_stats* stats, int64_t* n_kept, int64_t* n_attempts);

@@ -2,6 +2,7 @@
 flow's density / sampling / gradients against the CPU restatement
 (oracle/flow_oracle.py), and the multichannel path with flows against the
 plain path and the oracle."""
+import ctypes
 import os
 
 import numpy as np
@@ -12,6 +13,7 @@ from chili_amd import abi, lib
 from chili_amd import processes as P
 from oracle import flow_oracle as FO
 from oracle import oracle as O
+from tests.helpers import assert_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -533,3 +535,63 @@ def test_flow_path_errors():
     st, na = gpu.flow_iterate(fl, uni, 1000)
     assert na >= 1000 and st.n == na
 
+
+
+def flow_generate(g, fl, uni, n, wmax, a0):
+    ev = torch.empty(4 * g.ntot * n, dtype=torch.float64, device="cuda")
+    ew = torch.empty(n, dtype=torch.float64, device="cuda")
+    st, na = g.flow_generate(fl, uni, n, wmax, ctypes.c_void_p(ev.data_ptr()), ctypes.c_void_p(ew.data_ptr()), n, a0=a0)
+    torch.cuda.synchronize()
+    return st, na, ev.view(g.ntot * 4, n).cpu().numpy(), ew.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", [0, 13])
+def test_flow_generate_vs_records(mode):
+    """The unweighting stage on the learned proposal (chili_mc_flow_generate)
+    is its definition applied to the flow path's per-attempt records: an
+    attempt that passes is kept when its uniform d + 1 is below |v| / wmax,
+    the first n kept are the events, weights sign(v) max(1, |v| / wmax), and
+    StatsData sums the kept values over all attempts used.  Mode 13 adds the
+    exact early rejection to the specialised kernels."""
+    _, _, _, g = adapted_context()
+    if mode:
+        g.specialize(mode)
+    fl, _ = make_flow(g.ndims, 4, g.nchannels, 0.05, seed=31)
+    uni = abi.philox_uniforms(SEED + 70, stream=4)
+    a0, span = 5, 120000
+    rec = g.flow_points(fl, uni, a0, span)
+    ok = rec["status"] == 1
+    wmax = float(np.quantile(np.abs(rec["val"][ok]), 0.95))
+    u = np.array([lib.philox_uniform(uni.seed, uni.stream, a0 + i, g.ndims + 1) for i in range(span)])
+    keep = ok & (u < np.abs(rec["val"]) / wmax)
+    n = 2000
+    idx = np.flatnonzero(keep)[:n]
+    assert len(idx) == n, "record span too short"
+    st, na, ev, ew = flow_generate(g, fl, uni, n, wmax, a0)
+    assert na == idx[-1] + 1, f"attempts {na} vs {idx[-1] + 1}"
+    v = rec["val"][idx]
+    assert np.array_equal(ew, np.where(np.abs(v) > wmax, v / wmax, np.copysign(1.0, v)))
+    assert st.n == na and st.n_finite == np.count_nonzero(v)
+    assert_rel(np.array([st.sum, st.sum2]), np.array([v.sum(), (v * v).sum()]), 1e-12, what="stats")
+
+
+def test_flow_generate_early_rejection_same_events():
+    """With the exact early rejection the flow path weights in full only the
+    points whose picked channel's term can pass the accept test; attempts,
+    StatsData, event momenta and weights equal the stage without it bit for
+    bit (specialised kernels in both, so only the rejection differs)."""
+    _, _, _, g = adapted_context("gg_4g")
+    fl, _ = make_flow(g.ndims, 4, g.nchannels, 0.05, seed=37)
+    uni = abi.philox_uniforms(SEED + 71, stream=6)
+    probe = g.flow_points(fl, uni, 1 << 20, 20000)
+    v = np.abs(probe["val"][probe["status"] == 1])
+    for q, n in ((0.9, 3000), (0.995, 12000)):
+        wmax = float(np.quantile(v, q))
+        out = []
+        for mode in (5, 13):
+            g.specialize(mode)
+            out.append(flow_generate(g, fl, uni, n, wmax, 17))
+        (s0, n0, e0, w0), (s1, n1, e1, w1) = out
+        assert n0 == n1, f"attempts {n0} vs {n1}"
+        assert (s0.n, s0.min, s0.max, s0.sum, s0.sum2, s0.n_finite) == (s1.n, s1.min, s1.max, s1.sum, s1.sum2, s1.n_finite)
+        assert np.array_equal(e0, e1) and np.array_equal(w0, w1)
