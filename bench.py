@@ -12,14 +12,20 @@ Procedure (the 2->6 resonant workload by default):
      forward map + cut, every channel's inverse map + VEGAS density,
      multichannel weight, statistics), values kept in HBM;
   4. eps_unw = <w> / w_max over the production sample (also with w_max = the
-     99.9% quantile), unweighted events/s = accepted points/s x eps_unw;
+     99.9% quantile), derived unweighted events/s = accepted points/s x eps_unw;
   5. flow leg: a coupling-layer spline flow per channel is trained on points
      and weights the frozen plain proposal generates, the training epoch is
      chosen on a validation sample and the defensive-mixture weight on a
      selection sample (their own Philox streams), then the production steps
-     run through the flow-assisted path.  Plain multichannel VEGAS is the
-     fallback: the leg with more unweighted events/s on the selection sample
-     is reported as `value`; both legs' production figures are reported.
+     run through the flow-assisted path;
+  6. the headline: each leg's unweighting stage (chili_mc_generate /
+     chili_mc_flow_generate, with the exact early rejection of
+     chili_mc_specialize bit 3: the same events as weighting every point)
+     generates unweighted events against its production w_max; the leg with
+     more events/s on a selection pass (plain multichannel VEGAS the
+     fallback) is timed over `--warmup` + `--steps` passes, one step = one
+     pass of 2^22 events, and `value` = generated events/s.  Both legs'
+     production and generation figures are reported.
 The roofline entry prices the dominant plain kernel (k_main, the all-channel
 inverse/weight kernel) by its algorithmic FP64 flop-equivalents per point
 (N_arith + 20 N_trans, counted by the op-counting CPU oracle) over its
@@ -507,8 +513,11 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         atts = 0
+        step_ms = []
         for s in range(steps):
+            ts = time.perf_counter()
             atts += step_fn(abi.philox_uniforms(seed, stream=stream0 + s), ctypes.c_void_p(vals.data_ptr() + 8 * n * s), n)
+            step_ms.append(round((time.perf_counter() - ts) * 1e3, 2))   # each step ends synchronised
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         w = vals[: n * steps]
@@ -522,7 +531,7 @@ def main():
                    ms_per_step=round(dt / steps * 1e3, 3), w_max=wmax, w_mean=wmean,
                    w_rel_std=float(w.std()) / wmean, acceptance=n * steps / atts,
                    integral=wmean * n * steps / atts, steps=steps, first_stream=stream0,
-                   eps_unw_per_step=[round(float(e), 5) for e in eps_steps])
+                   eps_unw_per_step=[round(float(e), 5) for e in eps_steps], step_ms=step_ms)
         log(f"  [{tag}] {pts_s:.3e} points/s, eps {res['eps_unw']:.4f} (q999 {res['eps_unw_q999']:.4f}, per step "
             f"{eps_steps.min():.4f}..{eps_steps.max():.4f}), {res['unweighted_per_s']:.3e} unweighted/s, rel std "
             f"{res['w_rel_std']:.3f}, {res['ms_per_step']:.2f} ms/step")
@@ -731,17 +740,65 @@ def main():
                         seconds=g1["seconds"], events_per_s=g1["events_per_s"], attempts=g1["attempts"],
                         same_attempts=bool(g1["attempts"] == g0["attempts"]), same_sum=bool(g1["sum"] == g0["sum"]))
             del ev, ew
-        del fl
-        return prod, scan[lam_best], roof
 
+        def gen_fn(stream, ne, ev_ptr, ew_ptr):
+            return gpu.flow_generate(fl, abi.philox_uniforms(seed, stream=stream), ne, prod["w_max"], ev_ptr, ew_ptr, ne)
+
+        return prod, scan[lam_best], roof, gen_fn
+
+    flow_gen = None
     if not args.no_flow:
         # a failing flow leg (e.g. training that diverged) is reported in
         # the bench line; the plain leg still stands
         try:
-            flow, flow_sel, roof_flow = flow_leg()
+            flow, flow_sel, roof_flow, flow_gen = flow_leg()
         except (lib.ChiliError, RuntimeError, ValueError, ZeroDivisionError) as e:
             log(f"  flow leg failed: {e}")
             flow = dict(error=str(e))
+
+    # ---- the headline: unweighted events actually generated ----
+    # each leg's unweighting stage against its production w_max (with the
+    # exact early rejection when the specialised kernels are in use: the
+    # same events as weighting every point); the leg with more events/s on a
+    # selection pass (its own streams) is timed over --warmup + --steps
+    # passes of `ne_step` events each (production streams)
+    flow_ok = bool(flow and "unweighted_per_s" in flow)
+    ne_step = max(1 << 16, args.gen_events // 2)
+    mode = {v: k for k, v in VARIANTS.items()}.get(variant, 0)
+    if not args.no_specialize:
+        gpu.specialize(mode | 8)
+    legs = {"plain": lambda st, ev_p, ew_p: gpu.generate(abi.philox_uniforms(seed, stream=st), ne_step, wmax, ev_p, ew_p, ne_step)}
+    if flow_ok and flow_gen is not None:
+        legs["flow"] = lambda st, ev_p, ew_p: flow_gen(st, ne_step, ev_p, ew_p)
+    ev = torch.empty(4 * gpu.ntot * ne_step, dtype=torch.float64, device="cuda:0")
+    ew = torch.empty(ne_step, dtype=torch.float64, device="cuda:0")
+    evp, ewp = ctypes.c_void_p(ev.data_ptr()), ctypes.c_void_p(ew.data_ptr())
+    gen_sel = {}
+    for leg, fn in legs.items():
+        fn(3050, evp, ewp)                                  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(S_SEL + 100, evp, ewp)
+        torch.cuda.synchronize()
+        gen_sel[leg] = round(ne_step / (time.perf_counter() - t0), 1)
+    gen_leg = max(gen_sel, key=gen_sel.get)
+    fn = legs[gen_leg]
+    for s_ in range(args.warmup):
+        fn(S_PROD + 1100 + s_, evp, ewp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gen_atts, gen_ms = 0, []
+    for s_ in range(args.steps):
+        ts = time.perf_counter()
+        _, a_ = fn(S_PROD + 1200 + s_, evp, ewp)
+        gen_atts += a_
+        gen_ms.append(round((time.perf_counter() - ts) * 1e3, 2))
+    torch.cuda.synchronize()
+    gen_dt = time.perf_counter() - t0
+    gen_value = ne_step * args.steps / gen_dt
+    log(f"  [headline] {gen_leg}: {gen_value:.3e} unweighted events/s generated ({args.steps} passes of {ne_step} events; "
+        f"selection {gen_sel})")
+    del ev, ew
 
     # ---- roofline of k_main ----
     ora = O.Oracle(proc, gpu.nbins, f, count=True)
@@ -787,12 +844,12 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(proc, f, gpu.grids(), gpu.alpha(), seed, args.cpu_seconds, eps, args.workload)
 
-    # plain multichannel VEGAS is the fallback: the flow leg is selected only
-    # when it yields more unweighted events/s on the selection sample
-    flow_ok = bool(flow and "unweighted_per_s" in flow)
+    # the production samples' derived rates (accepted points/s x <w>/w_max)
+    # name the leg the integration would pick; the headline is the leg that
+    # generates more events on the selection pass (plain VEGAS the fallback)
     selected = "flow" if (flow_ok and flow_sel["unweighted_per_s"] > plain_sel["unweighted_per_s"]) else "plain"
-    value = flow["unweighted_per_s"] if selected == "flow" else unw_s
-    ms_step = flow["ms_per_step"] if selected == "flow" else plain["ms_per_step"]
+    value = gen_value
+    ms_step = gen_dt / args.steps * 1e3
     recipe = None
     if flow_ok:
         recipe = dict(layers=W["flow_layers"], hidden="3x64 ReLU", spline="rational-quadratic, 16 bins",
@@ -809,19 +866,31 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (Philox4x32-10 uniforms)",
         "config": {"workload": args.workload, "description": W["desc"], "channels": gpu.nchannels, "dims": gpu.ndims,
-                   "bins": gpu.nbins, "points_per_step": n, "selected_leg": selected,
-                   "selection": dict(rule="leg and mixture weight with more unweighted events/s on a selection sample "
-                                          f"({args.sel_steps} steps, Philox streams {S_SEL}..); value = that leg's production "
-                                          f"sample (streams {S_PROD}..)",
+                   "bins": gpu.nbins, "points_per_step": n, "events_per_step": ne_step, "selected_leg": gen_leg,
+                   "step": f"one pass of the selected leg's unweighting stage: {ne_step} unweighted events against its "
+                           "production w_max (forward map, cut, weights of the candidates, accept test, events and "
+                           "weights written to HBM)",
+                   "selection": dict(rule="value: the leg with more generated events/s on a selection pass (Philox "
+                                          f"streams {S_SEL + 100}), timed over the production passes (streams "
+                                          f"{S_PROD + 1200}..); derived rates (accepted points/s x <w>/w_max) of the "
+                                          f"production samples under plain / flow, the mixture weight chosen on a "
+                                          f"selection sample ({args.sel_steps} steps, streams {S_SEL}..)",
+                                     generated_selection_events_per_s=gen_sel,
+                                     derived_selected_leg=selected,
                                      plain_selection_unweighted_per_s=plain_sel["unweighted_per_s"],
                                      flow_selection_unweighted_per_s=flow_sel["unweighted_per_s"] if flow_sel else None),
+                   "early_rejection": not args.no_specialize,
                    "kernel_variant": variant, "specialize_compile_s": compile_s, "flow_recipe": recipe},
-        # the selected leg's rate and efficiency (both legs in full below)
+        # the headline leg's generation passes, then its production
+        # figures (both legs in full below)
+        "generated": dict(leg=gen_leg, events=ne_step * args.steps, attempts=int(gen_atts), seconds=round(gen_dt, 4),
+                          pass_ms=gen_ms),
         **(dict(points_per_s=flow["points_per_s"], eps_unw=flow["eps_unw"], eps_unw_q999=flow["eps_unw_q999"],
-                w_max=flow["w_max"], w_mean=flow["w_mean"], attempts_per_step=n / flow["acceptance"])
-           if selected == "flow" else
+                w_max=flow["w_max"], w_mean=flow["w_mean"], attempts_per_step=n / flow["acceptance"],
+                derived_unweighted_per_s=flow["unweighted_per_s"])
+           if gen_leg == "flow" else
            dict(points_per_s=round(pts_s, 1), eps_unw=eps, eps_unw_q999=eps999, w_max=wmax, w_mean=wmean,
-                attempts_per_step=atts / args.steps)),
+                attempts_per_step=atts / args.steps, derived_unweighted_per_s=unw_s)),
         "adaptation": ad, "roofline": roof, "roofline_flow": roof_flow, "cpu_baseline": cpu, "plain": plain, "flow": flow,
         "configs": configs,
         # the CPU baseline runs the plain algorithm: its points/s against the plain leg's
