@@ -676,18 +676,72 @@ __global__ __launch_bounds__(kTile) void k_unw(MCArgs A, int64_t kept, const uin
     gpos[a] = sp;
 }
 
-// StatsData partial of the accepted events among attempts [0, used) of one
-// block of kTile attempts: [n, min, max, sum, sum2, nf, nnan].
-__global__ __launch_bounds__(kTile) void k_gen_stats(int64_t used, const uint8_t* __restrict__ flag, const double* __restrict__ gv,
-                                                     double* __restrict__ part) {
+// Accepted events among attempts [0, used) in attempt order, one workgroup
+// per kGenSub consecutive blocks of kTile attempts: momenta of the external
+// legs, SoA ev[(i*4+k)][cap] from column base + blkoff[first block] + the
+// event's rank in the workgroup, the event weight sign(v) max(1, |v| / wmax),
+// and the workgroup's StatsData partial of the accepted values [n, min, max,
+// sum, sum2, nf, nnan] (each lane accumulates its attempts in order, then
+// lanes and waves in order).  The momenta are copied by the whole workgroup
+// (one double per thread and step, rows of consecutive columns).
+constexpr int kGenSub = 16;
+template <int NT>
+__global__ __launch_bounds__(kTile) void k_gen_out(int64_t used, int nblk, const uint8_t* __restrict__ flag,
+                                                   const int* __restrict__ blkoff, const uint32_t* __restrict__ gpos,
+                                                   const double* __restrict__ gv, double wmax, const double* __restrict__ mom,
+                                                   int64_t nmax, double* __restrict__ ev, double* __restrict__ ew, int64_t cap,
+                                                   int64_t base, double* __restrict__ part) {
+    __shared__ int s_cnt[kGenSub][kWaves];
+    __shared__ int s_pre[kGenSub + 1];
+    __shared__ uint32_t s_sp[kGenSub * kTile];
     __shared__ double s_part[kWaves][kStatW];
-    const int64_t a = (int64_t)blockIdx.x * kTile + threadIdx.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const bool ok = a < used && flag[a];
-    const double v = ok ? gv[a] : 0.0;
-    const double n = wave_sum(ok ? 1.0 : 0.0), s1 = wave_sum(v), s2 = wave_sum(v * v);
-    const double nf = wave_sum((ok && v != 0) ? 1.0 : 0.0);
-    const double mn = wave_min(ok ? v : __DBL_MAX__), mx = wave_max(ok ? v : __DBL_MIN__);
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    const int b0 = blockIdx.x * kGenSub;
+    const int nb = min(kGenSub, nblk - b0);
+    uint64_t masks[kGenSub];
+#pragma unroll
+    for (int t = 0; t < kGenSub; ++t) {
+        const int64_t i = (int64_t)(b0 + t) * kTile + threadIdx.x;
+        masks[t] = __ballot(t < nb && i < used && flag[i]);
+        if (lane == 0) s_cnt[t][w] = __popcll(masks[t]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < kGenSub; ++t) {
+            s_pre[t] = run;
+            for (int ww = 0; ww < kWaves; ++ww) run += s_cnt[t][ww];
+        }
+        s_pre[kGenSub] = run;
+    }
+    __syncthreads();
+    const int64_t col0 = base + blkoff[b0];
+    double a[kStatW] = {0, __DBL_MAX__, __DBL_MIN__, 0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < kGenSub; ++t) {
+        if (!((masks[t] >> lane) & 1)) continue;
+        const int64_t i = (int64_t)(b0 + t) * kTile + threadIdx.x;
+        int r = s_pre[t] + __popcll(masks[t] & below);
+        for (int ww = 0; ww < w; ++ww) r += s_cnt[t][ww];
+        s_sp[r] = gpos[i];
+        const double v = gv[i];
+        ew[col0 + r] = (fabs(v) > wmax) ? v / wmax : copysign(1.0, v);
+        a[0] += 1.0;
+        a[1] = (v < a[1]) ? v : a[1];
+        a[2] = (a[2] < v) ? v : a[2];
+        a[3] += v;
+        a[4] += v * v;
+        if (v != 0) a[5] += 1.0;
+    }
+    __syncthreads();
+    const int cnt = s_pre[kGenSub];
+    for (int j = threadIdx.x; j < cnt * 4 * NT; j += kTile) {
+        const int q = j / cnt, e = j - q * cnt;
+        ev[(int64_t)q * cap + col0 + e] = mom[(int64_t)q * nmax + s_sp[e]];
+    }
+    const double n = wave_sum(a[0]), s1 = wave_sum(a[3]), s2 = wave_sum(a[4]), nf = wave_sum(a[5]);
+    const double mn = wave_min(a[1]), mx = wave_max(a[2]);
     if (lane == 0) {
         double* q = s_part[w];
         q[0] = n; q[1] = mn; q[2] = mx; q[3] = s1; q[4] = s2; q[5] = nf; q[6] = 0.0;
@@ -700,28 +754,20 @@ __global__ __launch_bounds__(kTile) void k_gen_stats(int64_t used, const uint8_t
     }
 }
 
-// Accepted events among attempts [0, used) in attempt order: momenta of the
-// external legs, SoA ev[(i*4+k)][cap] from column base + rank, and the event
-// weight sign(v) max(1, |v| / wmax).
-__global__ __launch_bounds__(kTile) void k_gen_events(int NT, int64_t used, const uint8_t* __restrict__ flag, const int* __restrict__ blkoff,
-                                                      const uint32_t* __restrict__ gpos, const double* __restrict__ gv, double wmax,
-                                                      const double* __restrict__ mom, int64_t nmax, double* __restrict__ ev,
-                                                      double* __restrict__ ew, int64_t cap, int64_t base) {
-    __shared__ int s_cnt[kWaves];
-    const int64_t a = (int64_t)blockIdx.x * kTile + threadIdx.x;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const bool ok = a < used && flag[a];
-    const uint64_t mask = __ballot(ok);
-    if (lane == 0) s_cnt[w] = __popcll(mask);
+// Sum of the nb per-block counts into *total (one workgroup, fixed order).
+__global__ __launch_bounds__(1024) void k_count_total(const int* __restrict__ cnt, int64_t nb, int64_t* __restrict__ total) {
+    __shared__ int64_t s[1024 / 64];
+    int64_t v = 0;
+    for (int64_t i = threadIdx.x; i < nb; i += 1024) v += cnt[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (!ok) return;
-    int rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
-    for (int ww = 0; ww < w; ++ww) rank += s_cnt[ww];
-    const int64_t col = base + blkoff[blockIdx.x] + rank;
-    const uint32_t sp = gpos[a];
-    for (int i = 0; i < 4 * NT; ++i) ev[(int64_t)i * cap + col] = mom[(int64_t)i * nmax + sp];
-    const double v = gv[a];
-    ew[col] = (fabs(v) > wmax) ? v / wmax : copysign(1.0, v);
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int w = 0; w < 1024 / 64; ++w) t += s[w];
+        *total = t;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -806,6 +852,7 @@ struct chili_mc {
     DevBuf<uint8_t> d_acc;
     DevBuf<int> d_counts, d_offs, d_seg, d_tiles, d_seg2, d_tiles2, d_blkacc, d_kept, d_keptoff, d_grp2;
     DevBuf<int64_t> d_total, d_bsum;
+    int64_t* h_pin = nullptr;      // pinned host words for the per-sub-batch counts
     // learned-proposal path: z/r [fmax][d], r_j [C][fmax][d], a_j and log q_j
     // [C][fmax], f [fmax], sampling log-density [fmax], StatsData partials
     DevBuf<double> d_z, d_rall, d_abuf, d_lq, d_fbuf, d_lqs, d_lqs2, d_fpart, d_xw;
@@ -838,6 +885,7 @@ struct chili_mc {
     void* host_user = nullptr;
     DevBuf<double> d_fhost;                // callback values by sorted position
     ~chili_mc() {
+        if (h_pin) hipHostFree(h_pin);
         if (producer_ev) hipEventDestroy(producer_ev);
         for (auto& e : ev_live) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto& e : fev_live) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
@@ -1164,6 +1212,18 @@ void harvest_events(chili_mc* m) {
 // per-block flag counts stay in d_blkacc.
 void find_cutoff(chili_mc* m, const uint8_t* flags, int64_t n, int nblk, int64_t need, int64_t& kept, int64_t& used) {
     hipLaunchKernelGGL(k_acc_count, dim3(nblk), dim3(kTile), 0, m->stream, flags, n, m->d_blkacc.ptr);
+    // the total first (8 bytes to pinned memory): the block counts are only
+    // read when the cutoff lies in this sub-batch
+    m->d_total.resize(1);
+    if (!m->h_pin) HIP_CHECK(hipHostMalloc((void**)&m->h_pin, 8 * sizeof(int64_t), hipHostMallocDefault));
+    hipLaunchKernelGGL(k_count_total, dim3(1), dim3(1024), 0, m->stream, m->d_blkacc.ptr, (int64_t)nblk, m->d_total.ptr);
+    HIP_CHECK(hipMemcpyAsync(m->h_pin, m->d_total.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, m->stream));
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+    if (m->h_pin[0] < need) {
+        kept = m->h_pin[0];
+        used = n;
+        return;
+    }
     std::vector<int> blkacc(nblk);
     m->d_blkacc.download(blkacc.data(), nblk, m->stream);
     HIP_CHECK(hipStreamSynchronize(m->stream));
@@ -1316,13 +1376,14 @@ void gen_tail(chili_mc* m, const MCArgs& A, int64_t n, int nblk, int64_t need, d
     if (got + kept > cap) throw chili_error(CHILI_ERANGE, "event buffer too small");
     const int ublk = (int)((used + kTile - 1) / kTile);
     if (ublk == 0) return;
-    m->d_gpart.resize((size_t)ublk * kStatW);
-    hipLaunchKernelGGL(k_gen_stats, dim3(ublk), dim3(kTile), 0, m->stream, used, m->d_gflag.ptr, m->d_gv.ptr, m->d_gpart.ptr);
-    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, m->stream, (int64_t)ublk, m->d_gpart.ptr, m->d_tot.ptr);
+    const int oblk = (ublk + kGenSub - 1) / kGenSub;
+    m->d_gpart.resize((size_t)oblk * kStatW);
     m->d_gblkoff.resize(ublk);
     scan_int(m, m->d_blkacc.ptr, m->d_gblkoff.ptr, (int64_t)ublk);
-    hipLaunchKernelGGL(k_gen_events, dim3(ublk), dim3(kTile), 0, m->stream, m->NT, used, m->d_gflag.ptr, m->d_gblkoff.ptr, m->d_gpos.ptr,
-                       m->d_gv.ptr, wmax, m->d_mom.ptr, m->nmax, ev, ew, cap, got);
+    CHILI_DISPATCH_NT(m->NT, hipLaunchKernelGGL((k_gen_out<NT>), dim3(oblk), dim3(kTile), 0, m->stream, used, ublk, m->d_gflag.ptr,
+                                                m->d_gblkoff.ptr, m->d_gpos.ptr, m->d_gv.ptr, wmax, m->d_mom.ptr, m->nmax, ev, ew,
+                                                cap, got, m->d_gpart.ptr));
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, m->stream, (int64_t)oblk, m->d_gpart.ptr, m->d_tot.ptr);
 }
 
 // One sub-batch of attempts [a0, a0 + n): returns kept (accepted below the

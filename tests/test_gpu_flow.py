@@ -578,8 +578,9 @@ def test_flow_generate_vs_records(mode):
 def test_flow_generate_early_rejection_same_events():
     """With the exact early rejection the flow path weights in full only the
     points whose picked channel's term can pass the accept test; attempts,
-    StatsData, event momenta and weights equal the stage without it bit for
-    bit (specialised kernels in both, so only the rejection differs)."""
+    StatsData's counts, min and max, event momenta and weights equal the
+    stage without it bit for bit, the sums to rounding (specialised kernels in
+    both, so only the rejection differs)."""
     _, _, _, g = adapted_context("gg_4g")
     fl, _ = make_flow(g.ndims, 4, g.nchannels, 0.05, seed=37)
     uni = abi.philox_uniforms(SEED + 71, stream=6)
@@ -593,5 +594,8 @@ def test_flow_generate_early_rejection_same_events():
             out.append(flow_generate(g, fl, uni, n, wmax, 17))
         (s0, n0, e0, w0), (s1, n1, e1, w1) = out
         assert n0 == n1, f"attempts {n0} vs {n1}"
-        assert (s0.n, s0.min, s0.max, s0.sum, s0.sum2, s0.n_finite) == (s1.n, s1.min, s1.max, s1.sum, s1.sum2, s1.n_finite)
+        assert (s0.n, s0.min, s0.max, s0.n_finite) == (s1.n, s1.min, s1.max, s1.n_finite)
+        # the sums are grouped by sub-batch, whose sizes follow a running
+        # acceptance estimate that the previous call moved
+        assert_rel(np.array([s0.sum, s0.sum2]), np.array([s1.sum, s1.sum2]), 1e-13, what="sums")
         assert np.array_equal(e0, e1) and np.array_equal(w0, w1)

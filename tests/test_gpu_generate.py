@@ -112,8 +112,11 @@ def test_generate_errors():
 def test_generate_early_rejection_same_events(case, modes):
     """Exact early rejection (chili_mc_specialize bit 3): a passing point is
     weighted in full only when its picked channel's bound can pass the accept
-    test.  Attempts, StatsData and the events (momenta and weights) equal
-    the stage without it bit for bit, on two sub-batch sizes of the loop."""
+    test.  Attempts, the events (momenta and weights) and StatsData's counts,
+    min and max equal the stage without it bit for bit, on two sub-batch sizes
+    of the loop; the sums agree to rounding (the loop sizes its sub-batches
+    from a running acceptance estimate that the previous call moved, and the
+    sums are grouped by sub-batch)."""
     proc, ch, gpu, ora = setup(case)
     uni = abi.philox_uniforms(SEED + 1, stream=5)
     probe = gpu.points(uni, 1 << 20, 4000)
@@ -126,5 +129,7 @@ def test_generate_early_rejection_same_events(case, modes):
             out.append(generate(gpu, uni, n, wmax, a0=11))
         (s0, n0, e0, w0), (s1, n1, e1, w1) = out
         assert n0 == n1, f"attempts {n0} vs {n1}"
-        assert np.array_equal(stats_tuple(s0), stats_tuple(s1))
+        t0, t1 = stats_tuple(s0), stats_tuple(s1)
+        assert np.array_equal(t0[[0, 1, 2, 5]], t1[[0, 1, 2, 5]])
+        assert_rel(t0[3:5], t1[3:5], 1e-13, what=f"{case} sums")
         assert np.array_equal(e0, e1) and np.array_equal(w0, w1)
