@@ -41,9 +41,17 @@ __global__ __launch_bounds__(kTile) void k_pick(MCArgs A, int64_t n, uint16_t* _
     for (int c = threadIdx.x; c < A.C; c += blockDim.x) counts[(int64_t)c * nblk + blockIdx.x] = s_cnt[c];
 }
 
-// Attempts per sub-batch of the learned-proposal path (its buffers hold
-// C x attempts x d doubles twice: the r_j and the flows' working copy)
-constexpr int64_t kFlowSubMax = int64_t(1) << 23;
+// Attempts per sub-batch of the learned-proposal path: its buffers hold
+// C x attempts x d doubles twice (the r_j and the flows' working copy), kept
+// within 24 GiB, between 2^23 and 2^25 attempts
+int64_t flow_sub_max(int C, int d) {
+    const int64_t fit = (int64_t(24) << 30) / ((int64_t)C * d * 16);
+    return std::min<int64_t>(int64_t(1) << 25, std::max<int64_t>(int64_t(1) << 23, fit));
+}
+// Attempts per sub-batch of the plain path (iterate and the unweighting
+// stage): each sub-batch synchronises with the host a few times, so large
+// sub-batches keep the device busy; momenta [4 NT][attempts] stay well inside HBM
+constexpr int64_t kSubMax = int64_t(1) << 25;
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of an int array in three passes: sums of 1024-element blocks,
@@ -2366,7 +2374,7 @@ int chili_mc_iterate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64_
         int64_t att = 0;
         // the host callback path moves every attempt's momenta to the host:
         // smaller sub-batches
-        const int64_t cap = m->host_fn ? (int64_t(1) << 16) : (int64_t(1) << 23);
+        const int64_t cap = m->host_fn ? (int64_t(1) << 16) : kSubMax;
         iterate_loop(m, uni, a0, ncalls, cap, tot, att, stats,
                      [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
                          int64_t& used) {
@@ -2397,7 +2405,7 @@ int chili_mc_generate(chili_mc* m, const chili_uniforms* uni, uint64_t a0, int64
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0;
-        iterate_loop(m, uni, a0, nevents, int64_t(1) << 23, tot, att, stats,
+        iterate_loop(m, uni, a0, nevents, kSubMax, tot, att, stats,
                      [&](const double*, int64_t, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept, int64_t& used) {
                          run_subbatch_gen(m, uni->seed, uni->stream, ab, n, need, wmax, ev_dev, ew_dev, cap, got, kept, used);
                      },
@@ -2572,7 +2580,7 @@ int chili_mc_flow_iterate(chili_mc* m, chili_flow* fl, const chili_uniforms* uni
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0;
-        iterate_loop(m, uni, a0, ncalls, kFlowSubMax, tot, att, stats,
+        iterate_loop(m, uni, a0, ncalls, flow_sub_max(m->C, m->d), tot, att, stats,
                      [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
                          int64_t& used) {
                          run_subbatch_flow(m, fl, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, vals_dev, got, vals_cap,
@@ -2600,7 +2608,7 @@ int chili_mc_flow_generate(chili_mc* m, chili_flow* fl, const chili_uniforms* un
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0;
-        iterate_loop(m, uni, a0, nevents, kFlowSubMax, tot, att, stats,
+        iterate_loop(m, uni, a0, nevents, flow_sub_max(m->C, m->d), tot, att, stats,
                      [&](const double*, int64_t, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept, int64_t& used) {
                          const GenOut g{wmax, ev_dev, ew_dev, cap, got};
                          run_subbatch_flow(m, fl, nullptr, 0, ab, uni->seed, uni->stream, ab, n, need, nullptr, 0, 0,
@@ -2625,7 +2633,7 @@ int chili_mc_flow_data(chili_mc* m, chili_flow* fl, const chili_uniforms* uni, u
         select_device(m->device);
         std::vector<double> tot;
         int64_t att = 0, total_kept = 0;
-        iterate_loop(m, uni, a0, ncalls, kFlowSubMax, tot, att, stats,
+        iterate_loop(m, uni, a0, ncalls, flow_sub_max(m->C, m->d), tot, att, stats,
                      [&](const double* ud, int64_t ustride, uint64_t ab, int64_t n, int64_t need, int64_t got, int64_t& kept,
                          int64_t& used) {
                          run_subbatch_flow(m, fl, ud, ustride, a0, uni->seed, uni->stream, ab, n, need, nullptr, 0, 0,
