@@ -109,17 +109,21 @@ def adapt(gpu, W, seed, iters=None, ncalls=None):
 
 
 def measured_traffic(workload, variant, pts_launch):
-    """HBM bytes per k_main launch at this launch size, projected from the
+    """HBM bytes per k_main launch (variant "forward": per forward-kernel
+    launch, pts_launch attempts) at this launch size, projected from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/),
     scaled by the counters' reading of a copy kernel of known size; None if
     absent."""
-    kind = "split" if "split" in variant else ("specialized" if variant in ("specialized", "specialized-weight") else "generic")
-    path = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}_{kind}.json")
+    if variant == "forward":
+        path, key = os.path.join(ROOT, "profiles", f"k_forward_traffic_{workload}.json"), "bytes_per_attempt"
+    else:
+        kind = "split" if "split" in variant else ("specialized" if variant in ("specialized", "specialized-weight") else "generic")
+        path, key = os.path.join(ROOT, "profiles", f"k_main_traffic_{workload}_{kind}.json"), "bytes_per_point"
     if not os.path.exists(path):
         return None, None
     with open(path) as fh:
         t = json.load(fh)
-    return round(t["bytes_per_point"] * pts_launch, 1), t["bytes_per_point"]
+    return round(t[key] * pts_launch, 1), t[key]
 
 
 def fixture_f_main(workload):
@@ -594,7 +598,7 @@ def main():
             gpu.specialize(mode)
             plain["generated"]["early_rejection"] = dict(
                 seconds=round(dt2, 4), events_per_s=round(ne / dt2, 1), attempts=int(gatt2),
-                same_attempts=bool(gatt2 == gatt), same_sum=bool(gst2.sum == gst.sum),
+                same_attempts=bool(gatt2 == gatt), sum_rel_diff=abs(gst2.sum - gst.sum) / abs(gst.sum),
                 note="chili_mc_specialize bit 3: a passing point is weighted in full only when the bound "
                      "f / (alpha_c / ps_c / vegas_c) of its picked channel c can pass the accept test (the "
                      "multichannel weight never exceeds it): the same events")
@@ -738,12 +742,14 @@ def main():
                     g1 = gen["early_rejection"]
                     prod["generated"]["early_rejection"] = dict(
                         seconds=g1["seconds"], events_per_s=g1["events_per_s"], attempts=g1["attempts"],
-                        same_attempts=bool(g1["attempts"] == g0["attempts"]), same_sum=bool(g1["sum"] == g0["sum"]))
+                        same_attempts=bool(g1["attempts"] == g0["attempts"]), sum_rel_diff=abs(g1["sum"] - g0["sum"]) / abs(g0["sum"]))
             del ev, ew
 
         def gen_fn(stream, ne, ev_ptr, ew_ptr):
             return gpu.flow_generate(fl, abi.philox_uniforms(seed, stream=stream), ne, prod["w_max"], ev_ptr, ew_ptr, ne)
 
+        gen_fn.flow = fl
+        gen_fn.per_row = per_row / W["flow_layers"]
         return prod, scan[lam_best], roof, gen_fn
 
     flow_gen = None
@@ -785,6 +791,10 @@ def main():
     fn = legs[gen_leg]
     for s_ in range(args.warmup):
         fn(S_PROD + 1100 + s_, evp, ewp)
+    # the step's dominant kernel timed live with HIP events on the stream it
+    # runs on: the forward map (plain leg) or the coupling kernel (flow leg)
+    timer = gpu if gen_leg == "plain" else flow_gen.flow
+    timer.timing(enable=True, reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     gen_atts, gen_ms = 0, []
@@ -795,12 +805,13 @@ def main():
         gen_ms.append(round((time.perf_counter() - ts) * 1e3, 2))
     torch.cuda.synchronize()
     gen_dt = time.perf_counter() - t0
+    tm_gen = timer.timing(enable=False, reset=True)
     gen_value = ne_step * args.steps / gen_dt
     log(f"  [headline] {gen_leg}: {gen_value:.3e} unweighted events/s generated ({args.steps} passes of {ne_step} events; "
         f"selection {gen_sel})")
     del ev, ew
 
-    # ---- roofline of k_main ----
+    # ---- rooflines: the headline step's dominant kernel, and k_main ----
     ora = O.Oracle(proc, gpu.nbins, f, count=True)
     ora.set_grids(gpu.grids())
     ora.set_alpha(gpu.alpha())
@@ -824,6 +835,42 @@ def main():
                 flop_eq_fixture=fixture_f_main(args.workload),
                 points_per_launch=round(pts_launch, 1), avg_launch_ms=round(avg_ms, 4), launches=tm["launches"],
                 forward_kernel_ms_per_step=round(tm["forward_ms"] / (args.steps + args.warmup), 4))
+    if gen_leg == "plain":
+        # the specialised forward kernel: Philox uniforms, VEGAS map, channel
+        # mapping, cut, packed momenta of the passing attempts
+        f_fwd = (cnt["fwd_arith"] + 20 * cnt["fwd_trans"]) / max(cnt["attempts"], 1)
+        nl = max(tm_gen["forward_launches"], 1)
+        fwd_ms = tm_gen["forward_ms"] / nl
+        att_launch = tm_gen["forward_attempts"] / nl
+        ach = f_fwd * att_launch / (fwd_ms * 1e-3) / 1e12
+        ftraffic, fbpa = measured_traffic(args.workload, "forward", att_launch)
+        roof_head = dict(bound="fp64-valu", achieved=round(ach, 4), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                         frac=round(ach / FP64_PEAK_TFLOPS, 5), traffic=ftraffic,
+                         traffic_source=("projected: bytes per attempt from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE "
+                                         f"run (profiles/k_forward_traffic_{args.workload}.md, {fbpa} B/attempt) x this "
+                                         "run's attempts per launch" if ftraffic is not None else None),
+                         kernel="chili_k_forward (run-time specialised: Philox uniforms, VEGAS forward map, channel "
+                                "mapping, cut, packed momenta of the passing attempts)",
+                         compute="FP64 vector ALU (no matrix work); MI355X's FP64 vector peak 78.6 TFLOP/s",
+                         scope="the dominant kernel of the headline step (the plain leg's unweighting stage)",
+                         flop_eq_per_attempt=round(f_fwd, 1),
+                         flop_eq_count="N_arith + 20 N_trans of the forward map, op-counting oracle on 4000 attempts of "
+                                       "the frozen state (integer Philox work not counted)",
+                         attempts_per_launch=round(att_launch, 1), avg_launch_ms=round(fwd_ms, 4), launches=nl,
+                         share_of_step=round(tm_gen["forward_ms"] / (gen_dt * 1e3), 4))
+    else:
+        kc_ms = tm_gen["ms"] / max(tm_gen["launches"], 1)
+        kc_tf = tm_gen["flops"] / (tm_gen["ms"] * 1e-3) / 1e12
+        roof_head = dict(bound="mfma", achieved=round(kc_tf, 2), peak=BF16_PEAK_TFLOPS, unit="TFLOP/s",
+                         frac=round(kc_tf / BF16_PEAK_TFLOPS, 5), traffic=None,
+                         kernel="k_couple (fused coupling layer: BF16 MFMA conditioner MLP + FP32/FP64 spline)",
+                         scope="the dominant kernel of the headline step (the flow leg's unweighting stage)",
+                         launches=tm_gen["launches"], avg_launch_ms=round(kc_ms, 4),
+                         rows_per_launch=round(tm_gen["rows"] / max(tm_gen["launches"], 1), 1),
+                         flops_per_row_layer=round(flow_gen.per_row, 1),
+                         share_of_step=round(tm_gen["ms"] / (gen_dt * 1e3), 4),
+                         note="algorithmic conditioner flops of the timed launches over their HIP-event time; traffic "
+                              "not measured (MFMA/VALU bound)")
 
     # ---- short legs of the other workloads ----
     configs = {}
@@ -891,7 +938,8 @@ def main():
            if gen_leg == "flow" else
            dict(points_per_s=round(pts_s, 1), eps_unw=eps, eps_unw_q999=eps999, w_max=wmax, w_mean=wmean,
                 attempts_per_step=atts / args.steps, derived_unweighted_per_s=unw_s)),
-        "adaptation": ad, "roofline": roof, "roofline_flow": roof_flow, "cpu_baseline": cpu, "plain": plain, "flow": flow,
+        "adaptation": ad, "roofline": roof_head, "roofline_weight": roof, "roofline_flow": roof_flow, "cpu_baseline": cpu,
+        "plain": plain, "flow": flow,
         "configs": configs,
         # the CPU baseline runs the plain algorithm: its points/s against the plain leg's
         "gpu_vs_cpu": round(pts_s / cpu["points_per_s"], 1) if cpu else None,

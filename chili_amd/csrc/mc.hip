@@ -50,8 +50,10 @@ int64_t flow_sub_max(int C, int d) {
 }
 // Attempts per sub-batch of the plain path (iterate and the unweighting
 // stage): each sub-batch synchronises with the host a few times, so large
-// sub-batches keep the device busy; momenta [4 NT][attempts] stay well inside HBM
-constexpr int64_t kSubMax = int64_t(1) << 25;
+// sub-batches keep the device busy (2^25 + 2^22: one sub-batch holds a
+// 2^24-point iteration down to a cut acceptance of 0.45); momenta
+// [4 NT][attempts] stay well inside HBM
+constexpr int64_t kSubMax = (int64_t(1) << 25) + (int64_t(1) << 22);
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of an int array in three passes: sums of 1024-element blocks,
@@ -887,6 +889,8 @@ struct chili_mc {
     double main_ms = 0, fwd_ms = 0;
     int64_t main_launches = 0, main_points = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> fev_live;
+    std::vector<int64_t> fev_points;       // attempts of each timed forward launch
+    int64_t fwd_launches = 0, fwd_points = 0;
     hipEvent_t producer_ev = nullptr;      // chili_mc_evaluate: the caller's stream -> the context's
     // host integrand / PreProcess callback (the reference's std::function path)
     chili_host_integrand_fn host_fn = nullptr;
@@ -1202,15 +1206,18 @@ void harvest_events(chili_mc* m) {
         m->main_points += m->ev_points[i];
         m->ev_pool.push_back(m->ev_live[i]);
     }
-    for (auto& e : m->fev_live) {
+    for (size_t i = 0; i < m->fev_live.size(); ++i) {
         float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+        HIP_CHECK(hipEventElapsedTime(&ms, m->fev_live[i].first, m->fev_live[i].second));
         m->fwd_ms += ms;
-        m->ev_pool.push_back(e);
+        m->fwd_launches += 1;
+        m->fwd_points += m->fev_points[i];
+        m->ev_pool.push_back(m->fev_live[i]);
     }
     m->ev_live.clear();
     m->ev_points.clear();
     m->fev_live.clear();
+    m->fev_points.clear();
 }
 
 // Cutoff (the attempt at which `need` accepted attempts are reached; host
@@ -1297,6 +1304,7 @@ int pick_sort_forward(chili_mc* m, const MCArgs& A, int64_t n, const PointRec& R
     if (m->timing) {
         HIP_CHECK(hipEventRecord(fev.second, m->stream));
         m->fev_live.push_back(fev);
+        m->fev_points.push_back(n);
     }
     launch_ktcut(m, n, R);
     return tgrid;
@@ -1625,6 +1633,7 @@ void run_subbatch_flow(chili_mc* m, chili_flow* fl, const double* u, int64_t ust
     if (m->timing) {
         HIP_CHECK(hipEventRecord(fev.second, m->stream));
         m->fev_live.push_back(fev);
+        m->fev_points.push_back(n);
     }
     launch_ktcut(m, n, R);
     // own ranges need the [C][fmax] row layout within the int offsets of a pass
@@ -2503,8 +2512,21 @@ int chili_mc_timing(chili_mc* m, int enable, int reset, double* main_ms, int64_t
         if (launches) *launches = m->main_launches;
         if (points) *points = m->main_points;
         if (fwd_ms) *fwd_ms = m->fwd_ms;
-        if (reset) { m->main_ms = 0; m->fwd_ms = 0; m->main_launches = 0; m->main_points = 0; }
+        if (reset) { m->main_ms = 0; m->fwd_ms = 0; m->main_launches = 0; m->main_points = 0; m->fwd_launches = 0; m->fwd_points = 0; }
         m->timing = enable != 0;
+    });
+}
+
+// Launches and attempts of the forward kernel timed since the last reset of
+// chili_mc_timing (fwd_ms covers them).
+int chili_mc_timing_forward(chili_mc* m, int64_t* launches, int64_t* attempts) {
+    return guarded([&] {
+        if (!m) throw chili_error(CHILI_EINVAL, "null context");
+        select_device(m->device);
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+        harvest_events(m);
+        if (launches) *launches = m->fwd_launches;
+        if (attempts) *attempts = m->fwd_points;
     });
 }
 

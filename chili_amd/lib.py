@@ -18,7 +18,7 @@ EXPORTS = [
     "chili_last_error", "chili_device_count", "chili_philox_uniform",
     "chili_construct_channels", "chili_channels_count", "chili_channels_ndims", "chili_channels_get", "chili_channels_free",
     "chili_mc_create", "chili_mc_destroy", "chili_mc_info", "chili_mc_set_grids", "chili_mc_get_grids",
-    "chili_mc_set_alpha", "chili_mc_get_alpha", "chili_mc_iterate", "chili_mc_points", "chili_mc_timing",
+    "chili_mc_set_alpha", "chili_mc_get_alpha", "chili_mc_iterate", "chili_mc_points", "chili_mc_timing", "chili_mc_timing_forward",
     "chili_vegas_create", "chili_vegas_destroy", "chili_vegas_set_grid", "chili_vegas_get_grid", "chili_vegas_iterate",
     "chili_vegas_iterate_static", "chili_mc_set_host_integrand", "chili_channels_match", "chili_mc_state_read",
     "chili_mc_state_write", "chili_mc_optimize", "chili_vegas_optimize", "chili_mc_params_default",
@@ -78,6 +78,7 @@ def load():
                                    C.POINTER(C.c_int64), C.c_void_p, I64]
     L.chili_mc_points.argtypes = [P, C.POINTER(abi.Uniforms), U64, I64, I32, I32, D, D, D, D, D, I32, D]
     L.chili_mc_timing.argtypes = [P, C.c_int, C.c_int, D, C.POINTER(C.c_int64), C.POINTER(C.c_int64), D]
+    L.chili_mc_timing_forward.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.chili_vegas_create.argtypes = [C.c_int, C.c_int, C.POINTER(abi.Integrand), C.c_int, C.POINTER(P)]
     L.chili_vegas_destroy.argtypes = [P]
     L.chili_vegas_set_grid.argtypes = [P, D, C.c_int]
@@ -335,11 +336,15 @@ class MultiChannelGPU:
         return t.value
 
     def timing(self, enable=True, reset=False):
+        """Weight and forward kernel times since the last reset (chili_mc_timing,
+        chili_mc_timing_forward), then enable / disable and optionally reset."""
         ms, fms = C.c_double(), C.c_double()
-        nl, npts = C.c_int64(), C.c_int64()
+        nl, npts, fl, fpts = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(load().chili_mc_timing_forward(self.h, C.byref(fl), C.byref(fpts)))
         check(load().chili_mc_timing(self.h, 1 if enable else 0, 1 if reset else 0, C.byref(ms), C.byref(nl), C.byref(npts),
                                      C.byref(fms)))
-        return dict(main_ms=ms.value, launches=nl.value, points=npts.value, forward_ms=fms.value)
+        return dict(main_ms=ms.value, launches=nl.value, points=npts.value, forward_ms=fms.value,
+                    forward_launches=fl.value, forward_attempts=fpts.value)
 
     def points(self, uni, a0, n):
         d, c, nt = self.ndims, self.nchannels, self.ntot

@@ -252,6 +252,9 @@ int chili_channels_compile_check(const chili_channels* ch, double* compile_s);
 /* Kernel timing on the context's stream (HIP events around the weight and
  * forward kernels); no reference counterpart (measurement only). */
 int chili_mc_timing(chili_mc* m, int enable, int reset, double* main_ms, int64_t* launches, int64_t* points, double* fwd_ms);
+/* Launches and attempts of the forward kernel that fwd_ms covers (since the
+ * last reset); no reference counterpart (measurement only). */
+int chili_mc_timing_forward(chili_mc* m, int64_t* launches, int64_t* attempts);
 
 /* ---- batched jobs: many independent MultiChannel::operator() iterations
  *      (include/Channel/MultiChannel.hh:196-251), each job a chili_mc context

@@ -133,3 +133,22 @@ def test_generate_early_rejection_same_events(case, modes):
         assert np.array_equal(t0[[0, 1, 2, 5]], t1[[0, 1, 2, 5]])
         assert_rel(t0[3:5], t1[3:5], 1e-13, what=f"{case} sums")
         assert np.array_equal(e0, e1) and np.array_equal(w0, w1)
+
+
+def test_generate_forward_timing():
+    """chili_mc_timing / chili_mc_timing_forward over the unweighting stage:
+    one timed forward launch per sub-batch, covering every attempt the loop
+    consumed (the last sub-batch's attempts past the cutoff included)."""
+    proc, ch, gpu, ora = setup("gg_4g")
+    uni = abi.philox_uniforms(SEED + 2, stream=9)
+    probe = gpu.points(uni, 1 << 20, 4000)
+    wmax = float(np.quantile(np.abs(probe["val"][probe["status"] == 1]), 0.99))
+    gpu.timing(enable=True, reset=True)
+    st, na, ev, ew = generate(gpu, uni, 5000, wmax)
+    tm = gpu.timing(enable=False, reset=True)
+    assert tm["forward_launches"] >= 1 and tm["forward_ms"] > 0
+    assert tm["forward_attempts"] >= na
+    assert tm["forward_attempts"] < na + 2 * tm["forward_attempts"] / tm["forward_launches"] + 1
+    # reset: nothing left to report
+    tm2 = gpu.timing(enable=False, reset=False)
+    assert tm2["forward_launches"] == 0 and tm2["forward_attempts"] == 0 and tm2["forward_ms"] == 0
